@@ -1,0 +1,176 @@
+/*
+ * timeevolver.h — C-ABI of the MI355X (gfx950) Krylov time-evolution library
+ * (libtimeevolver_hip.so). This is the drop-in boundary for TimeEvolver's hot path:
+ * the reference's C++ API (core/krylovTimeEvolver.h, core/matrixDataTypes.h,
+ * core/krylovObservables.h, helper/Basis.h, helper/hamiltonian.h) is re-implemented by a
+ * thin C++ layer on top of these entry points, and any FFI (ctypes, cgo, JNI) can bind
+ * them directly. Plain pointers and sizes only; complex numbers are interleaved
+ * (re, im) doubles exactly as std::complex<double> lays them out.
+ *
+ * Error convention: every int-returning function returns TE_OK (0) or a TE_ERR_* code
+ * and records a message retrievable with te_last_error(ctx). Where the reference logs
+ * FATAL and calls exit(1), the library returns TE_ERR_FATAL instead; the C++ layer turns
+ * that back into exit(1). No C++ exception crosses this boundary.
+ *
+ * Threading: a context owns one HIP stream and is thread-compatible, not thread-safe.
+ * All computation runs on the device of the context; host pointers are copied in/out.
+ */
+#ifndef TIMEEVOLVER_H
+#define TIMEEVOLVER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TE_OK 0
+#define TE_ERR_ARG 1          /* invalid argument (size, null pointer, dimension mismatch) */
+#define TE_ERR_HIP 2          /* a HIP runtime call failed */
+#define TE_ERR_ALLOC 3        /* device or pinned-host allocation failed */
+#define TE_ERR_FATAL 4        /* the reference would log FATAL and exit(1) */
+#define TE_ERR_UNSUPPORTED 5  /* outside what this build supports (e.g. nnz >= 2^31) */
+#define TE_ERR_STOP 6         /* a host observable requested a stop (requestStopException) */
+
+typedef struct te_ctx te_ctx;
+typedef struct te_matrix te_matrix;
+typedef struct te_basis te_basis;
+typedef struct te_evolver te_evolver;
+
+/* ---- context ---------------------------------------------------------------------- */
+/* No reference counterpart (the reference has no device); owns the HIP stream. */
+int te_ctx_create(int device, te_ctx** out);
+void te_ctx_destroy(te_ctx* ctx);
+const char* te_last_error(const te_ctx* ctx);
+int te_ctx_synchronize(te_ctx* ctx);
+/* Library version: TIMEEVOLVER_VERSION of core/version.h:7 (201). */
+int te_version(void);
+
+/* ---- sparse matrix: replaces TE::smatrix (core/matrixDataTypes.h:59-99) ------------- */
+/* smatrix(val, col, row, nbV, nn, mm) + initialize() (core/matrixDataTypes.cpp:108-133,
+ * :196-240): zero-based COO with size_t indices (duplicates are kept and summed by the
+ * product, as MKL does), converted to device CSR with int32 indices. values: 2*nnz doubles. */
+int te_matrix_from_coo(te_ctx* ctx, const double* values, const size_t* col, const size_t* row,
+                       size_t nnz, size_t n_rows, size_t n_cols, te_matrix** out);
+void te_matrix_destroy(te_matrix* A);
+int te_matrix_info(const te_matrix* A, size_t* n_rows, size_t* n_cols, size_t* nnz);
+/* smatrix::norm1 (core/matrixDataTypes.cpp:67-79): maximal absolute column sum. */
+int te_matrix_norm1(te_matrix* A, double* out);
+/* Copies the device CSR back as COO (rows ascending, columns ascending within a row):
+ * the public values/columns/rowIndex arrays of smatrix. */
+int te_matrix_to_coo(te_matrix* A, double* values, size_t* col, size_t* row);
+/* The device CSR as it is held (the CSR that mkl_sparse_convert_csr builds in the reference,
+ * core/matrixDataTypes.cpp:220-221): row_ptr n_rows+1 entries, col and values nnz entries.
+ * Any pointer may be NULL to skip that array. */
+int te_matrix_to_csr(te_matrix* A, double* values, int64_t* row_ptr, int* col);
+/* smatrix::spMV (core/matrixDataTypes.cpp:169-190): out = alpha * A * in, host pointers. */
+int te_spmv(te_matrix* A, double alpha_re, double alpha_im, const double* in, double* out);
+/* Same on device pointers (n_cols and n_rows complex128 each), asynchronous on the
+ * context's stream. */
+int te_spmv_device(te_matrix* A, double alpha_re, double alpha_im, const void* in_dev, void* out_dev);
+
+/* ---- basis: replaces basis / tensorBasis (helper/Basis.h:277-357) -------------------- */
+/* basis(N, K, nbQudits, capacity, quant) (helper/Basis.cpp:37-120), built on the device. */
+int te_basis_create(te_ctx* ctx, int N, int K, int nbQudits, int capacity, int quant, te_basis** out);
+/* tensorBasis(N1, K1, N2, K2, C2) (helper/Basis.cpp:131-182). */
+int te_tensor_basis_create(te_ctx* ctx, int N1, int K1, int N2, int K2, int C2, te_basis** out);
+void te_basis_destroy(te_basis* b);
+/* numberElements and numberModes (helper/Basis.h:280-282). */
+int te_basis_info(const te_basis* b, size_t* n_elements, int* n_modes);
+/* basisElements (helper/Basis.h:283): n_elements x n_modes ints, row-major. */
+int te_basis_states(const te_basis* b, int* occ);
+/* hashTable.find (helper/Basis.h:284): index of each of `count` occupation vectors
+ * (count x n_modes ints), or -1 when absent. */
+int te_basis_find(const te_basis* b, const int* occ, size_t count, int64_t* index_out);
+
+/* ---- operator builder: replaces Hamiltonian::createMatrix (helper/hamiltonian.cpp:318-401) */
+/* One opTerm (helper/hamiltonian.h:18-35): coef * product of operators applied in list
+ * order; operations[i] = +1 creation, -1 annihilation on modes[i], capacities[i] = 0 for
+ * no maximal occupation. */
+typedef struct {
+    double coef_re, coef_im;
+    int n_ops;
+    const int* operations;
+    const int* modes;
+    const int* capacities;
+} te_op_term;
+/* createMatrix(terms, basis): bit-identical CSR (rows ascending, columns ascending, entries
+ * with |v| < DBL_EPSILON dropped), computed on the device. */
+int te_operator_build(te_ctx* ctx, const te_basis* b, const te_op_term* terms, int n_terms, te_matrix** out);
+
+/* ---- time evolution: replaces krylovTimeEvolver (core/krylovTimeEvolver.h:56-133) -------- */
+typedef struct {
+    double t;              /* total time (:26 t) */
+    double sampling_step;  /* :26 samplingStep */
+    double exp_factor;     /* :26 expFactor (default 1.0): evolves exp(-i expFactor H t) */
+    double tol;            /* :26 tol (default 1e-6) */
+    int m;                 /* :26 mm (default 40; capped at 64 and at the dimension) */
+    int fast_integration;  /* :26 fastIntegration (default 0) */
+} te_evolve_params;
+
+#define TE_OBS_SPARSE 0    /* krylovSpMatrixObservable: Re<psi|O|psi> (core/krylovObservables.cpp:173-194) */
+#define TE_OBS_VECTOR 1    /* krylovVectorObservable: |<psi|o>|^2 (core/krylovObservables.cpp:216-236) */
+#define TE_OBS_HOST 2      /* any other krylovBasicObservable::expectation, called on the host */
+
+/* Host observable callback: state is n complex128 (host memory, valid during the call).
+ * Return 0 and set *value, or nonzero to request a stop (requestStopException,
+ * core/krylovObservables.h:16-20). */
+typedef int (*te_obs_callback)(void* user, const double* state, size_t n, double* value);
+
+typedef struct {
+    int kind;                /* TE_OBS_* */
+    te_matrix* matrix;       /* TE_OBS_SPARSE */
+    const double* vector;    /* TE_OBS_VECTOR: n complex128, copied at evolver creation */
+    te_obs_callback callback;/* TE_OBS_HOST */
+    void* user;
+} te_observable;
+
+/* krylovReturn (core/krylovTimeEvolver.h:32-51) minus the moved-out objects. */
+typedef struct {
+    double err;
+    double evolved_time;
+    size_t num_samples;
+    size_t n_steps;
+    size_t dim;
+    size_t krylov_dim;
+    int status_code;
+} te_result;
+
+/* Constructor (core/krylovTimeEvolver.cpp:26-84): validates (dimension > 0, |‖psi0‖-1| <= tol,
+ * else TE_ERR_FATAL), computes norm1, m = min(m, dim), n_samples = floor(t/step)+1 and uploads
+ * psi0. The matrix and observables are borrowed and must outlive the evolver. */
+int te_evolver_create(te_ctx* ctx, te_matrix* H, const double* psi0, const te_evolve_params* params,
+                      const te_observable* obs, int n_obs, te_evolver** out);
+/* Same, with psi0 already in device memory (n complex128). */
+int te_evolver_create_device(te_ctx* ctx, te_matrix* H, const void* psi0_dev, const te_evolve_params* params,
+                             const te_observable* obs, int n_obs, te_evolver** out);
+void te_evolver_destroy(te_evolver* ev);
+/* Number of samples per observable (n_samples of :57). */
+int te_evolver_num_samples(const te_evolver* ev, size_t* n_samples);
+/* timeEvolve() (core/krylovTimeEvolver.cpp:261-512), from psi0 each time it is called.
+ * obs_values: n_obs x n_samples doubles (may be NULL); evolved_state: 2*dim doubles, the last
+ * sampled state (krylovReturn::evolvedState, :665; may be NULL); result may not be NULL.
+ * Returns TE_OK also for soft failures reported through result->status_code. */
+int te_evolver_run(te_evolver* ev, double* obs_values, double* evolved_state, te_result* result);
+/* Device pointer to the last sampled state (valid until the next run/destroy). */
+int te_evolver_state_device(te_evolver* ev, const void** state_dev);
+/* The Krylov step sizes of the last run (up to cap entries; returns the count in *n). */
+int te_evolver_step_trace(te_evolver* ev, double* t_steps, size_t cap, size_t* n);
+
+/* Kernel timing of te_evolver_run, for measurement harnesses (no reference counterpart):
+ * HIP events recorded around the first 4096 launches of each kind on the context's stream.
+ * kind: 0 Lanczos apply (fused SpMV, the dominant kernel), 1 Lanczos project,
+ * 2 step controller, 3 combine (state update, samples, observables). */
+int te_evolver_set_profiling(te_evolver* ev, int enable);
+int te_evolver_kernel_time(const te_evolver* ev, int kind, double* total_ms, size_t* launches);
+
+/* Log level filter (krylovLogger::loggingLevel, core/krylovLogger.h:17: DEBUG, INFO, WARNING,
+ * ERROR, FATAL = 0..4; default 0 prints everything, as the unfiltered Boost.Log core does).
+ * Messages go to stderr. */
+void te_set_log_level(int level);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TIMEEVOLVER_H */

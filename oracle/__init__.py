@@ -70,6 +70,7 @@ def lib():
             "or_tanh_sinh_poly": (_d, [_i, _dp, _d, _d, _d, _dp, _dp]),
             "or_gauss_poly": (_d, [_i, _i, _dp, _d, _d]),
             "or_tanh_sinh_rows": (_i, [_i]),
+            "or_evolve_csr": (_i, [_l, _lp, _ip, _dp, _dp, _d, _d, _d, _d, _i, _i, _l, _i, _dp, _dp, _dp, _dp]),
             "or_evolve": (_i, [_l, _l, _dp, _lp, _lp, _dp, _d, _d, _d, _d, _i, _i, _l, _l, _i, _ip, _lp,
                                _dp, _lp, _lp, _dp, _dp, _dp, _dp, _dp, _l]),
         }
@@ -305,3 +306,30 @@ def evolve(n, H, psi0, t, samplingStep, observables=(), expFactor=1.0, tol=1e-6,
         "n_samples": int(result[7]),
         "trace": trace[:min(n_steps, trace_cap)].copy(),
     }
+
+
+def evolve_csr(ptr, col, vals, psi0, t, samplingStep, diags=(), expFactor=1.0, tol=1e-6, m=40,
+               fastIntegration=False, max_steps=0, want_state=False):
+    """timeEvolve on H given as CSR with diagonal observables (the CPU-baseline entry)."""
+    n = len(ptr) - 1
+    ptr = np.ascontiguousarray(ptr, dtype=np.int64)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    vals = np.ascontiguousarray(vals, dtype=np.complex128).view(np.float64)
+    psi = np.ascontiguousarray(psi0, dtype=np.complex128)
+    d = np.ascontiguousarray(np.array(diags, dtype=np.float64).reshape(len(diags), n) if len(diags)
+                             else np.zeros((0, n)))
+    n_samples = int(np.floor(t / samplingStep)) + 1
+    obs_out = np.zeros(max(len(diags), 1) * n_samples)
+    state = np.zeros(n, dtype=np.complex128) if want_state else None
+    result = np.zeros(8)
+    rc = lib().or_evolve_csr(n, _ptr(ptr, ctypes.c_long), _ptr(col, ctypes.c_int), _ptr(vals, ctypes.c_double),
+                             _ptr(psi.view(np.float64), ctypes.c_double), t, samplingStep, expFactor, tol, m,
+                             int(fastIntegration), max_steps, len(diags),
+                             _ptr(d, ctypes.c_double) if len(diags) else None, _ptr(obs_out, ctypes.c_double),
+                             _ptr(state.view(np.float64), ctypes.c_double) if want_state else None,
+                             _ptr(result, ctypes.c_double))
+    if rc != 0:
+        raise RuntimeError("oracle evolution hit a fatal condition")
+    return {"err": result[0], "evolvedTime": result[1], "numSamples": int(result[2]), "n_steps": int(result[3]),
+            "statusCode": int(result[6]), "state": state,
+            "observables": obs_out[:len(diags) * n_samples].reshape(len(diags), n_samples)}

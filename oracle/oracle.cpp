@@ -977,6 +977,18 @@ double or_gauss_poly(int n, int deg, const double* c, double a, double b) {
 }
 int or_tanh_sinh_rows(int level) { return (int)tanh_sinh().rows[level].size(); }
 
+// Diagonal observables given as dense vectors of their diagonal (CPU-baseline entry point).
+static Csr diag_csr(long n, const double* d) {
+    Csr A;
+    A.n = A.ncols = n;
+    A.ptr.resize(n + 1);
+    A.col.resize(n);
+    A.val.resize(n);
+    for (long r = 0; r <= n; r++) A.ptr[r] = r;
+    for (long r = 0; r < n; r++) { A.col[r] = r; A.val[r] = cd(d[r], 0.0); }
+    return A;
+}
+
 // Full time evolution. H and sparse observables given as COO (int64 indices).
 // obs_kind[i]: 0 sparse (obs_nnz[i] entries taken in turn from obs_vals/cols/rows),
 // 1 vector (D complex entries taken in turn from obs_vecs).
@@ -1027,6 +1039,56 @@ int or_evolve(long n, long nnz, const double* hv, const long* hc, const long* hr
     result[6] = (double)R.statusCode; result[7] = (double)ev.n_samples;
     if (trace)
         for (size_t i = 0; i < ev.step_trace.size() && (long)i < trace_cap; i++) trace[i] = ev.step_trace[i];
+    return 0;
+}
+
+// Time evolution with H given as CSR (int64 row pointers, int32 columns) and diagonal
+// observables as dense diagonals (n_obs x n): the CPU baseline on a large workload, where a
+// COO sort would dominate. max_steps bounds the number of Krylov steps (0 = all).
+int or_evolve_csr(long n, const long* ptr, const int* col, const double* vals, const double* psi0, double t,
+                  double samplingStep, double expFactor, double tol, int m, int fastIntegration, long max_steps,
+                  int n_obs, const double* diags, double* obs_out, double* state_out, double* result) {
+    Csr H;
+    H.n = H.ncols = n;
+    long nnz = ptr[n];
+    H.ptr.assign(ptr, ptr + n + 1);
+    H.col.resize(nnz);
+    H.val.resize(nnz);
+#pragma omp parallel for schedule(static)
+    for (long k = 0; k < nnz; k++) {
+        H.col[k] = col[k];
+        H.val[k] = cd(vals[2 * k], vals[2 * k + 1]);
+    }
+    std::vector<double> colVal(n, 0.0);
+    for (long k = 0; k < nnz; k++) colVal[H.col[k]] += std::abs(H.val[k]);
+    Evolver ev;
+    ev.P = Params{t, samplingStep, expFactor, tol, m, fastIntegration, max_steps, -1};
+    ev.H = &H;
+    ev.Hsize = n;
+    ev.matrixNorm = *std::max_element(colVal.begin(), colVal.end());
+    ev.m = (int)std::min<long>(m, n);
+    ev.expFactor = cd(0.0, -1.0 * expFactor);
+    if (std::abs(dznrm2(n, (const cd*)psi0) - 1.0) > tol) return -1;
+    ev.n_samples = (long)std::floor(t / samplingStep) + 1;
+    ev.currentVec.assign((const cd*)psi0, (const cd*)psi0 + n);
+    ev.sampledState = ev.currentVec;
+    ev.tmp.assign(n, cd(0.0, 0.0));
+    std::vector<Observable> obs(n_obs);
+    for (int i = 0; i < n_obs; i++) {
+        obs[i].kind = 0;
+        obs[i].A = diag_csr(n, diags + (size_t)i * n);
+        obs[i].values.assign(ev.n_samples, 0.0);
+    }
+    ev.obs = &obs;
+    Result R;
+    int rc = ev.run(R);
+    if (rc != 0) return rc;
+    for (int i = 0; i < n_obs; i++)
+        for (long s = 0; s < ev.n_samples; s++) obs_out[i * ev.n_samples + s] = obs[i].values[s];
+    if (state_out) std::memcpy(state_out, ev.sampledState.data(), sizeof(cd) * n);
+    result[0] = R.err; result[1] = R.evolvedTime; result[2] = (double)R.numSamples;
+    result[3] = (double)R.n_steps; result[4] = (double)R.dim; result[5] = (double)R.krylovDim;
+    result[6] = (double)R.statusCode; result[7] = (double)ev.n_samples;
     return 0;
 }
 

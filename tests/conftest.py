@@ -21,3 +21,12 @@ def oracle_lib():
     import oracle
     oracle.build()
     return oracle
+
+
+@pytest.fixture(scope="session")
+def gpu():
+    """C-ABI context on device 0 (the HIP library must be built: no CPU fallback)."""
+    from timeevolver_amd import capi
+    ctx = capi.Context(0)
+    yield ctx
+    ctx.close()

@@ -1,0 +1,250 @@
+"""GPU parity of the HIP path (through the C-ABI) against the oracle and the reference goldens.
+
+Bars: integer/index work (basis, CSR structure) and the builder's values bit-exact; SpMV
+within 1e-14 relative (summation order differs); time evolution within the rigorous error
+bound: ||psi_gpu - psi_oracle|| <= err_gpu + err_oracle + 1e-10, observables within
+2 ||O|| (err_gpu + err_oracle) + 1e-10, and the reference ctests' own 1e-8 on the goldens.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from models import blackhole, number_operator_terms, two_oscillators
+from timeevolver_amd import capi
+from timeevolver_amd.hamiltonian import as_tuples, createNumberOperator
+from timeevolver_amd.workloads import (fock_state, random_state, ring_bose_hubbard_terms, ring_couplings,
+                                       ring_dimension, ring_nnz)
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _coo_equal(gpu_m, ref):
+    v, c, r = gpu_m.to_coo()
+    rv, rc, rr = ref
+    assert len(v) == len(rv)
+    assert np.array_equal(c, rc) and np.array_equal(r, rr)
+    assert np.array_equal(v.view(np.uint64), np.ascontiguousarray(rv).view(np.uint64))   # bitwise
+
+
+# ---------------------------------------------------------------------------------------- basis
+@pytest.mark.parametrize("args", [(2, 3, 0, 0, 1), (8, 8, 0, 0, 1), (6, 4, 2, 2, 1), (5, 4, 4, 2, 1), (7, 3, 1, 3, 2),
+                                  (3, 1, 0, 0, 1), (0, 4, 0, 0, 1), (2, 8, 8, 1, 1)])
+def test_basis_bitexact(gpu, oracle_lib, args):
+    g = capi.Basis.basis(gpu, *args)
+    o = oracle_lib.Basis.basis(*args)
+    assert g.D == o.D
+    so = o.states()
+    assert np.array_equal(g.states(), so)
+    idx = g.find(so)
+    assert np.array_equal(idx, np.arange(o.D))
+    # absent states: one particle too many, a negative entry
+    bad = so.copy()
+    bad[:, 0] += 1
+    assert (g.find(bad) == -1).all()
+
+
+def test_tensor_basis_bitexact(gpu, oracle_lib):
+    for args in [(20, 2, 2, 8, 1), (1, 2, 1, 2, 1), (3, 2, 2, 4, 1)]:
+        g = capi.Basis.tensor(gpu, *args)
+        o = oracle_lib.Basis.tensor(*args)
+        assert g.D == o.D
+        s = o.states()
+        assert np.array_equal(g.states(), s)
+        assert np.array_equal(g.find(s), np.arange(o.D))
+
+
+# ---------------------------------------------------------------------------------------- operators
+def _ring_terms(L, seed=11, U=0.5):
+    eps, J = ring_couplings(L, seed)
+    return ring_bose_hubbard_terms(L, eps, J, U)
+
+
+@pytest.mark.parametrize("L,N", [(3, 3), (6, 6), (8, 8), (10, 10)])
+def test_ring_operator_bitexact(gpu, oracle_lib, L, N):
+    terms = as_tuples(_ring_terms(L))
+    gb = capi.Basis.basis(gpu, N, L)
+    ob = oracle_lib.Basis.basis(N, L)
+    gm = capi.operator_build(gpu, gb, terms)
+    ref = oracle_lib.build_operator(ob, terms)
+    assert gm.nnz == ring_nnz(L, N) and gb.D == ring_dimension(L, N)
+    _coo_equal(gm, ref)
+    assert gm.norm1() == oracle_lib.norm1(ob.D, ref[0], ref[1])      # exact COO-order column sums
+
+
+def test_blackhole_operators_bitexact(gpu, oracle_lib):
+    args, ham, init = blackhole()
+    gb = capi.Basis.tensor(gpu, *args)
+    ob = oracle_lib.Basis.tensor(*args)
+    _coo_equal(capi.operator_build(gpu, gb, ham), oracle_lib.build_operator(ob, as_tuples(ham)))
+    for t in number_operator_terms(ob.K):
+        _coo_equal(capi.operator_build(gpu, gb, t), oracle_lib.build_operator(ob, as_tuples(t)))
+
+
+def test_general_terms_bitexact(gpu, oracle_lib):
+    from timeevolver_amd.hamiltonian import createFourPointA, createFourPointC, createSixPoint, linInteraction, OpTerm
+    terms = [createNumberOperator(0, 0.3), linInteraction(0, 3, 0, 2, False, -0.7),
+             linInteraction(0, 3, 0, 2, True, -0.7), createFourPointA(0, 1, 2, 3, 0, 0, 2, 2, 0.11),
+             createFourPointC(1, 0, 1, 0, 0, 0, 0, 0, 0.05), createSixPoint(0, 1, 2, 3, 0, 1, 0, 0, 2, 2, 0, 0, 0.2),
+             OpTerm(complex(0.25, -0.5), [-1, 1], [1, 2], [0, 2])]
+    for args in [(4, 4, 2, 2, 1), (6, 4, 0, 0, 2)]:
+        gb = capi.Basis.basis(gpu, *args)
+        ob = oracle_lib.Basis.basis(*args)
+        _coo_equal(capi.operator_build(gpu, gb, terms), oracle_lib.build_operator(ob, as_tuples(terms)))
+
+
+# ---------------------------------------------------------------------------------------- spmv
+def test_spmv_vs_oracle(gpu, oracle_lib):
+    L = N = 8
+    terms = as_tuples(_ring_terms(L))
+    ob = oracle_lib.Basis.basis(N, L)
+    v, c, r = oracle_lib.build_operator(ob, terms)
+    A = capi.Matrix.from_coo(gpu, v, c, r, ob.D)
+    x = random_state(ob.D, 5)
+    for alpha in (1.0, -0.3j, 0.2 - 0.7j):
+        y = A.spmv(x, alpha)
+        yr = oracle_lib.spmv(ob.D, v, c, r, x, alpha)
+        assert np.linalg.norm(y - yr) <= 1e-14 * np.linalg.norm(yr)
+
+
+# ---------------------------------------------------------------------------------------- evolution
+def _gpu_model(gpu, basis_kind, args, terms, obs_terms):
+    b = capi.Basis.tensor(gpu, *args) if basis_kind == "tensor" else capi.Basis.basis(gpu, *args)
+    H = capi.operator_build(gpu, b, terms)
+    obs = [capi.operator_build(gpu, b, t) for t in obs_terms]
+    return b, H, obs
+
+
+def _bh_gpu(gpu, N0, Nm, K, maxT, step):
+    args, ham, init = blackhole(N0=N0, Nm=Nm, K=K)
+    b, H, obs = _gpu_model(gpu, "tensor", args, ham, number_operator_terms(args[1] + args[3]))
+    psi = np.zeros(b.D, complex)
+    psi[b.find(init)[0]] = 1
+    ev = capi.Evolver(gpu, H, psi, maxT, step, obs, tol=1e-8, m=40)
+    return ev.run()
+
+
+def test_blackhole_golden_gpu(gpu):
+    r = _bh_gpu(gpu, 20, 2, 4, 10.0, 0.01)
+    g = np.loadtxt(os.path.join(GOLD, "blackhole_N20_Nm2_K4.txt"))
+    print("black hole: n_steps", r["n_steps"], "err", r["err"], "max |diff|", np.abs(r["observables"] - g).max())
+    assert np.abs(r["observables"] - g).max() <= 1e-8
+    assert r["observables"][0, 0] == 20.000000000000004
+    assert r["numSamples"] == 1001 and r["err"] <= 1e-8
+
+
+def test_lucky_breakdown_golden_gpu(gpu):
+    r = _bh_gpu(gpu, 1, 1, 1, 1000.0, 1.0)
+    g = np.loadtxt(os.path.join(GOLD, "blackhole_N1_Nm1_K1.txt"))
+    assert np.abs(r["observables"] - g).max() <= 1e-8
+    assert r["statusCode"] == 1 and r["krylovDim"] <= 4 and r["n_steps"] == 1
+
+
+def test_simple_example_csv_gpu(gpu):
+    args, terms, init = two_oscillators()
+    b, H, obs = _gpu_model(gpu, "basis", args, terms, number_operator_terms(2))
+    psi = np.zeros(b.D, complex)
+    psi[b.find(init)[0]] = 1
+    r = capi.Evolver(gpu, H, psi, 10.0, 0.01, obs).run()     # default constructor: tol 1e-6, m 40
+    for k in range(2):
+        txt = ", ".join("%g" % v for v in r["observables"][k])
+        with open(os.path.join(GOLD, "simple_example_mode%d.csv" % k)) as f:
+            assert txt == f.read()
+
+
+def _ring_pair(gpu, oracle, L, N, psi_fn, T, step, m=40, tol=1e-8, **kw):
+    terms = as_tuples(_ring_terms(L))
+    ob = oracle.Basis.basis(N, L)
+    H = oracle.build_operator(ob, terms)
+    oobs = [("sparse", oracle.build_operator(ob, as_tuples([createNumberOperator(k, 1)]))) for k in range(L)]
+    psi = psi_fn(ob.D)
+    ro = oracle.evolve(ob.D, H, psi, T, step, oobs, tol=tol, m=m, **kw)
+    b, Hg, gobs = _gpu_model(gpu, "basis", (N, L), terms, [[createNumberOperator(k, 1)] for k in range(L)])
+    gkw = {"fast_integration": kw["fastIntegration"]} if "fastIntegration" in kw else {}
+    rg = capi.Evolver(gpu, Hg, psi, T, step, gobs, tol=tol, m=m, **gkw).run()
+    return ro, rg
+
+
+def _check_pair(ro, rg, obs_norm):
+    bound = ro["err"] + rg["err"]
+    dpsi = np.linalg.norm(rg["state"] - ro["state"])
+    dobs = np.abs(rg["observables"] - ro["observables"]).max()
+    print("steps gpu/oracle %d/%d  err %.3e/%.3e  |dpsi| %.3e  max|dobs| %.3e  status %d/%d" %
+          (rg["n_steps"], ro["n_steps"], rg["err"], ro["err"], dpsi, dobs, rg["statusCode"], ro["statusCode"]))
+    assert dpsi <= bound + 1e-10
+    assert dobs <= 2 * obs_norm * bound + 1e-10
+    assert rg["numSamples"] == ro["numSamples"] == ro["n_samples"]
+    assert rg["statusCode"] == ro["statusCode"]
+
+
+def test_ring_small_config_vs_oracle(gpu, oracle_lib):
+    """Ring Bose-Hubbard L = N = 8 (D = 6435), Fock |8,0,...>, T = 10, m = 40, tol = 1e-8, step 0.1."""
+    ro, rg = _ring_pair(gpu, oracle_lib, 8, 8, fock_state, 10.0, 0.1)
+    _check_pair(ro, rg, 8)
+
+
+def test_ring_random_state_vs_oracle(gpu, oracle_lib):
+    ro, rg = _ring_pair(gpu, oracle_lib, 6, 6, lambda D: random_state(D, 9), 3.0, 0.05, m=30)
+    _check_pair(ro, rg, 6)
+
+
+def test_fast_integration_vs_oracle(gpu, oracle_lib):
+    ro, rg = _ring_pair(gpu, oracle_lib, 6, 6, fock_state, 2.0, 0.1, fastIntegration=True)
+    _check_pair(ro, rg, 6)
+
+
+def test_generic_observables(gpu, oracle_lib):
+    """Non-diagonal sparse (H itself: energy), vector and host-callback observables."""
+    L = N = 5
+    terms = as_tuples(_ring_terms(L))
+    ob = oracle_lib.Basis.basis(N, L)
+    Hc = oracle_lib.build_operator(ob, terms)
+    psi = random_state(ob.D, 4)
+    proj = random_state(ob.D, 8)
+    n0 = oracle_lib.build_operator(ob, as_tuples([createNumberOperator(0, 1)]))
+    ro = oracle_lib.evolve(ob.D, Hc, psi, 2.0, 0.1, [("sparse", Hc), ("vector", proj), ("sparse", n0)], tol=1e-8)
+    H = capi.Matrix.from_coo(gpu, *Hc, ob.D)
+    N0 = capi.Matrix.from_coo(gpu, *n0, ob.D)
+    diag0 = np.zeros(ob.D)
+    diag0[n0[2]] = n0[0].real
+    host = ("host", lambda st: float(np.sum(diag0 * np.abs(st) ** 2)))
+    rg = capi.Evolver(gpu, H, psi, 2.0, 0.1, [H, ("vector", proj), host, N0], tol=1e-8).run()
+    bound = ro["err"] + rg["err"]
+    assert np.linalg.norm(rg["state"] - ro["state"]) <= bound + 1e-10
+    e0 = ro["observables"][0, 0]
+    assert np.abs(rg["observables"][0] - ro["observables"][0]).max() <= 1e-9       # energy
+    assert np.abs(rg["observables"][0] - e0).max() <= 1e-9                            # conserved
+    assert np.abs(rg["observables"][1] - ro["observables"][1]).max() <= 1e-9
+    assert np.abs(rg["observables"][2] - ro["observables"][2]).max() <= 1e-9         # host == diag
+    assert np.abs(rg["observables"][3] - ro["observables"][2]).max() <= 1e-9
+
+
+def test_stop_request(gpu, oracle_lib):
+    """An observable throwing requestStopException (core/krylovTimeEvolver.cpp:423-436)."""
+    L = N = 4
+    ob = oracle_lib.Basis.basis(N, L)
+    Hc = oracle_lib.build_operator(ob, as_tuples(_ring_terms(L)))
+    H = capi.Matrix.from_coo(gpu, *Hc, ob.D)
+    calls = {"n": 0}
+
+    def obs(st):
+        calls["n"] += 1
+        if calls["n"] == 6:
+            raise StopIteration
+        return 0.0
+
+    r = capi.Evolver(gpu, H, fock_state(ob.D), 2.0, 0.1, [("host", obs)], tol=1e-8).run()
+    ro = oracle_lib.evolve(ob.D, Hc, fock_state(ob.D), 2.0, 0.1, [], tol=1e-8, stop_at_sample=5)
+    assert r["statusCode"] == 3 and r["numSamples"] == 5
+    assert np.linalg.norm(r["state"] - ro["state"]) <= r["err"] + ro["err"] + 1e-10
+
+
+def test_fatal_unnormalised(gpu, oracle_lib):
+    L = N = 3
+    ob = oracle_lib.Basis.basis(N, L)
+    Hc = oracle_lib.build_operator(ob, as_tuples(_ring_terms(L)))
+    H = capi.Matrix.from_coo(gpu, *Hc, ob.D)
+    with pytest.raises(capi.TEError) as e:
+        capi.Evolver(gpu, H, 2 * fock_state(ob.D), 1.0, 0.1, [], tol=1e-8)
+    assert e.value.code == capi.TE_ERR_FATAL

@@ -1,0 +1,81 @@
+// Internal declarations shared by the HIP translation units of libtimeevolver_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/timeevolver.h"
+
+namespace te {
+
+constexpr int kMaxKrylov = 64;      // Krylov dimension cap: one wavefront holds a row of Q
+constexpr int kBlock = 256;         // threads per block of the streaming kernels
+
+extern int g_log_level;
+void log_msg(int level, const char* fmt, ...);
+
+// Sets ctx->err and returns code.
+int fail(te_ctx* ctx, int code, const char* fmt, ...);
+
+#define TE_HIP(ctx, call)                                                                  \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return ::te::fail((ctx), TE_ERR_HIP, "%s failed: %s (%s:%d)", #call,          \
+                              hipGetErrorString(e_), __FILE__, __LINE__);                  \
+    } while (0)
+
+struct SpmvShape {
+    int lanes_per_row;    // lanes of a wavefront cooperating on one row (1..64, power of 2)
+    int grid;             // blocks of the grid-stride launch
+};
+
+}  // namespace te
+
+struct te_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cus = 256;
+    int max_lds = 65536;        // bytes of LDS one workgroup may use
+    std::string err;
+};
+
+struct te_matrix {
+    te_ctx* ctx = nullptr;
+    int64_t n_rows = 0, n_cols = 0, nnz = 0;
+    int* row_ptr = nullptr;     // device, n_rows + 1
+    int* col = nullptr;         // device, nnz
+    double2* val = nullptr;     // device, nnz complex128
+    double norm1 = 0.0;
+    bool norm1_valid = false;
+    bool diagonal = false;      // every entry on the diagonal (observables of n_k type)
+    double* diag_re = nullptr;  // device, n_rows: Re of the diagonal when `diagonal`
+    te::SpmvShape shape{8, 1024};
+};
+
+namespace te {
+
+// SpMV launch shape for a matrix (lanes per row from the mean row length, grid from the CUs).
+SpmvShape choose_shape(const te_ctx* ctx, int64_t n_rows, int64_t nnz);
+
+// ---- kernels launched across translation units (te_spmv.hip) ----
+void launch_spmv(const te_matrix* A, double2 alpha, const double2* x, double2* y, hipStream_t s);
+// Deterministic reductions with a last-block-done finish. partials: >= grid entries.
+struct ReduceScratch {
+    double2* partials = nullptr;   // grid * 32 doubles2 (observable chunks use up to 32 slots)
+    unsigned int* ticket = nullptr;
+    double2* result = nullptr;     // 64 slots
+    int grid = 0;
+};
+int reduce_scratch_alloc(te_ctx* ctx, ReduceScratch& rs, int grid);
+void reduce_scratch_free(ReduceScratch& rs);
+// result[slot] = sum conj(x) y  (device)
+void launch_zdotc(const double2* x, const double2* y, int64_t n, ReduceScratch& rs, int slot, hipStream_t s);
+// result[slot].x = sum |x|^2
+void launch_nrm2sq(const double2* x, int64_t n, ReduceScratch& rs, int slot, hipStream_t s);
+
+}  // namespace te

@@ -1,0 +1,101 @@
+// Device-resident state of one time evolution and the launchers of its kernels.
+#pragma once
+
+#include "te_internal.h"
+
+namespace te {
+
+// Scalars of the current Krylov space, written by the Lanczos kernels and consumed by the
+// step controller. v_j = s_j u_j; H[j][j] = a_j; H[j+1][j] = -H[j][j+1] = beta_j.
+struct KrylovScalars {
+    double2 a[kMaxKrylov];
+    double beta[kMaxKrylov];
+    double s[kMaxKrylov + 1];
+    double h;          // last Lanczos norm (core/krylovTimeEvolver.cpp:566-567)
+    int broken;        // lucky breakdown in this Krylov step (:551-555)
+    int m_hbd;         // Krylov dimension at the breakdown
+};
+
+struct LanczosArgs {
+    int64_t n;
+    const int* ptr;
+    const int* col;
+    const double2* val;
+    double2* U;        // (m+1) columns of length ld, column-major
+    int64_t ld;
+    int j;
+    int m;
+    double2 alpha;     // -i * expFactor (core/krylovTimeEvolver.cpp:31)
+    double tolRate;
+    KrylovScalars* ks;
+    double2* partials;
+    unsigned int* ticket;
+};
+
+// Persistent control state of timeEvolve (core/krylovTimeEvolver.cpp:261-512), updated by
+// the step controller once per Krylov step and read by the host after each step.
+struct StepState {
+    double t_now, t_sampling, err, t_step;
+    double t_now_prev, err_prev;        // values before the current step
+    long long index_samples;
+    long long sample_base;              // first sample index taken in the current step
+    int S;                              // samples taken in the current step
+    int n_steps;
+    int m;                              // Krylov dimension (shrinks after a breakdown, :358)
+    int statusCode;
+    int errorCodeFindSubstep;           // persists across steps as in the reference (:298)
+    int nbErrRoundOff, nbErrInt;
+    int fatal;                          // 1: no small enough step (:203-207); 2: eigen-solver
+    int breakdown;                      // lucky breakdown in the current step
+    int write_w;                        // the current step updates currentVec (:374-383)
+    int pad;
+};
+
+struct StepConsts {
+    double t, samplingStep, tolRate, tol, numErrEst;
+    long long n_samples;
+    int fastIntegration;
+    int ts_rows;
+    const double2* ts_nodes;            // tanh-sinh (complement, weight), rows concatenated
+    const int* ts_off;                  // row offsets, ts_rows + 1 entries
+    const double2* gl15;                // Gauss-Legendre (x, w): centre first, then x > 0
+    const double2* gl7;
+    int gl15_n, gl7_n;
+    double2* Y;                         // n_samples x kMaxKrylov sample coefficients
+    double2* wcoef;                     // kMaxKrylov coefficients of currentVec
+    double* trace;                      // t_step per Krylov step
+    long long trace_cap;
+};
+
+struct CombineArgs {
+    int64_t n;
+    const double2* U;
+    int64_t ld;
+    int m;
+    const double2* wcoef;
+    int write_w;                        // U[:,0] = U w   (currentVec, :382)
+    double2* U0;                        // == U (column 0), written when write_w
+    const double2* Y;                   // sample coefficients, row y_base + s
+    long long y_base;
+    int S;
+    double2* last_out;                  // state of sample y_base + S - 1 when write_last
+    int write_last;
+    const double* diag;                 // nobs x n, Re of diagonal observables
+    int nobs;
+    int obs_base;                       // index of the first of these observables in obs_values
+    double* obs_values;                 // total_obs x n_samples
+    long long n_samples;
+    double2* partials;
+    unsigned int* ticket;
+};
+
+void launch_lanczos_apply(const LanczosArgs& a, int lpr, int grid, hipStream_t s);
+void launch_lanczos_project(const LanczosArgs& a, int grid, hipStream_t s);
+void launch_step_control(const StepConsts& c, StepState* st, KrylovScalars* ks, hipStream_t s);
+// Picks the sample/observable chunking; S <= max_samples_per_launch(nobs).
+int combine_samples_per_launch(int nobs);
+int combine_obs_per_launch();
+void launch_combine(const CombineArgs& a, int grid, hipStream_t s);
+void launch_store_obs(const double2* res, int slot, int mode, double* out, hipStream_t s);
+
+}  // namespace te

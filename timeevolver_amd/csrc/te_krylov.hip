@@ -1,0 +1,568 @@
+// Krylov-space work of timeEvolve kept on the device (gfx950), compiled with
+// -ffp-contract=off so that every scalar of the step control is rounded exactly as the
+// reference's x86-64 build rounds it.
+//
+//  * step_control_kernel (one block): the m x m eigen-decomposition that replaces
+//    LAPACKE_zhseqr (core/matrixDataTypes.h:135-156, called at
+//    core/krylovTimeEvolver.cpp:365), the error-bound quadrature (integrateError, :584-618),
+//    the step search (findMaximalStepSize, :153-254) and the sample schedule (:405-437),
+//    producing the Krylov coefficient vectors of the new state and of every sample.
+//  * combine_kernel (grid): currentVec = V w (:382), sampledState = V y (:421) and the
+//    diagonal-observable expectations Re<psi|O|psi> = sum_r Re(O_rr) |psi_r|^2
+//    (krylovSpMatrixObservable::expectation, core/krylovObservables.cpp:173-194), one pass over
+//    the Krylov basis for a whole chunk of samples.
+//
+// Krylov exponential: the Lanczos matrix is anti-Hermitian tridiagonal, H[j][j] = a_j,
+// H[j+1][j] = -H[j][j+1] = beta_j. With D = diag((-i)^j), D H D^{-1} = -i T, T real symmetric
+// tridiagonal with T[j][j] = -Im a_j and off-diagonal beta_j, so
+//   exp(x H) e1 = D^{-1} Q exp(-i x Lambda) Q^T e1,   T = Q Lambda Q^T,
+// and the error integrand is |[exp(xH)e1]_{m-1}| = |sum_k p_k e^{-i x lambda_k}|,
+// p_k = Q[m-1][k] Q[0][k]: O(m) per quadrature node instead of two m x m products.
+#include "te_krylov.h"
+#include "te_reduce.h"
+
+#include <cfloat>
+
+namespace te {
+
+constexpr int kCtl = 256;            // threads of the step controller
+constexpr int kPts = 256;            // integrand points per evaluation pass
+constexpr int kGrp = 8;              // lanes per integrand point
+
+struct CtlShared {
+    double Q[kMaxKrylov * kMaxKrylov];  // eigenvectors: column k at Q[k * 64 + row]
+    double lam[kMaxKrylov];
+    double d[kMaxKrylov], e[kMaxKrylov];
+    double p[kMaxKrylov], g[kMaxKrylov], s[kMaxKrylov];
+    double2 ph[kMaxKrylov];
+    double xs[kPts], fx[kPts], wt[kPts];
+    double bc[4];                       // broadcast slots
+    double h;
+    int m;
+    int info;
+};
+
+// ---------------------------------------------------------------------------------------
+// Implicit QL with Wilkinson shifts on a symmetric tridiagonal matrix (Golub & Van Loan,
+// section 8.3), run by one wavefront: every lane computes the rotation scalars (uniform
+// values kept in LDS) and lane r applies each rotation to row r of Q.
+// ---------------------------------------------------------------------------------------
+__device__ int wave_tridiag_ql(int n, double* d, double* e, double* Q) {
+    const int lane = threadIdx.x & 63;
+    for (int k = 0; k < n; k++) Q[k * 64 + lane] = (lane == k) ? 1.0 : 0.0;
+    if (n == 1) return 0;
+    e[n - 1] = 0.0;
+    for (int l = 0; l < n; l++) {
+        int iter = 0;
+        while (true) {
+            int mm;
+            for (mm = l; mm < n - 1; mm++) {
+                const double dd = fabs(d[mm]) + fabs(d[mm + 1]);
+                if (fabs(e[mm]) <= DBL_EPSILON * dd) break;
+            }
+            if (mm == l) break;
+            if (++iter > 60) return 1;
+            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+            double r = hypot(g, 1.0);
+            g = d[mm] - d[l] + e[l] / (g + copysign(r, g));
+            double s = 1.0, c = 1.0, p = 0.0;
+            bool early = false;
+            for (int i = mm - 1; i >= l; i--) {
+                const double f = s * e[i];
+                const double bb = c * e[i];
+                r = hypot(f, g);
+                e[i + 1] = r;
+                if (r == 0.0) {
+                    d[i + 1] -= p;
+                    e[mm] = 0.0;
+                    early = true;
+                    break;
+                }
+                s = f / r;
+                c = g / r;
+                g = d[i + 1] - p;
+                r = (d[i] - g) * s + 2.0 * c * bb;
+                p = s * r;
+                d[i + 1] = g + p;
+                g = c * r - bb;
+                if (lane < n) {
+                    const double qf = Q[(i + 1) * 64 + lane];
+                    const double qi = Q[i * 64 + lane];
+                    Q[(i + 1) * 64 + lane] = s * qi + c * qf;
+                    Q[i * 64 + lane] = c * qi - s * qf;
+                }
+            }
+            if (early) continue;
+            d[l] -= p;
+            e[l] = g;
+            e[mm] = 0.0;
+        }
+    }
+    return 0;
+}
+
+// f(x) = h |sum_k p_k e^{-i x lambda_k}| at S.xs[0..npts) -> S.fx; kGrp lanes per point.
+__device__ void eval_points(CtlShared& S, int npts) {
+    const int tid = threadIdx.x;
+    const int grp = tid / kGrp, q = tid % kGrp;
+    const int m = S.m;
+    for (int base = 0; base < npts; base += kCtl / kGrp) {
+        const int pt = base + grp;
+        double re = 0.0, im = 0.0;
+        if (pt < npts) {
+            const double x = S.xs[pt];
+            for (int k = q; k < m; k += kGrp) {
+                double sn, cs;
+                sincos(x * S.lam[k], &sn, &cs);
+                re += S.p[k] * cs;
+                im -= S.p[k] * sn;
+            }
+        }
+#pragma unroll
+        for (int off = kGrp / 2; off > 0; off >>= 1) {
+            re += __shfl_xor(re, off, kGrp);
+            im += __shfl_xor(im, off, kGrp);
+        }
+        if (q == 0 && pt < npts) S.fx[pt] = S.h * hypot(re, im);
+    }
+    __syncthreads();
+}
+
+// Weighted sums over slots [first, first+cnt) in slot order (thread 0), broadcast to all.
+__device__ void weighted_sums(CtlShared& S, int first, int cnt, double& s, double& as) {
+    if (threadIdx.x == 0) {
+        double a = s, b = as;
+        for (int i = first; i < first + cnt; i++) {
+            a += S.fx[i] * S.wt[i];
+            b += fabs(S.fx[i]) * S.wt[i];
+        }
+        S.bc[0] = a;
+        S.bc[1] = b;
+    }
+    __syncthreads();
+    s = S.bc[0];
+    as = S.bc[1];
+    __syncthreads();
+}
+
+// One tanh-sinh row: nodes [first, end) of row `row`, right then left point of each node;
+// points that round onto an end point get weight 0. When fc != nullptr the centre c is
+// evaluated in the same pass and returned in *fc.
+__device__ void ts_row(CtlShared& S, const StepConsts& C, int row, int first, double a, double b, double c,
+                       double d, double& s, double& as, double* fc) {
+    const int off0 = C.ts_off[row] + first, off1 = C.ts_off[row + 1];
+    const int nn = off1 - off0;
+    s = 0.0;
+    as = 0.0;
+    int done = 0;
+    bool first_chunk = true;
+    while (done < nn || (first_chunk && fc != nullptr)) {
+        const int slot = (first_chunk && fc != nullptr) ? 1 : 0;
+        int cnt = (kPts - slot) / 2;
+        if (cnt > nn - done) cnt = nn - done;
+        const int tid = threadIdx.x;
+        if (slot == 1 && tid == 0) {
+            S.xs[0] = c;
+            S.wt[0] = 0.0;
+        }
+        if (tid < cnt) {
+            const double2 nd = C.ts_nodes[off0 + done + tid];
+            const double off = d * nd.x;
+            const double xr = b - off, xl = a + off;
+            S.xs[slot + 2 * tid] = xr;
+            S.wt[slot + 2 * tid] = (xr != b) ? nd.y : 0.0;
+            S.xs[slot + 2 * tid + 1] = xl;
+            S.wt[slot + 2 * tid + 1] = (xl != a) ? nd.y : 0.0;
+        }
+        __syncthreads();
+        eval_points(S, slot + 2 * cnt);
+        if (slot == 1) *fc = S.fx[0];
+        weighted_sums(S, slot, 2 * cnt, s, as);
+        done += cnt;
+        first_chunk = false;
+    }
+}
+
+// Tanh-sinh with Boost's driver structure (h = 1 on row 0, halving per row, error
+// |I_k - I_{k-1}|, stop when error <= tol * L1, at most 14 refinements).
+__device__ double ts_integrate(CtlShared& S, const StepConsts& C, double a, double b, double tol, double& error,
+                               double& L1, bool& finite) {
+    const double half_pi = 1.5707963267948966;
+    const double c = 0.5 * (a + b), d = 0.5 * (b - a);
+    double h = 1.0;
+    double y0 = 0.0, s, as;
+    ts_row(S, C, 0, 1, a, b, c, d, s, as, &y0);
+    double I0 = half_pi * y0, L0 = half_pi * fabs(y0);
+    I0 = (I0 + s) * h;
+    L0 = (L0 + as) * h;
+    double I1 = I0, L11 = L0, err = DBL_MAX;
+    finite = true;
+    for (int k = 1; k < C.ts_rows; k++) {
+        I0 = I1;
+        L0 = L11;
+        I1 = 0.5 * I0;
+        L11 = 0.5 * L0;
+        h *= 0.5;
+        ts_row(S, C, k, 0, a, b, c, d, s, as, nullptr);
+        I1 += s * h;
+        L11 += as * h;
+        err = fabs(I0 - I1);
+        if (!isfinite(I1)) {
+            finite = false;
+            break;
+        }
+        if (err <= tol * L11) break;
+    }
+    error = err * d;
+    L1 = L11 * d;
+    return I1 * d;
+}
+
+// Gauss-Legendre: centre first (odd n), then (f(c + d x_i) + f(c - d x_i)) w_i.
+__device__ double gl_integrate(CtlShared& S, const double2* nodes, int nn, int n, double a, double b) {
+    const double c = 0.5 * (a + b), d = 0.5 * (b - a);
+    const int tid = threadIdx.x;
+    const bool odd = (n & 1) != 0;
+    const int first = odd ? 1 : 0;
+    if (tid == 0 && odd) S.xs[0] = c;
+    if (tid >= first && tid < nn) {
+        const double2 nd = nodes[tid];
+        const int k = first + 2 * (tid - first);
+        S.xs[k] = c + d * nd.x;
+        S.xs[k + 1] = c - d * nd.x;
+    }
+    __syncthreads();
+    eval_points(S, first + 2 * (nn - first));
+    if (tid == 0) {
+        double s = odd ? S.fx[0] * nodes[0].y : 0.0;
+        for (int i = first; i < nn; i++) {
+            const int k = first + 2 * (i - first);
+            s += (S.fx[k] + S.fx[k + 1]) * nodes[i].y;
+        }
+        S.bc[2] = s * d;
+    }
+    __syncthreads();
+    const double r = S.bc[2];
+    __syncthreads();
+    return r;
+}
+
+// integrateError (core/krylovTimeEvolver.cpp:584-618); method 0 GL15, 1 GL7, 2 tanh-sinh.
+__device__ double integrate_error(CtlShared& S, const StepConsts& C, double a, double b, int method,
+                                  bool& successful) {
+    const double termination = 1e-3;                                 // :54
+    double ret;
+    bool success = true;
+    if (method == 0) ret = gl_integrate(S, C.gl15, C.gl15_n, 15, a, b);
+    else if (method == 1) ret = gl_integrate(S, C.gl7, C.gl7_n, 7, a, b);
+    else {
+        double error = 0.0, L1 = 0.0;
+        bool finite = true;
+        ret = ts_integrate(S, C, a, b, termination, error, L1, finite);
+        if (!finite) {                                               // Boost throws (:602-605)
+            ret = C.tol * 10;
+            error = 0.0;
+        }
+        if (error / ret > termination && ret > termination * (b - a) * C.tolRate) success = false;   // :607
+    }
+    successful = successful && success;
+    return ret;
+}
+
+// out[j] = s_j [exp(x H) e1]_j  (expKrylov, :685-702, scaled for the unnormalised basis).
+__device__ void exp_e1_scaled(CtlShared& S, double x, double2* out) {
+    const int tid = threadIdx.x, m = S.m;
+    if (tid < m) {
+        double sn, cs;
+        sincos(x * S.lam[tid], &sn, &cs);
+        S.ph[tid] = make_double2(S.g[tid] * cs, -S.g[tid] * sn);
+    }
+    __syncthreads();
+    if (tid < m) {
+        double re = 0.0, im = 0.0;
+        for (int k = 0; k < m; k++) {
+            const double q = S.Q[k * 64 + tid];
+            re += q * S.ph[k].x;
+            im += q * S.ph[k].y;
+        }
+        double2 v;
+        switch (tid & 3) {                    // multiply by i^j exactly
+            case 0: v = make_double2(re, im); break;
+            case 1: v = make_double2(-im, re); break;
+            case 2: v = make_double2(-re, -im); break;
+            default: v = make_double2(im, -re); break;
+        }
+        const double sj = S.s[tid];
+        out[tid] = make_double2(v.x * sj, v.y * sj);
+    }
+    __syncthreads();
+}
+
+// findMaximalStepSize (core/krylovTimeEvolver.cpp:153-254). Returns the status code, or -1
+// where the reference exits after GO_MAX halvings.
+__device__ int find_step(CtlShared& S, const StepConsts& C, double t_stepSuggestion, double t_step_max,
+                         int n_substeps, bool increaseStep, int mLong, int mShort, double* t_stepRet,
+                         double* err_stepRet) {
+    const unsigned GO_MAX = 20;
+    const double tolRate = C.tolRate;
+    unsigned nbReductions = 0;
+    double t_step = t_stepSuggestion;
+    int code = 0;
+    bool okInt = true;
+    bool skipSubsteps = false;
+    if (t_step > t_step_max) {
+        t_step = t_step_max;
+        skipSubsteps = true;
+    }
+    double err_step = integrate_error(S, C, 0, t_step, mLong, okInt);
+    if (increaseStep && !skipSubsteps) {
+        double err_new = err_step, t_new = t_step;
+        while (err_new < tolRate * t_new && t_step < t_step_max) {
+            t_step = t_new;
+            err_step = err_new;
+            if (t_step >= t_step_max) {
+                skipSubsteps = true;
+                break;
+            }
+            t_new *= 2;
+            if (t_new > t_step_max) t_new = t_step_max;
+            err_new = integrate_error(S, C, 0, t_new, mLong, okInt);
+        }
+    }
+    while (err_step > tolRate * t_step) {
+        nbReductions++;
+        t_step = t_step / 2.0;
+        err_step = integrate_error(S, C, 0, t_step, mLong, okInt);
+        if (nbReductions == GO_MAX) return -1;
+        skipSubsteps = false;
+    }
+    const double s = t_step / n_substeps;
+    int n_s = 0;
+    double deltaError = 0;
+    if (!skipSubsteps) {
+        while (err_step + deltaError < tolRate * (t_step + n_s * s)) {
+            err_step += deltaError;
+            n_s++;
+            if (t_step + (n_s - 1) * s >= t_step_max) break;
+            deltaError = integrate_error(S, C, t_step + (n_s - 1) * s, t_step + n_s * s, mShort, okInt);
+        }
+        t_step += (n_s - 1) * s;
+    }
+    if (err_step < C.numErrEst) {
+        if (!skipSubsteps || err_step > 0.1 * tolRate * t_step) code += 1;
+    }
+    if (!okInt) code += 20;
+    *t_stepRet = t_step;
+    *err_stepRet = err_step;
+    exp_e1_scaled(S, t_step, C.wcoef);
+    return code;
+}
+
+// One Krylov step of timeEvolve after the Lanczos recurrence (:342-441).
+__global__ __launch_bounds__(kCtl) void step_control_kernel(StepConsts C, StepState* st, KrylovScalars* ks) {
+    __shared__ CtlShared S;
+    const int tid = threadIdx.x;
+    StepState L = *st;
+    const bool hbd = ks->broken != 0;
+    int m = L.m;
+    if (hbd) m = ks->m_hbd;
+    L.n_steps += 1;
+    L.breakdown = hbd ? 1 : 0;
+    double t_step = L.t_step;
+    if (hbd) {                                        // :342-363
+        t_step = C.t - L.t_now;
+        L.m = m;
+        L.statusCode = 1;
+    }
+    if (tid < m) {
+        S.d[tid] = -ks->a[tid].y;
+        S.e[tid] = (tid + 1 < m) ? ks->beta[tid] : 0.0;
+        S.s[tid] = ks->s[tid];
+    }
+    if (tid == 0) {
+        S.m = m;
+        S.h = ks->h;
+    }
+    __syncthreads();
+    if (tid < 64) {
+        const int info = wave_tridiag_ql(m, S.d, S.e, S.Q);
+        if (tid == 0) S.info = info;
+    }
+    __syncthreads();
+    if (S.info != 0) {                               // :366-370
+        if (tid == 0) st->fatal = 2;
+        return;
+    }
+    if (tid < m) {
+        S.lam[tid] = S.d[tid];
+        S.g[tid] = S.Q[tid * 64];
+        S.p[tid] = S.Q[tid * 64 + m - 1] * S.Q[tid * 64];
+    }
+    __syncthreads();
+
+    double err_step = 0;
+    if (!hbd) {                                       // :374-383
+        const int mLong = C.fastIntegration ? 0 : 2, mShort = C.fastIntegration ? 1 : 2;   // :269-277
+        const double s0 = 0.97 * t_step;
+        const int code = find_step(S, C, s0, C.t - L.t_now, 50, L.t_now == 0, mLong, mShort, &t_step, &err_step);
+        if (code < 0) {
+            if (tid == 0) st->fatal = 1;
+            return;
+        }
+        L.errorCodeFindSubstep = code;
+    }
+    switch (L.errorCodeFindSubstep) {                 // :386-399
+        case 0: break;
+        case 1: L.nbErrRoundOff++; break;
+        case 20: L.nbErrInt++; break;
+        case 21: L.nbErrInt++; L.nbErrRoundOff++; break;
+    }
+    if (tid == 0 && L.n_steps - 1 < C.trace_cap) C.trace[L.n_steps - 1] = t_step;
+
+    L.sample_base = L.index_samples;                  // :405-437
+    L.S = 0;
+    while (L.index_samples < C.n_samples &&
+           (L.t_sampling + C.samplingStep <= L.t_now + t_step ||
+            (L.index_samples == C.n_samples - 1 && fabs(L.t_now + t_step - C.t) <= DBL_EPSILON))) {
+        L.t_sampling += C.samplingStep;
+        exp_e1_scaled(S, L.t_sampling - L.t_now, C.Y + (size_t)L.index_samples * kMaxKrylov);
+        L.index_samples++;
+        L.S++;
+    }
+    L.t_now_prev = L.t_now;
+    L.err_prev = L.err;
+    L.t_now += t_step;                                // :440-441
+    L.err += err_step;
+    L.t_step = t_step;
+    L.write_w = hbd ? 0 : 1;
+    __syncthreads();
+    if (tid == 0) {
+        *st = L;
+        ks->broken = 0;
+        ks->s[0] = 1.0;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// V w, V y_s and diagonal observables in one pass over the Krylov basis.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ void block_rows_k(int64_t n, int64_t& lo, int64_t& hi) {
+    const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const int lb = xcd_block_id();
+    lo = (int64_t)lb * chunk;
+    hi = lo + chunk < n ? lo + chunk : n;
+}
+
+template <int SC, int NOB>
+__global__ __launch_bounds__(kBlock) void combine_kernel(CombineArgs a) {
+    __shared__ double2 cf[(1 + SC) * kMaxKrylov];
+    __shared__ double2 sh[kBlock / 64];
+    const int tid = threadIdx.x;
+    const int m = a.m, S = a.S;
+    for (int i = tid; i < (1 + SC) * kMaxKrylov; i += kBlock) {
+        const int c = i / kMaxKrylov, j = i % kMaxKrylov;
+        double2 v = make_double2(0.0, 0.0);
+        if (j < m) {
+            if (c == 0) {
+                if (a.write_w) v = a.wcoef[j];
+            } else if (c - 1 < S) {
+                v = a.Y[(size_t)(a.y_base + c - 1) * kMaxKrylov + j];
+            }
+        }
+        cf[i] = v;
+    }
+    __syncthreads();
+    double acc[NOB > 0 ? NOB * SC : 1];
+#pragma unroll
+    for (int i = 0; i < (NOB > 0 ? NOB * SC : 1); i++) acc[i] = 0.0;
+    int64_t lo, hi;
+    block_rows_k(a.n, lo, hi);
+    for (int64_t row = lo + tid; row < hi; row += kBlock) {
+        double2 aw = make_double2(0.0, 0.0);
+        double2 as[SC];
+#pragma unroll
+        for (int s = 0; s < SC; s++) as[s] = make_double2(0.0, 0.0);
+        for (int j = 0; j < m; j++) {
+            const double2 u = a.U[(size_t)j * a.ld + row];
+            if (a.write_w) {
+                const double2 t = cmul(u, cf[j]);
+                aw.x += t.x;
+                aw.y += t.y;
+            }
+#pragma unroll
+            for (int s = 0; s < SC; s++) {
+                const double2 t = cmul(u, cf[(1 + s) * kMaxKrylov + j]);
+                as[s].x += t.x;
+                as[s].y += t.y;
+            }
+        }
+        if (a.write_w) a.U0[row] = aw;
+        if (a.write_last && S > 0) {
+            double2 v = as[0];
+#pragma unroll
+            for (int s = 1; s < SC; s++)
+                if (s == S - 1) v = as[s];
+            a.last_out[row] = v;
+        }
+        if (NOB > 0) {
+#pragma unroll
+            for (int s = 0; s < SC; s++) {
+                const double pr = as[s].x * as[s].x + as[s].y * as[s].y;
+#pragma unroll
+                for (int o = 0; o < (NOB > 0 ? NOB : 1); o++)
+                    if (o < a.nobs) acc[o * SC + s] += a.diag[(size_t)o * a.n + row] * pr;
+            }
+        }
+    }
+    if (NOB == 0) return;
+#pragma unroll
+    for (int i = 0; i < (NOB > 0 ? NOB * SC : 1); i++) {
+        const double2 t = block_sum2<kBlock>(make_double2(acc[i], 0.0), sh);
+        if (tid == 0) a.partials[(size_t)blockIdx.x * (NOB * SC) + i] = t;
+    }
+    if (last_block_arrive(a.ticket)) {
+        for (int i = 0; i < NOB * SC; i++) {
+            const int o = i / SC, s = i % SC;
+            const double2 tot = reduce_partials<kBlock>(a.partials + i, gridDim.x, NOB * SC, sh);
+            if (tid == 0 && o < a.nobs && s < S)
+                a.obs_values[(size_t)(a.obs_base + o) * a.n_samples + a.y_base + s] = tot.x;
+        }
+        if (tid == 0) *a.ticket = 0;
+    }
+}
+
+__global__ void store_obs_kernel(const double2* res, int slot, int mode, double* out) {
+    const double2 r = res[slot];
+    *out = mode == 0 ? r.x : r.x * r.x + r.y * r.y;
+}
+
+// ---------------------------------------------------------------------------------------
+void launch_step_control(const StepConsts& c, StepState* st, KrylovScalars* ks, hipStream_t s) {
+    hipLaunchKernelGGL(step_control_kernel, dim3(1), dim3(kCtl), 0, s, c, st, ks);
+}
+
+int combine_obs_per_launch() { return 32; }
+
+int combine_samples_per_launch(int nobs) {
+    if (nobs <= 4) return 8;
+    if (nobs <= 8) return 4;
+    if (nobs <= 16) return 2;
+    return 1;
+}
+
+void launch_combine(const CombineArgs& a, int grid, hipStream_t s) {
+    const dim3 g(grid), b(kBlock);
+    if (a.nobs == 0) hipLaunchKernelGGL((combine_kernel<8, 0>), g, b, 0, s, a);
+    else if (a.nobs <= 1) hipLaunchKernelGGL((combine_kernel<8, 1>), g, b, 0, s, a);
+    else if (a.nobs <= 2) hipLaunchKernelGGL((combine_kernel<8, 2>), g, b, 0, s, a);
+    else if (a.nobs <= 4) hipLaunchKernelGGL((combine_kernel<8, 4>), g, b, 0, s, a);
+    else if (a.nobs <= 8) hipLaunchKernelGGL((combine_kernel<4, 8>), g, b, 0, s, a);
+    else if (a.nobs <= 16) hipLaunchKernelGGL((combine_kernel<2, 16>), g, b, 0, s, a);
+    else hipLaunchKernelGGL((combine_kernel<1, 32>), g, b, 0, s, a);
+}
+
+void launch_store_obs(const double2* res, int slot, int mode, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(store_obs_kernel, dim3(1), dim3(1), 0, s, res, slot, mode, out);
+}
+
+}  // namespace te

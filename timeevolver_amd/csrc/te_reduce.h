@@ -1,0 +1,79 @@
+// Device-side reduction helpers shared by the streaming kernels. All reductions are
+// deterministic for a given launch shape: fixed shuffle trees inside a wavefront, fixed
+// order across the waves of a block, and a last-block-done finish that sums the block
+// partials in index order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace te {
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// Sum over a block of kBlock (256) threads; the result is valid in every thread.
+template <int NT>
+__device__ __forceinline__ double2 block_sum2(double2 v, double2* sh /* >= NT/64 entries */) {
+    v.x = wave_sum(v.x);
+    v.y = wave_sum(v.y);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double2 r = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int i = 0; i < NT / 64; i++) {
+        r.x += sh[i].x;
+        r.y += sh[i].y;
+    }
+    return r;
+}
+
+// Called by all threads after thread 0 stored this block's partial. Returns true in every
+// thread of the block that arrives last; that block then sees every other block's partial.
+__device__ __forceinline__ bool last_block_arrive(unsigned int* ticket) {
+    __shared__ bool last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        unsigned int t = atomicAdd(ticket, 1u);
+        last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (last) __threadfence();
+    return last;
+}
+
+// Sums partials[0..n) (stride `stride` doubles2 between blocks) in the last block.
+template <int NT>
+__device__ __forceinline__ double2 reduce_partials(const double2* partials, int n, int stride, double2* sh) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int i = threadIdx.x; i < n; i += NT) {
+        double2 p = partials[(size_t)i * stride];
+        acc.x += p.x;
+        acc.y += p.y;
+    }
+    return block_sum2<NT>(acc, sh);
+}
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// conj(a) * b
+__device__ __forceinline__ double2 cmulc(double2 a, double2 b) {
+    return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+
+// Logical block id remapped so that each of the 8 XCDs (which receive blocks round-robin)
+// works on one contiguous eighth of the logical index space: neighbouring row ranges then
+// share an L2.
+__device__ __forceinline__ int xcd_block_id() {
+    const int g = gridDim.x, b = blockIdx.x;
+    if ((g & 7) != 0) return b;
+    return (b & 7) * (g >> 3) + (b >> 3);
+}
+
+}  // namespace te

@@ -1,0 +1,251 @@
+// Complex128 CSR SpMV and the fused Lanczos kernels for gfx950.
+//
+// Replaces smatrix::spMV = mkl_sparse_z_mv (core/matrixDataTypes.cpp:169-190) and the
+// BLAS-1 chain of arnoldiAlgorithm (core/krylovTimeEvolver.cpp:523-571).
+//
+// Layout: CSR with int32 row pointers and column indices, complex128 values as double2
+// (16-byte loads). A group of LPR lanes of a wavefront shares one row: lane l reads entries
+// start+l, start+l+LPR, ..., so neighbouring groups stream neighbouring value/index ranges
+// and a wave issues wide, mostly contiguous loads. Partial row sums are combined with
+// xor-shuffles inside the group. Blocks own contiguous row ranges (xcd_block_id keeps an
+// XCD's rows contiguous so the gather of x reuses its L2).
+//
+// Lanczos in unnormalised form: the Krylov vectors are stored as u_j with v_j = s_j u_j,
+// s_0 = 1, s_{j+1} = 1/beta_j, so no separate normalisation pass exists:
+//   apply  : w = (alpha s_j) A u_j + (beta_{j-1} s_{j-1}) u_{j-1};  a_j = s_j <u_j, w>
+//   project: w' = w - (a_j s_j) u_j;  beta_j = |w'|;  u_{j+1} = w' (stored in place)
+// which is the reference's recurrence (:530-565) with H[j][j] = a_j, H[j+1][j] = beta_j.
+#include "te_internal.h"
+#include "te_krylov.h"
+#include "te_reduce.h"
+
+namespace te {
+
+// ---------------------------------------------------------------------------------------
+// Row-range partition: block `lb` owns rows [lb*chunk, min((lb+1)*chunk, n)).
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ void block_rows(int64_t n, int64_t& lo, int64_t& hi) {
+    const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const int lb = xcd_block_id();
+    lo = (int64_t)lb * chunk;
+    hi = lo + chunk < n ? lo + chunk : n;
+}
+
+// Sum over the row's entries of val[k] * x[col[k]] by the LPR lanes of a group.
+template <int LPR>
+__device__ __forceinline__ double2 row_dot(int row, const int* __restrict__ ptr, const int* __restrict__ col,
+                                           const double2* __restrict__ val, const double2* __restrict__ x,
+                                           int lane) {
+    const int start = ptr[row], end = ptr[row + 1];
+    double sr = 0.0, si = 0.0;
+    for (int k = start + lane; k < end; k += LPR) {
+        const double2 a = val[k];
+        const double2 b = x[col[k]];
+        sr += a.x * b.x - a.y * b.y;
+        si += a.x * b.y + a.y * b.x;
+    }
+#pragma unroll
+    for (int off = LPR / 2; off > 0; off >>= 1) {
+        sr += __shfl_xor(sr, off, LPR);
+        si += __shfl_xor(si, off, LPR);
+    }
+    return make_double2(sr, si);
+}
+
+template <int LPR>
+__global__ __launch_bounds__(kBlock) void spmv_kernel(int64_t n, const int* __restrict__ ptr,
+                                                      const int* __restrict__ col, const double2* __restrict__ val,
+                                                      const double2* __restrict__ x, double2* __restrict__ y,
+                                                      double2 alpha) {
+    constexpr int RPB = kBlock / LPR;
+    int64_t lo, hi;
+    block_rows(n, lo, hi);
+    const int lane = threadIdx.x % LPR;
+    for (int64_t base = lo; base < hi; base += RPB) {
+        const int64_t row = base + threadIdx.x / LPR;
+        if (row < hi) {
+            double2 s = row_dot<LPR>((int)row, ptr, col, val, x, lane);
+            if (lane == 0) y[row] = cmul(alpha, s);
+        }
+    }
+}
+
+template <int LPR>
+__global__ __launch_bounds__(kBlock) void lanczos_apply_kernel(LanczosArgs a) {
+    KrylovScalars* ks = a.ks;
+    if (ks->broken) return;
+    constexpr int RPB = kBlock / LPR;
+    __shared__ double2 sh[kBlock / 64];
+    const int j = a.j;
+    const double sj = ks->s[j];
+    const double2 c1 = make_double2(a.alpha.x * sj, a.alpha.y * sj);
+    const double c2 = j > 0 ? ks->beta[j - 1] * ks->s[j - 1] : 0.0;
+    const double2* __restrict__ uj = a.U + (size_t)j * a.ld;
+    const double2* __restrict__ up = j > 0 ? a.U + (size_t)(j - 1) * a.ld : nullptr;
+    double2* __restrict__ w = a.U + (size_t)(j + 1) * a.ld;
+    int64_t lo, hi;
+    block_rows(a.n, lo, hi);
+    const int lane = threadIdx.x % LPR;
+    double2 dot = make_double2(0.0, 0.0);
+    for (int64_t base = lo; base < hi; base += RPB) {
+        const int64_t row = base + threadIdx.x / LPR;
+        if (row < hi) {
+            double2 s = row_dot<LPR>((int)row, a.ptr, a.col, a.val, uj, lane);
+            if (lane == 0) {
+                double2 wr = cmul(c1, s);
+                if (j > 0) {
+                    const double2 p = up[row];
+                    wr.x += c2 * p.x;
+                    wr.y += c2 * p.y;
+                }
+                w[row] = wr;
+                const double2 u = uj[row];
+                dot.x += u.x * wr.x + u.y * wr.y;
+                dot.y += u.x * wr.y - u.y * wr.x;
+            }
+        }
+    }
+    double2 bs = block_sum2<kBlock>(dot, sh);
+    if (threadIdx.x == 0) a.partials[blockIdx.x] = bs;
+    if (last_block_arrive(a.ticket)) {
+        double2 tot = reduce_partials<kBlock>(a.partials, gridDim.x, 1, sh);
+        if (threadIdx.x == 0) {
+            ks->a[j] = make_double2(sj * tot.x, sj * tot.y);
+            *a.ticket = 0;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void lanczos_project_kernel(LanczosArgs a) {
+    KrylovScalars* ks = a.ks;
+    if (ks->broken) return;
+    __shared__ double2 sh[kBlock / 64];
+    const int j = a.j;
+    const double sj = ks->s[j];
+    const double2 aj = ks->a[j];
+    const double2 c3 = make_double2(aj.x * sj, aj.y * sj);
+    const double2* __restrict__ uj = a.U + (size_t)j * a.ld;
+    double2* __restrict__ w = a.U + (size_t)(j + 1) * a.ld;
+    const bool store = j + 1 < a.m;
+    int64_t lo, hi;
+    block_rows(a.n, lo, hi);
+    double nrm = 0.0;
+    for (int64_t row = lo + threadIdx.x; row < hi; row += kBlock) {
+        const double2 u = uj[row];
+        double2 wr = w[row];
+        wr.x -= c3.x * u.x - c3.y * u.y;
+        wr.y -= c3.x * u.y + c3.y * u.x;
+        if (store) w[row] = wr;
+        nrm += wr.x * wr.x + wr.y * wr.y;
+    }
+    double2 bs = block_sum2<kBlock>(make_double2(nrm, 0.0), sh);
+    if (threadIdx.x == 0) a.partials[blockIdx.x] = bs;
+    if (last_block_arrive(a.ticket)) {
+        double2 tot = reduce_partials<kBlock>(a.partials, gridDim.x, 1, sh);
+        if (threadIdx.x == 0) {
+            const double beta = sqrt(tot.x);
+            if (beta < a.tolRate) {            // lucky breakdown (core/krylovTimeEvolver.cpp:551-555)
+                ks->broken = 1;
+                ks->m_hbd = j + 1;
+                ks->h = beta;
+            } else if (j + 1 == a.m) {
+                ks->h = beta;                  // :566-567
+            } else {
+                ks->beta[j] = beta;            // :558-565
+                ks->s[j + 1] = 1.0 / beta;
+            }
+            *a.ticket = 0;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void zdotc_kernel(const double2* __restrict__ x, const double2* __restrict__ y,
+                                                       int64_t n, double2* partials, unsigned int* ticket,
+                                                       double2* result, int slot, int mode) {
+    __shared__ double2 sh[kBlock / 64];
+    int64_t lo, hi;
+    block_rows(n, lo, hi);
+    double2 acc = make_double2(0.0, 0.0);
+    if (mode == 0) {
+        for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+            const double2 u = x[i], v = y[i];
+            acc.x += u.x * v.x + u.y * v.y;
+            acc.y += u.x * v.y - u.y * v.x;
+        }
+    } else {
+        for (int64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+            const double2 u = x[i];
+            acc.x += u.x * u.x + u.y * u.y;
+        }
+    }
+    double2 bs = block_sum2<kBlock>(acc, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = bs;
+    if (last_block_arrive(ticket)) {
+        double2 tot = reduce_partials<kBlock>(partials, gridDim.x, 1, sh);
+        if (threadIdx.x == 0) {
+            result[slot] = tot;
+            *ticket = 0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------------------
+#define TE_LPR_SWITCH(lpr, MACRO) \
+    switch (lpr) {                \
+        case 1: MACRO(1); break;  \
+        case 2: MACRO(2); break;  \
+        case 4: MACRO(4); break;  \
+        case 8: MACRO(8); break;  \
+        case 16: MACRO(16); break;\
+        case 32: MACRO(32); break;\
+        default: MACRO(64); break;\
+    }
+
+void launch_spmv(const te_matrix* A, double2 alpha, const double2* x, double2* y, hipStream_t s) {
+    const int grid = A->shape.grid;
+#define L_(LPR) hipLaunchKernelGGL(spmv_kernel<LPR>, dim3(grid), dim3(kBlock), 0, s, (int64_t)A->n_rows, \
+                                   A->row_ptr, A->col, A->val, x, y, alpha)
+    TE_LPR_SWITCH(A->shape.lanes_per_row, L_)
+#undef L_
+}
+
+void launch_lanczos_apply(const LanczosArgs& a, int lpr, int grid, hipStream_t s) {
+#define L_(LPR) hipLaunchKernelGGL(lanczos_apply_kernel<LPR>, dim3(grid), dim3(kBlock), 0, s, a)
+    TE_LPR_SWITCH(lpr, L_)
+#undef L_
+}
+
+void launch_lanczos_project(const LanczosArgs& a, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(lanczos_project_kernel, dim3(grid), dim3(kBlock), 0, s, a);
+}
+
+void launch_zdotc(const double2* x, const double2* y, int64_t n, ReduceScratch& rs, int slot, hipStream_t s) {
+    hipLaunchKernelGGL(zdotc_kernel, dim3(rs.grid), dim3(kBlock), 0, s, x, y, n, rs.partials, rs.ticket, rs.result,
+                       slot, 0);
+}
+
+void launch_nrm2sq(const double2* x, int64_t n, ReduceScratch& rs, int slot, hipStream_t s) {
+    hipLaunchKernelGGL(zdotc_kernel, dim3(rs.grid), dim3(kBlock), 0, s, x, x, n, rs.partials, rs.ticket, rs.result,
+                       slot, 1);
+}
+
+int reduce_scratch_alloc(te_ctx* ctx, ReduceScratch& rs, int grid) {
+    rs.grid = grid;
+    TE_HIP(ctx, hipMalloc(&rs.partials, sizeof(double2) * (size_t)grid * 64));
+    TE_HIP(ctx, hipMalloc(&rs.ticket, sizeof(unsigned int) * 4));
+    TE_HIP(ctx, hipMalloc(&rs.result, sizeof(double2) * 64));
+    TE_HIP(ctx, hipMemset(rs.ticket, 0, sizeof(unsigned int) * 4));
+    TE_HIP(ctx, hipMemset(rs.result, 0, sizeof(double2) * 64));
+    return TE_OK;
+}
+
+void reduce_scratch_free(ReduceScratch& rs) {
+    if (rs.partials) hipFree(rs.partials);
+    if (rs.ticket) hipFree(rs.ticket);
+    if (rs.result) hipFree(rs.result);
+    rs = ReduceScratch{};
+}
+
+}  // namespace te
