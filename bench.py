@@ -1,0 +1,176 @@
+#!/usr/bin/env python3
+"""Time-to-T of TimeEvolver's Krylov evolution on one MI355X, with the SpMV roofline.
+
+Workload (default): ring Bose-Hubbard, L = N = 14 modes/particles, D = 20,058,300
+occupation states, nnz = 311,275,100 complex128 CSR entries, seeded couplings
+(eps_k ~ U[0,1], J_k ~ U[-1,1], U = 0.5), initial Fock state |14,0,...,0>, evolved to
+T = 10 with Krylov dimension m = 40, tol = 1e-8, sampling step 0.5, observables n_k for all
+14 modes. Basis, Hamiltonian and observables are built on the device (timed separately);
+a "step" is one complete timeEvolve() from psi0 to T with all inputs resident in HBM.
+
+Prints one JSON line. `value` is the time-to-T in seconds (lower is better). `roofline` is
+the fused Lanczos SpMV kernel (te::lanczos_apply_kernel): algorithmic bytes per launch
+20 nnz + 4 (D+1) (CSR with int32 indices) + 16 D (x) + 16 D (w written) + 16 D (v_{j-1})
++ 16 D (v_j for <v_j, w>), divided by its mean duration from HIP events recorded around
+every launch on the library's stream during the timed runs. `cpu_baseline` times the
+oracle's CPU restatement of the reference evolver on this host, on a bounded sample (the
+first Krylov step of the same workload) and extrapolates to the full time-to-T.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+METRIC = "time-to-T + SpMV HBM GB/s at dim 2e7, tol 1e-8; batched/adjoint vs plain"
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1, help="timed time-to-T runs")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--size", type=int, default=14, help="L = N of the ring (14 = the 2e7 workload)")
+    ap.add_argument("--T", type=float, default=10.0)
+    ap.add_argument("--sampling-step", type=float, default=0.5)
+    ap.add_argument("--m", type=int, default=40)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=1, help="Krylov steps the CPU baseline runs")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (sharding is described in "
+              "DESIGN.md); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    from timeevolver_amd import capi
+    from timeevolver_amd.hamiltonian import as_tuples, createNumberOperator
+    from timeevolver_amd.workloads import (fock_state, ring_bose_hubbard_terms, ring_couplings, ring_dimension,
+                                           ring_nnz)
+
+    L = N = args.size
+    D = ring_dimension(L, N)
+    eps, J = ring_couplings(L, args.seed)
+    terms = as_tuples(ring_bose_hubbard_terms(L, eps, J, 0.5))
+    obs_terms = [as_tuples([createNumberOperator(k, 1)]) for k in range(L)]
+
+    ctx = capi.Context(0)
+    log("building basis/H/observables on the device: L=N=%d D=%d" % (L, D))
+    t0 = time.perf_counter()
+    basis = capi.Basis.basis(ctx, N, L)
+    H = capi.operator_build(ctx, basis, terms)
+    obs = [capi.operator_build(ctx, basis, t) for t in obs_terms]
+    ctx.synchronize()
+    build_s = time.perf_counter() - t0
+    assert basis.D == D and H.nnz == ring_nnz(L, N), (basis.D, H.nnz)
+    log("built in %.2f s: nnz=%d" % (build_s, H.nnz))
+
+    psi0 = fock_state(D)
+    ev = capi.Evolver(ctx, H, psi0, args.T, args.sampling_step, obs, tol=args.tol, m=args.m)
+    del psi0
+    ev.set_profiling(True)
+    for w in range(args.warmup):
+        t = time.perf_counter()
+        r = ev.run(want_state=False)
+        log("warmup %d: %.3f s, n_steps=%d err=%.3e status=%d" % (w, time.perf_counter() - t, r["n_steps"], r["err"],
+                                                                  r["statusCode"]))
+    times, kts = [], []
+    for k in range(args.steps):
+        ctx.synchronize()
+        t = time.perf_counter()
+        r = ev.run(want_state=False)
+        ctx.synchronize()
+        times.append(time.perf_counter() - t)
+        kts.append(ev.kernel_times())
+        log("step %d: %.3f s, n_steps=%d err=%.3e status=%d" % (k, times[-1], r["n_steps"], r["err"], r["statusCode"]))
+    time_to_T = float(np.mean(times))
+
+    # roofline of the fused Lanczos SpMV
+    apply_ms = sum(kt["apply"][0] for kt in kts)
+    apply_n = sum(kt["apply"][1] for kt in kts)
+    t_apply = apply_ms / 1e3 / max(apply_n, 1)
+    b_spmv = 20 * H.nnz + 4 * (D + 1) + 32 * D
+    b_apply = b_spmv + 32 * D
+    achieved = b_apply / t_apply / 1e9
+    kernel_ms = {k: round(sum(kt[k][0] for kt in kts) / len(kts), 3) for k in kts[0]}
+    kernel_launches = {k: kts[0][k][1] for k in kts[0]}
+
+    line = {
+        "metric": METRIC,
+        "value": round(time_to_T, 4),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(time_to_T * 1e3, 2),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "complex128",
+        "data": "synthetic (seeded ring Bose-Hubbard couplings, Fock initial state)",
+        "config": {"workload": "ring Bose-Hubbard L=N=%d, D=%d, nnz=%d, T=%g, m=%d, tol=%g, sampling step %g, "
+                               "observables n_k x %d" % (L, D, H.nnz, args.T, args.m, args.tol, args.sampling_step, L),
+                   "dim": D, "nnz": H.nnz, "T": args.T, "m": args.m, "tol": args.tol,
+                   "sampling_step": args.sampling_step},
+        "n_krylov_steps": r["n_steps"],
+        "err": r["err"],
+        "status_code": r["statusCode"],
+        "build_s": round(build_s, 3),
+        "spmv_GBps": round(b_spmv / t_apply / 1e9, 1),
+        "kernel_ms_per_run": kernel_ms,
+        "kernel_launches_per_run": kernel_launches,
+        "roofline": {"kernel": "lanczos_apply_kernel (fused SpMV + axpy + dot)", "bound": "hbm",
+                     "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "bytes_per_launch": b_apply, "mean_launch_us": round(t_apply * 1e6, 2),
+                     "launches_timed": apply_n},
+    }
+
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(np, capi, H, obs, D, args, r["n_steps"])
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(np, capi, H, obs, D, args, gpu_steps):
+    """The oracle (CPU restatement of core/krylovTimeEvolver.cpp) on this host's cores, on the
+    first `cpu_steps` Krylov steps of the same workload, extrapolated to all GPU steps."""
+    import oracle
+    from timeevolver_amd.workloads import fock_state
+
+    log("cpu baseline: copying CSR to host")
+    ptr, col, val = H.to_csr()
+    diags = []
+    for o in obs:
+        p, c, v = o.to_csr()
+        d = np.zeros(D)
+        d[c] = v.real        # diagonal observables: one entry per row
+        diags.append(d)
+    threads = oracle.get_threads()
+    log("cpu baseline: %d Krylov step(s) on %d threads" % (args.cpu_steps, threads))
+    t = time.perf_counter()
+    r = oracle.evolve_csr(ptr, col, val, fock_state(D), args.T, args.sampling_step, diags, tol=args.tol, m=args.m,
+                          max_steps=args.cpu_steps)
+    dt = time.perf_counter() - t
+    per_step = dt / max(r["n_steps"], 1)
+    log("cpu baseline: %.2f s for %d step(s)" % (dt, r["n_steps"]))
+    return {"value": round(per_step * gpu_steps, 2), "unit": "s", "cores": threads, "kind": "port",
+            "sample": "first %d Krylov step(s) (m=%d Lanczos iterations + t=0 sample of %d observables) of the same "
+                      "workload took %.2f s on %d host threads; time-to-T extrapolated x %d Krylov steps"
+                      % (r["n_steps"], args.m, len(diags), dt, threads, gpu_steps),
+            "seconds_per_krylov_step": round(per_step, 3)}
+
+
+if __name__ == "__main__":
+    main()

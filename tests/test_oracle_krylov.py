@@ -151,3 +151,20 @@ def test_sampling_arithmetic(oracle_lib):
     b, eps, J, H, r = _ring_run(oracle_lib, L, N, fock_state, 10.0, 0.1)
     assert r["n_samples"] == 101 and r["numSamples"] == 101
     assert r["evolvedTime"] >= 10.0 - 1e-12
+
+
+def test_evolve_csr_entry_matches_coo_entry(oracle_lib):
+    """The CSR/diagonal entry point used by the CPU baseline gives the COO entry's results."""
+    L, N = 5, 5
+    b, eps, J, H, r = _ring_run(oracle_lib, L, N, fock_state, 1.0, 0.1)
+    vals, cols, rows = H
+    ptr = np.zeros(b.D + 1, dtype=np.int64)
+    np.add.at(ptr, rows + 1, 1)
+    ptr = np.cumsum(ptr)
+    S = b.states()
+    diags = [np.sqrt(S[:, k]) * np.sqrt(S[:, k]) for k in range(L)]
+    r2 = oracle_lib.evolve_csr(ptr, cols.astype(np.int32), vals, fock_state(b.D), 1.0, 0.1, diags, tol=1e-8,
+                               want_state=True)
+    assert r2["n_steps"] == r["n_steps"] and r2["statusCode"] == r["statusCode"]
+    assert np.array_equal(r2["state"], r["state"])
+    assert np.abs(r2["observables"] - r["observables"]).max() < 1e-13

@@ -66,8 +66,10 @@ namespace te {
 // ---------------------------------------------------------------------------------------
 __device__ __forceinline__ int64_t rank_factor(const FactorDev& f, const int* e) {
     const int K = f.K;
-    int64_t r = 0;
     int tot = 0;
+    for (int p = 1; p < K; p++) tot += e[p];
+    if (e[0] != f.N - tot) return -1;                  // particle number (e_0 = N - sum)
+    int64_t r = 0;
     for (int p = K - 1; p >= 1; p--) {
         const int x = e[p];
         if (x == 0) continue;
@@ -76,9 +78,7 @@ __device__ __forceinline__ int64_t rank_factor(const FactorDev& f, const int* e)
         const int64_t o0 = f.off[p + 1][r], o1 = f.off[p + 1][r + 1];
         if (c > o1 - o0) return -1;
         r = f.size[p + 1] + o0 + c - 1;
-        tot += x;
     }
-    if (K >= 1 && e[0] != f.N - tot) return -1;
     if (f.keep) {
         const int64_t k0 = f.keep[r], k1 = f.keep[r + 1];
         if (k1 == k0) return -1;
@@ -345,7 +345,9 @@ __device__ int build_row(const BasisDev& B, const TermsDev& T, int64_t j, double
         const double2 tc = T.coef[t];                             // scalarMultiplication (:308-311)
         const double vr = cr * tc.x - ci * tc.y;
         const double vi = cr * tc.y + ci * tc.x;
-        const int64_t idx = rank_basis(B, w);
+        bool same = true;                                          // image == source: its own index
+        for (int k = 0; k < K; k++) same = same && (w[k] == e[k]);
+        const int64_t idx = same ? j : rank_basis(B, w);
         if (idx < 0) continue;                                     // not in the hash table (:360-369)
         int k = 0;
         while (k < n_ent && cols[k] != idx) k++;
