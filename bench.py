@@ -47,7 +47,12 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1, help="Krylov steps the CPU baseline runs")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (HBM counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (sharding is described in "
               "DESIGN.md); run with --gpus 1", file=sys.stderr)
@@ -141,6 +146,19 @@ def main():
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(np, capi, H, obs, D, args, r["n_steps"])
     print(json.dumps(line), flush=True)
+
+
+def copy_calibration():
+    """A plain copy of 1 GiB on the device, repeated 3 times: rocprofv3's FETCH_SIZE/WRITE_SIZE for
+    this kernel give the counter-to-bytes factor used for the SpMV traffic figure."""
+    import torch
+    n = (1 << 30) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "repeats": 3}), flush=True)
 
 
 def cpu_baseline(np, capi, H, obs, D, args, gpu_steps):
