@@ -749,7 +749,7 @@ extern "C" int te_operator_build(te_ctx* ctx, const te_basis* B, const te_op_ter
         else if (hipStreamSynchronize(s) != hipSuccess) rc = fail(ctx, TE_ERR_HIP, "operator build failed");
     }
     if (rc == TE_OK) rc = finalize_matrix(ctx, A);
-    if (rc == TE_OK) A->shape = choose_shape(ctx, A->n_rows, A->nnz);
+    if (rc == TE_OK) rc = prepare_spmv(ctx, A);
     void* frees[] = {d_off, d_ops, d_modes, d_caps, d_coef, d_sqrt, ws, counts, rp64};
     for (void* p : frees)
         if (p) hipFree(p);

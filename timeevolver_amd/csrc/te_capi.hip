@@ -209,8 +209,7 @@ int matrix_upload(te_ctx* ctx, te_matrix* A, const std::vector<int>& ptr, const 
         TE_HIP(ctx, hipMalloc(&A->diag_re, sizeof(double) * std::max<int64_t>(A->n_rows, 1)));
         TE_HIP(ctx, hipMemcpy(A->diag_re, d.data(), sizeof(double) * A->n_rows, hipMemcpyHostToDevice));
     }
-    A->shape = choose_shape(ctx, A->n_rows, A->nnz);
-    return TE_OK;
+    return prepare_spmv(ctx, A);
 }
 
 }  // namespace te
@@ -270,6 +269,7 @@ extern "C" void te_matrix_destroy(te_matrix* A) {
     if (A->col) hipFree(A->col);
     if (A->val) hipFree(A->val);
     if (A->diag_re) hipFree(A->diag_re);
+    if (A->row_block) hipFree(A->row_block);
     delete A;
 }
 
@@ -803,6 +803,8 @@ extern "C" int te_evolver_run(te_evolver* ev, double* obs_values, double* evolve
     int m = ev->m0;
     LanczosArgs la{};
     la.n = n;
+    la.row_block = ev->H->row_block;
+    la.n_row_blocks = ev->H->n_row_blocks;
     la.ptr = ev->H->row_ptr;
     la.col = ev->H->col;
     la.val = ev->H->val;
@@ -843,7 +845,7 @@ extern "C" int te_evolver_run(te_evolver* ev, double* obs_values, double* evolve
         for (int j = 0; j < m; j++) {                                               // :338
             la.j = j;
             prof_begin(ev, 0);
-            launch_lanczos_apply(la, ev->H->shape.lanes_per_row, ev->H->shape.grid, s);
+            launch_lanczos_apply(la, ev->H->shape, s);
             prof_end(ev, 0);
             prof_begin(ev, 1);
             launch_lanczos_project(la, ev->grid_stream, s);

@@ -30,9 +30,16 @@ int fail(te_ctx* ctx, int code, const char* fmt, ...);
     } while (0)
 
 struct SpmvShape {
+    int kind;             // 0: LPR lanes per row; 1: row blocks of ~kStreamChunk entries staged in LDS
     int lanes_per_row;    // lanes of a wavefront cooperating on one row (1..64, power of 2)
-    int grid;             // blocks of the grid-stride launch
+    int grid;             // blocks of the launch
 };
+
+// Row-block ("CSR-stream") SpMV: block b holds the rows whose first entry lies in
+// [b * kStreamChunk, (b+1) * kStreamChunk); its entries (at most kStreamChunk + longest row - 1)
+// are multiplied into LDS with every load issued up front, then reduced row by row.
+constexpr int kStreamChunk = 2048;
+constexpr int kStreamCap = kStreamChunk + 256;     // LDS entries per block (rows up to 257 long)
 
 }  // namespace te
 
@@ -54,13 +61,19 @@ struct te_matrix {
     bool norm1_valid = false;
     bool diagonal = false;      // every entry on the diagonal (observables of n_k type)
     double* diag_re = nullptr;  // device, n_rows: Re of the diagonal when `diagonal`
-    te::SpmvShape shape{8, 1024};
+    te::SpmvShape shape{0, 8, 1024};
+    int* row_block = nullptr;   // device, n_row_blocks + 1 first rows of the stream row blocks
+    int n_row_blocks = 0;
+    int max_row_len = 0;
 };
 
 namespace te {
 
 // SpMV launch shape for a matrix (lanes per row from the mean row length, grid from the CUs).
 SpmvShape choose_shape(const te_ctx* ctx, int64_t n_rows, int64_t nnz);
+// Derives the row blocks of the stream kernel on the device and picks the kernel (call after
+// row_ptr is on the device).
+int prepare_spmv(te_ctx* ctx, te_matrix* A);
 
 // ---- kernels launched across translation units (te_spmv.hip) ----
 void launch_spmv(const te_matrix* A, double2 alpha, const double2* x, double2* y, hipStream_t s);

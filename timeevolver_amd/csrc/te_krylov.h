@@ -18,6 +18,8 @@ struct KrylovScalars {
 
 struct LanczosArgs {
     int64_t n;
+    const int* row_block;   // stream kernel row blocks (n_row_blocks + 1)
+    int n_row_blocks;
     const int* ptr;
     const int* col;
     const double2* val;
@@ -89,7 +91,7 @@ struct CombineArgs {
     unsigned int* ticket;
 };
 
-void launch_lanczos_apply(const LanczosArgs& a, int lpr, int grid, hipStream_t s);
+void launch_lanczos_apply(const LanczosArgs& a, const SpmvShape& sh, hipStream_t s);
 void launch_lanczos_project(const LanczosArgs& a, int grid, hipStream_t s);
 void launch_step_control(const StepConsts& c, StepState* st, KrylovScalars* ks, hipStream_t s);
 // Picks the sample/observable chunking; S <= max_samples_per_launch(nobs).

@@ -19,6 +19,9 @@
 #include "te_krylov.h"
 #include "te_reduce.h"
 
+#include <algorithm>
+#include <cstdlib>
+
 namespace te {
 
 // ---------------------------------------------------------------------------------------
@@ -130,7 +133,24 @@ __global__ __launch_bounds__(kBlock) void lanczos_project_kernel(LanczosArgs a) 
     int64_t lo, hi;
     block_rows(a.n, lo, hi);
     double nrm = 0.0;
-    for (int64_t row = lo + threadIdx.x; row < hi; row += kBlock) {
+    int64_t row = lo + threadIdx.x;
+    // four rows per thread in flight: all eight loads are issued before any is used
+    for (; row + 3 * kBlock < hi; row += 4 * kBlock) {
+        double2 u[4], wr[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            u[q] = uj[row + q * kBlock];
+            wr[q] = w[row + q * kBlock];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            wr[q].x -= c3.x * u[q].x - c3.y * u[q].y;
+            wr[q].y -= c3.x * u[q].y + c3.y * u[q].x;
+            if (store) w[row + q * kBlock] = wr[q];
+            nrm += wr[q].x * wr[q].x + wr[q].y * wr[q].y;
+        }
+    }
+    for (; row < hi; row += kBlock) {
         const double2 u = uj[row];
         double2 wr = w[row];
         wr.x -= c3.x * u.x - c3.y * u.y;
@@ -155,6 +175,155 @@ __global__ __launch_bounds__(kBlock) void lanczos_project_kernel(LanczosArgs a) 
                 ks->s[j + 1] = 1.0 / beta;
             }
             *a.ticket = 0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Row-block ("CSR-stream") kernels. Each block walks a contiguous range of row blocks; for one
+// row block every thread issues all of its value, index and x loads (kStreamCap / kBlock of
+// each) before using any, the products go to LDS, and thread t then sums rows r0 + t, ... in
+// entry order.
+// ---------------------------------------------------------------------------------------
+constexpr int kNI = kStreamCap / kBlock;
+typedef double d2v __attribute__((ext_vector_type(2)));
+
+// Streamed once per SpMV: loaded non-temporally so the gathered x keeps its place in L2/MALL.
+__device__ __forceinline__ double2 load_stream(const double2* p) {
+    const d2v t = __builtin_nontemporal_load(reinterpret_cast<const d2v*>(p));
+    return make_double2(t.x, t.y);
+}
+static_assert(kStreamCap % kBlock == 0, "stream capacity must be a multiple of the block size");
+
+template <class Epi>
+__device__ __forceinline__ void stream_rows(const int* __restrict__ rb, int nrb, const int* __restrict__ ptr,
+                                            const int* __restrict__ col, const double2* __restrict__ val,
+                                            const double2* __restrict__ x, double2* prod, Epi& epi) {
+    const int per = (nrb + gridDim.x - 1) / gridDim.x;
+    const int lb = xcd_block_id();
+    const int b0 = lb * per;
+    const int b1 = b0 + per < nrb ? b0 + per : nrb;
+    const int t = threadIdx.x;
+    for (int b = b0; b < b1; b++) {
+        const int r0 = rb[b], r1 = rb[b + 1];
+        const int k0 = ptr[r0];
+        const int nk = ptr[r1] - k0;
+        double2 v[kNI], xv[kNI];
+        int c[kNI];
+#pragma unroll
+        for (int i = 0; i < kNI; i++) {
+            const int k = t + i * kBlock;
+            if (k < nk) {
+                v[i] = load_stream(&val[k0 + k]);
+                c[i] = __builtin_nontemporal_load(&col[k0 + k]);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < kNI; i++) {
+            const int k = t + i * kBlock;
+            if (k < nk) xv[i] = x[c[i]];
+        }
+#pragma unroll
+        for (int i = 0; i < kNI; i++) {
+            const int k = t + i * kBlock;
+            if (k < nk) prod[k] = cmul(v[i], xv[i]);
+        }
+        __syncthreads();
+        for (int r = r0 + t; r < r1; r += kBlock) {
+            const int a = ptr[r] - k0, e = ptr[r + 1] - k0;
+            double sr = 0.0, si = 0.0;
+            for (int k = a; k < e; k++) {
+                sr += prod[k].x;
+                si += prod[k].y;
+            }
+            epi.row(r, make_double2(sr, si));
+        }
+        __syncthreads();
+    }
+}
+
+struct SpmvEpi {
+    double2 alpha;
+    double2* y;
+    __device__ __forceinline__ void row(int r, double2 s) { y[r] = cmul(alpha, s); }
+};
+
+__global__ __launch_bounds__(kBlock) void spmv_stream_kernel(const int* __restrict__ rb, int nrb,
+                                                             const int* __restrict__ ptr, const int* __restrict__ col,
+                                                             const double2* __restrict__ val,
+                                                             const double2* __restrict__ x, double2* __restrict__ y,
+                                                             double2 alpha) {
+    __shared__ double2 prod[kStreamCap];
+    SpmvEpi epi{alpha, y};
+    stream_rows(rb, nrb, ptr, col, val, x, prod, epi);
+}
+
+struct ApplyEpi {
+    double2 c1;
+    double c2;
+    const double2* up;
+    const double2* uj;
+    double2* w;
+    double2 dot;
+    __device__ __forceinline__ void row(int r, double2 s) {
+        double2 wr = cmul(c1, s);
+        if (up) {
+            const double2 p = up[r];
+            wr.x += c2 * p.x;
+            wr.y += c2 * p.y;
+        }
+        w[r] = wr;
+        const double2 u = uj[r];
+        dot.x += u.x * wr.x + u.y * wr.y;
+        dot.y += u.x * wr.y - u.y * wr.x;
+    }
+};
+
+__global__ __launch_bounds__(kBlock) void lanczos_apply_stream_kernel(LanczosArgs a) {
+    KrylovScalars* ks = a.ks;
+    if (ks->broken) return;
+    __shared__ double2 prod[kStreamCap];
+    __shared__ double2 sh[kBlock / 64];
+    const int j = a.j;
+    const double sj = ks->s[j];
+    ApplyEpi epi;
+    epi.c1 = make_double2(a.alpha.x * sj, a.alpha.y * sj);
+    epi.c2 = j > 0 ? ks->beta[j - 1] * ks->s[j - 1] : 0.0;
+    epi.up = j > 0 ? a.U + (size_t)(j - 1) * a.ld : nullptr;
+    epi.uj = a.U + (size_t)j * a.ld;
+    epi.w = a.U + (size_t)(j + 1) * a.ld;
+    epi.dot = make_double2(0.0, 0.0);
+    stream_rows(a.row_block, a.n_row_blocks, a.ptr, a.col, a.val, epi.uj, prod, epi);
+    double2 bs = block_sum2<kBlock>(epi.dot, sh);
+    if (threadIdx.x == 0) a.partials[blockIdx.x] = bs;
+    if (last_block_arrive(a.ticket)) {
+        double2 tot = reduce_partials<kBlock>(a.partials, gridDim.x, 1, sh);
+        if (threadIdx.x == 0) {
+            ks->a[j] = make_double2(sj * tot.x, sj * tot.y);
+            *a.ticket = 0;
+        }
+    }
+}
+
+// rb[b] = first row whose first entry is >= b * kStreamChunk (rb[nrb] = n); max row length.
+__global__ void row_blocks_kernel(const int* __restrict__ ptr, int64_t n, int nrb, int* rb, int* maxlen) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const int len = ptr[i + 1] - ptr[i];
+        atomicMax(maxlen, len);
+    }
+    if (i <= nrb) {
+        if (i == nrb) {
+            rb[i] = (int)n;
+        } else {
+            const int64_t target = i * (int64_t)kStreamChunk;
+            int64_t lo = 0, hi = n;                 // first r in [0, n] with ptr[r] >= target
+            while (lo < hi) {
+                const int64_t mid = (lo + hi) / 2;
+                if (ptr[mid] >= target) hi = mid;
+                else lo = mid + 1;
+            }
+            rb[i] = (int)lo;
         }
     }
 }
@@ -205,16 +374,54 @@ __global__ __launch_bounds__(kBlock) void zdotc_kernel(const double2* __restrict
 
 void launch_spmv(const te_matrix* A, double2 alpha, const double2* x, double2* y, hipStream_t s) {
     const int grid = A->shape.grid;
+    if (A->shape.kind == 1) {
+        hipLaunchKernelGGL(spmv_stream_kernel, dim3(grid), dim3(kBlock), 0, s, A->row_block, A->n_row_blocks,
+                           A->row_ptr, A->col, A->val, x, y, alpha);
+        return;
+    }
 #define L_(LPR) hipLaunchKernelGGL(spmv_kernel<LPR>, dim3(grid), dim3(kBlock), 0, s, (int64_t)A->n_rows, \
                                    A->row_ptr, A->col, A->val, x, y, alpha)
     TE_LPR_SWITCH(A->shape.lanes_per_row, L_)
 #undef L_
 }
 
-void launch_lanczos_apply(const LanczosArgs& a, int lpr, int grid, hipStream_t s) {
+void launch_lanczos_apply(const LanczosArgs& a, const SpmvShape& sh, hipStream_t s) {
+    const int grid = sh.grid;
+    if (sh.kind == 1) {
+        hipLaunchKernelGGL(lanczos_apply_stream_kernel, dim3(grid), dim3(kBlock), 0, s, a);
+        return;
+    }
 #define L_(LPR) hipLaunchKernelGGL(lanczos_apply_kernel<LPR>, dim3(grid), dim3(kBlock), 0, s, a)
-    TE_LPR_SWITCH(lpr, L_)
+    TE_LPR_SWITCH(sh.lanes_per_row, L_)
 #undef L_
+}
+
+int prepare_spmv(te_ctx* ctx, te_matrix* A) {
+    A->shape = choose_shape(ctx, A->n_rows, A->nnz);
+    const int nrb = (int)((A->nnz + kStreamChunk - 1) / kStreamChunk);
+    if (A->row_block) hipFree(A->row_block);
+    A->row_block = nullptr;
+    TE_HIP(ctx, hipMalloc(&A->row_block, sizeof(int) * (nrb + 1)));
+    int* d_max = nullptr;
+    TE_HIP(ctx, hipMalloc(&d_max, sizeof(int)));
+    TE_HIP(ctx, hipMemsetAsync(d_max, 0, sizeof(int), ctx->stream));
+    const int64_t items = std::max<int64_t>(A->n_rows, nrb + 1);
+    hipLaunchKernelGGL(row_blocks_kernel, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, ctx->stream, A->row_ptr,
+                       (int64_t)A->n_rows, nrb, A->row_block, d_max);
+    int mx = 0;
+    TE_HIP(ctx, hipMemcpyAsync(&mx, d_max, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    TE_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    hipFree(d_max);
+    A->n_row_blocks = nrb;
+    A->max_row_len = mx;
+    const char* force = std::getenv("TE_SPMV_KIND");
+    const bool fits = mx + kStreamChunk - 1 <= kStreamCap && nrb > 0;
+    if (fits && !(force && force[0] == '0')) {
+        A->shape.kind = 1;
+        const int cap = std::max(8, ctx->num_cus * 8);
+        A->shape.grid = (int)((std::min(nrb, cap) + 7) / 8 * 8);
+    }
+    return TE_OK;
 }
 
 void launch_lanczos_project(const LanczosArgs& a, int grid, hipStream_t s) {
