@@ -248,3 +248,28 @@ def test_fatal_unnormalised(gpu, oracle_lib):
     with pytest.raises(capi.TEError) as e:
         capi.Evolver(gpu, H, 2 * fock_state(ob.D), 1.0, 0.1, [], tol=1e-8)
     assert e.value.code == capi.TE_ERR_FATAL
+
+
+@pytest.mark.parametrize("part_mb", ["0.04", "0.02"])
+def test_column_split_spmv_and_evolution(gpu, oracle_lib, monkeypatch, part_mb):
+    """The column-split SpMV (x slices of at most TE_SPMV_PART_MB) gives the unsplit results:
+    3 and 5 slices of the L = N = 8 ring (x = 103 KB)."""
+    L = N = 8
+    terms = as_tuples(_ring_terms(L))
+    ob = oracle_lib.Basis.basis(N, L)
+    v, c, r = oracle_lib.build_operator(ob, terms)
+    monkeypatch.setenv("TE_SPMV_PART_MB", part_mb)
+    A = capi.Matrix.from_coo(gpu, v, c, r, ob.D)
+    gb = capi.Basis.basis(gpu, N, L)
+    Hg = capi.operator_build(gpu, gb, terms)
+    monkeypatch.delenv("TE_SPMV_PART_MB")
+    x = random_state(ob.D, 5)
+    for M in (A, Hg):
+        y = M.spmv(x, -0.3j)
+        yr = oracle_lib.spmv(ob.D, v, c, r, x, -0.3j)
+        assert np.linalg.norm(y - yr) <= 1e-14 * np.linalg.norm(yr)
+    obs = [capi.operator_build(gpu, gb, [createNumberOperator(k, 1)]) for k in range(L)]
+    rg = capi.Evolver(gpu, Hg, fock_state(ob.D), 3.0, 0.1, obs, tol=1e-8).run()
+    oobs = [("sparse", oracle_lib.build_operator(ob, as_tuples([createNumberOperator(k, 1)]))) for k in range(L)]
+    ro = oracle_lib.evolve(ob.D, (v, c, r), fock_state(ob.D), 3.0, 0.1, oobs, tol=1e-8)
+    _check_pair(ro, rg, N)

@@ -352,7 +352,8 @@ class Evolver:
     def kernel_times(self):
         """{kind: (total_ms, launches)} of the last run (0 apply, 1 project, 2 control, 3 combine)."""
         out = {}
-        for k, name in enumerate(("apply", "project", "control", "combine")):
+        for k, name in enumerate(("apply", "project", "control", "combine", "ctl_eigen", "ctl_search",
+                                   "ctl_samples")):
             ms, n = _c.c_double(), _c.c_size_t()
             self.ctx.check(self.ctx.L.te_evolver_kernel_time(self.h, k, _c.byref(ms), _c.byref(n)), "kernel_time")
             out[name] = (ms.value, n.value)

@@ -270,6 +270,7 @@ extern "C" void te_matrix_destroy(te_matrix* A) {
     if (A->val) hipFree(A->val);
     if (A->diag_re) hipFree(A->diag_re);
     if (A->row_block) hipFree(A->row_block);
+    free_parts(A);
     delete A;
 }
 
@@ -845,7 +846,7 @@ extern "C" int te_evolver_run(te_evolver* ev, double* obs_values, double* evolve
         for (int j = 0; j < m; j++) {                                               // :338
             la.j = j;
             prof_begin(ev, 0);
-            launch_lanczos_apply(la, ev->H->shape, s);
+            launch_lanczos_apply(la, ev->H, s);
             prof_end(ev, 0);
             prof_begin(ev, 1);
             launch_lanczos_project(la, ev->grid_stream, s);
@@ -977,7 +978,14 @@ extern "C" int te_evolver_set_profiling(te_evolver* ev, int enable) {
 }
 
 extern "C" int te_evolver_kernel_time(const te_evolver* ev, int kind, double* total_ms, size_t* launches) {
-    if (!ev || kind < 0 || kind > 3 || !total_ms || !launches) return TE_ERR_ARG;
+    if (!ev || kind < 0 || kind > 6 || !total_ms || !launches) return TE_ERR_ARG;
+    if (kind >= 4) {          // step-controller phases, from the device wall clock
+        int khz = 0;
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ev->ctx->device);
+        *total_ms = khz > 0 ? (double)ev->st_host->phase_ticks[kind - 4] / khz : 0.0;
+        *launches = (size_t)ev->st_host->n_steps;
+        return TE_OK;
+    }
     *total_ms = ev->prof_ms[kind];
     *launches = ev->prof_n[kind];
     return TE_OK;

@@ -41,6 +41,19 @@ struct SpmvShape {
 constexpr int kStreamChunk = 2048;
 constexpr int kStreamCap = kStreamChunk + 256;     // LDS entries per block (rows up to 257 long)
 
+// Column split: when x is larger than the last-level cache (256 MB MALL on MI355X), the matrix
+// is also held as kMaxParts or fewer column slices, each gathering from a slice of x that
+// stays cache-resident during its pass; partial row sums are carried between passes.
+constexpr int kMaxParts = 8;
+struct CsrPart {
+    int* ptr = nullptr;       // n_rows + 1
+    int* col = nullptr;
+    double2* val = nullptr;
+    int* row_block = nullptr; // n_row_blocks + 1
+    int n_row_blocks = 0;
+    int64_t nnz = 0;
+};
+
 }  // namespace te
 
 struct te_ctx {
@@ -65,6 +78,9 @@ struct te_matrix {
     int* row_block = nullptr;   // device, n_row_blocks + 1 first rows of the stream row blocks
     int n_row_blocks = 0;
     int max_row_len = 0;
+    int n_parts = 0;            // > 1: column-split copy in parts[] used by the stream kernels
+    te::CsrPart parts[te::kMaxParts];
+    double2* acc = nullptr;     // n_rows partial row sums between the passes of a split SpMV
 };
 
 namespace te {
@@ -74,6 +90,7 @@ SpmvShape choose_shape(const te_ctx* ctx, int64_t n_rows, int64_t nnz);
 // Derives the row blocks of the stream kernel on the device and picks the kernel (call after
 // row_ptr is on the device).
 int prepare_spmv(te_ctx* ctx, te_matrix* A);
+void free_parts(te_matrix* A);
 
 // ---- kernels launched across translation units (te_spmv.hip) ----
 void launch_spmv(const te_matrix* A, double2 alpha, const double2* x, double2* y, hipStream_t s);

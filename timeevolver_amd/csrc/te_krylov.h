@@ -32,6 +32,7 @@ struct LanczosArgs {
     KrylovScalars* ks;
     double2* partials;
     unsigned int* ticket;
+    const double2* acc_in;   // partial row sums of the preceding column slices (or null)
 };
 
 // Persistent control state of timeEvolve (core/krylovTimeEvolver.cpp:261-512), updated by
@@ -51,6 +52,7 @@ struct StepState {
     int breakdown;                      // lucky breakdown in the current step
     int write_w;                        // the current step updates currentVec (:374-383)
     int pad;
+    long long phase_ticks[3];           // controller phases (eigen, step search, samples), wall clock
 };
 
 struct StepConsts {
@@ -91,7 +93,7 @@ struct CombineArgs {
     unsigned int* ticket;
 };
 
-void launch_lanczos_apply(const LanczosArgs& a, const SpmvShape& sh, hipStream_t s);
+void launch_lanczos_apply(LanczosArgs a, const te_matrix* A, hipStream_t s);
 void launch_lanczos_project(const LanczosArgs& a, int grid, hipStream_t s);
 void launch_step_control(const StepConsts& c, StepState* st, KrylovScalars* ks, hipStream_t s);
 // Picks the sample/observable chunking; S <= max_samples_per_launch(nobs).

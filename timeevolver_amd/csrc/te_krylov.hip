@@ -362,6 +362,7 @@ __device__ int find_step(CtlShared& S, const StepConsts& C, double t_stepSuggest
 __global__ __launch_bounds__(kCtl) void step_control_kernel(StepConsts C, StepState* st, KrylovScalars* ks) {
     __shared__ CtlShared S;
     const int tid = threadIdx.x;
+    const long long clk0 = wall_clock64();
     StepState L = *st;
     const bool hbd = ks->broken != 0;
     int m = L.m;
@@ -399,6 +400,7 @@ __global__ __launch_bounds__(kCtl) void step_control_kernel(StepConsts C, StepSt
         S.p[tid] = S.Q[tid * 64 + m - 1] * S.Q[tid * 64];
     }
     __syncthreads();
+    const long long clk1 = wall_clock64();
 
     double err_step = 0;
     if (!hbd) {                                       // :374-383
@@ -419,6 +421,7 @@ __global__ __launch_bounds__(kCtl) void step_control_kernel(StepConsts C, StepSt
     }
     if (tid == 0 && L.n_steps - 1 < C.trace_cap) C.trace[L.n_steps - 1] = t_step;
 
+    const long long clk2 = wall_clock64();
     L.sample_base = L.index_samples;                  // :405-437
     L.S = 0;
     while (L.index_samples < C.n_samples &&
@@ -435,6 +438,10 @@ __global__ __launch_bounds__(kCtl) void step_control_kernel(StepConsts C, StepSt
     L.err += err_step;
     L.t_step = t_step;
     L.write_w = hbd ? 0 : 1;
+    const long long clk3 = wall_clock64();
+    L.phase_ticks[0] += clk1 - clk0;
+    L.phase_ticks[1] += clk2 - clk1;
+    L.phase_ticks[2] += clk3 - clk2;
     __syncthreads();
     if (tid == 0) {
         *st = L;

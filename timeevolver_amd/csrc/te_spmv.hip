@@ -19,8 +19,12 @@
 #include "te_krylov.h"
 #include "te_reduce.h"
 
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
+#include <vector>
 
 namespace te {
 
@@ -193,12 +197,19 @@ __device__ __forceinline__ double2 load_stream(const double2* p) {
     const d2v t = __builtin_nontemporal_load(reinterpret_cast<const d2v*>(p));
     return make_double2(t.x, t.y);
 }
+__device__ __forceinline__ void store_stream(double2* p, double2 v) {
+    d2v t;
+    t.x = v.x;
+    t.y = v.y;
+    __builtin_nontemporal_store(t, reinterpret_cast<d2v*>(p));
+}
 static_assert(kStreamCap % kBlock == 0, "stream capacity must be a multiple of the block size");
 
 template <class Epi>
 __device__ __forceinline__ void stream_rows(const int* __restrict__ rb, int nrb, const int* __restrict__ ptr,
                                             const int* __restrict__ col, const double2* __restrict__ val,
-                                            const double2* __restrict__ x, double2* prod, Epi& epi) {
+                                            const double2* __restrict__ x, double2* prod, Epi& epi,
+                                            const double2* __restrict__ acc_in) {
     const int per = (nrb + gridDim.x - 1) / gridDim.x;
     const int lb = xcd_block_id();
     const int b0 = lb * per;
@@ -232,6 +243,11 @@ __device__ __forceinline__ void stream_rows(const int* __restrict__ rb, int nrb,
         for (int r = r0 + t; r < r1; r += kBlock) {
             const int a = ptr[r] - k0, e = ptr[r + 1] - k0;
             double sr = 0.0, si = 0.0;
+            if (acc_in) {               // continue the row sum of the preceding column slices
+                const double2 p = load_stream(&acc_in[r]);
+                sr = p.x;
+                si = p.y;
+            }
             for (int k = a; k < e; k++) {
                 sr += prod[k].x;
                 si += prod[k].y;
@@ -252,10 +268,27 @@ __global__ __launch_bounds__(kBlock) void spmv_stream_kernel(const int* __restri
                                                              const int* __restrict__ ptr, const int* __restrict__ col,
                                                              const double2* __restrict__ val,
                                                              const double2* __restrict__ x, double2* __restrict__ y,
-                                                             double2 alpha) {
+                                                             double2 alpha, const double2* __restrict__ acc_in) {
     __shared__ double2 prod[kStreamCap];
     SpmvEpi epi{alpha, y};
-    stream_rows(rb, nrb, ptr, col, val, x, prod, epi);
+    stream_rows(rb, nrb, ptr, col, val, x, prod, epi, acc_in);
+}
+
+struct PartialEpi {
+    double2* acc;
+    __device__ __forceinline__ void row(int r, double2 s) { store_stream(&acc[r], s); }
+};
+
+// One column slice of a split SpMV: acc[r] (+)= row r of the slice times x, unscaled.
+__global__ __launch_bounds__(kBlock) void spmv_part_kernel(const int* __restrict__ rb, int nrb,
+                                                           const int* __restrict__ ptr, const int* __restrict__ col,
+                                                           const double2* __restrict__ val,
+                                                           const double2* __restrict__ x, const double2* acc_in,
+                                                           double2* acc, const int* broken) {
+    if (broken && *broken) return;
+    __shared__ double2 prod[kStreamCap];
+    PartialEpi epi{acc};
+    stream_rows(rb, nrb, ptr, col, val, x, prod, epi, acc_in);
 }
 
 struct ApplyEpi {
@@ -268,11 +301,11 @@ struct ApplyEpi {
     __device__ __forceinline__ void row(int r, double2 s) {
         double2 wr = cmul(c1, s);
         if (up) {
-            const double2 p = up[r];
+            const double2 p = load_stream(&up[r]);
             wr.x += c2 * p.x;
             wr.y += c2 * p.y;
         }
-        w[r] = wr;
+        store_stream(&w[r], wr);
         const double2 u = uj[r];
         dot.x += u.x * wr.x + u.y * wr.y;
         dot.y += u.x * wr.y - u.y * wr.x;
@@ -293,7 +326,7 @@ __global__ __launch_bounds__(kBlock) void lanczos_apply_stream_kernel(LanczosArg
     epi.uj = a.U + (size_t)j * a.ld;
     epi.w = a.U + (size_t)(j + 1) * a.ld;
     epi.dot = make_double2(0.0, 0.0);
-    stream_rows(a.row_block, a.n_row_blocks, a.ptr, a.col, a.val, epi.uj, prod, epi);
+    stream_rows(a.row_block, a.n_row_blocks, a.ptr, a.col, a.val, epi.uj, prod, epi, a.acc_in);
     double2 bs = block_sum2<kBlock>(epi.dot, sh);
     if (threadIdx.x == 0) a.partials[blockIdx.x] = bs;
     if (last_block_arrive(a.ticket)) {
@@ -375,8 +408,20 @@ __global__ __launch_bounds__(kBlock) void zdotc_kernel(const double2* __restrict
 void launch_spmv(const te_matrix* A, double2 alpha, const double2* x, double2* y, hipStream_t s) {
     const int grid = A->shape.grid;
     if (A->shape.kind == 1) {
+        if (A->n_parts > 1) {
+            const int P = A->n_parts;
+            for (int p = 0; p + 1 < P; p++) {
+                const CsrPart& q = A->parts[p];
+                hipLaunchKernelGGL(spmv_part_kernel, dim3(grid), dim3(kBlock), 0, s, q.row_block, q.n_row_blocks,
+                                   q.ptr, q.col, q.val, x, p > 0 ? A->acc : nullptr, A->acc, nullptr);
+            }
+            const CsrPart& q = A->parts[P - 1];
+            hipLaunchKernelGGL(spmv_stream_kernel, dim3(grid), dim3(kBlock), 0, s, q.row_block, q.n_row_blocks, q.ptr,
+                               q.col, q.val, x, y, alpha, A->acc);
+            return;
+        }
         hipLaunchKernelGGL(spmv_stream_kernel, dim3(grid), dim3(kBlock), 0, s, A->row_block, A->n_row_blocks,
-                           A->row_ptr, A->col, A->val, x, y, alpha);
+                           A->row_ptr, A->col, A->val, x, y, alpha, nullptr);
         return;
     }
 #define L_(LPR) hipLaunchKernelGGL(spmv_kernel<LPR>, dim3(grid), dim3(kBlock), 0, s, (int64_t)A->n_rows, \
@@ -385,15 +430,155 @@ void launch_spmv(const te_matrix* A, double2 alpha, const double2* x, double2* y
 #undef L_
 }
 
-void launch_lanczos_apply(const LanczosArgs& a, const SpmvShape& sh, hipStream_t s) {
+void launch_lanczos_apply(LanczosArgs a, const te_matrix* A, hipStream_t s) {
+    const SpmvShape& sh = A->shape;
     const int grid = sh.grid;
     if (sh.kind == 1) {
+        a.acc_in = nullptr;
+        if (A->n_parts > 1) {
+            const int P = A->n_parts;
+            const double2* x = a.U + (size_t)a.j * a.ld;
+            for (int p = 0; p + 1 < P; p++) {
+                const CsrPart& q = A->parts[p];
+                hipLaunchKernelGGL(spmv_part_kernel, dim3(grid), dim3(kBlock), 0, s, q.row_block, q.n_row_blocks,
+                                   q.ptr, q.col, q.val, x, p > 0 ? A->acc : nullptr, A->acc, &a.ks->broken);
+            }
+            const CsrPart& q = A->parts[P - 1];
+            a.row_block = q.row_block;
+            a.n_row_blocks = q.n_row_blocks;
+            a.ptr = q.ptr;
+            a.col = q.col;
+            a.val = q.val;
+            a.acc_in = A->acc;
+        }
         hipLaunchKernelGGL(lanczos_apply_stream_kernel, dim3(grid), dim3(kBlock), 0, s, a);
         return;
     }
 #define L_(LPR) hipLaunchKernelGGL(lanczos_apply_kernel<LPR>, dim3(grid), dim3(kBlock), 0, s, a)
     TE_LPR_SWITCH(sh.lanes_per_row, L_)
 #undef L_
+}
+
+// Per row and slice: number of entries with col in [bound[p], bound[p+1]) (columns ascending).
+__global__ void split_count_kernel(const int* __restrict__ ptr, const int* __restrict__ col, int64_t n, int P,
+                                   int width, int64_t* lens) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    int cnt[kMaxParts];
+    for (int p = 0; p < P; p++) cnt[p] = 0;
+    for (int k = ptr[r]; k < ptr[r + 1]; k++) {
+        int p = col[k] / width;
+        cnt[p < P ? p : P - 1]++;
+    }
+    for (int p = 0; p < P; p++) lens[(size_t)p * (n + 1) + r] = cnt[p];
+    if (r == 0)
+        for (int p = 0; p < P; p++) lens[(size_t)p * (n + 1) + n] = 0;
+}
+
+__global__ void split_fill_kernel(const int* __restrict__ ptr, const int* __restrict__ col,
+                                  const double2* __restrict__ val, int64_t n, int P, const int64_t* offs,
+                                  CsrPart* parts) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > n) return;
+    for (int p = 0; p < P; p++) parts[p].ptr[r] = (int)offs[(size_t)p * (n + 1) + r];
+    if (r == n) return;
+    int k = ptr[r];
+    for (int p = 0; p < P; p++) {
+        int o = (int)offs[(size_t)p * (n + 1) + r];
+        const int e = (int)offs[(size_t)p * (n + 1) + r + 1];
+        for (; o < e; o++, k++) {
+            parts[p].col[o] = col[k];
+            parts[p].val[o] = val[k];
+        }
+    }
+}
+
+static int exclusive_scan64(te_ctx* ctx, const int64_t* in, int64_t* out, int64_t n) {
+    size_t bytes = 0;
+    TE_HIP(ctx, hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, ctx->stream));
+    void* tmp = nullptr;
+    TE_HIP(ctx, hipMalloc(&tmp, std::max<size_t>(bytes, 1)));
+    hipError_t e = hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, (int)n, ctx->stream);
+    hipStreamSynchronize(ctx->stream);
+    hipFree(tmp);
+    TE_HIP(ctx, e);
+    return TE_OK;
+}
+
+static int row_blocks(te_ctx* ctx, const int* ptr, int64_t n, int64_t nnz, int** rb_out, int* nrb_out, int* maxlen) {
+    const int nrb = (int)((nnz + kStreamChunk - 1) / kStreamChunk);
+    int* rb = nullptr;
+    TE_HIP(ctx, hipMalloc(&rb, sizeof(int) * (nrb + 1)));
+    int* d_max = nullptr;
+    TE_HIP(ctx, hipMalloc(&d_max, sizeof(int)));
+    TE_HIP(ctx, hipMemsetAsync(d_max, 0, sizeof(int), ctx->stream));
+    const int64_t items = std::max<int64_t>(n, nrb + 1);
+    hipLaunchKernelGGL(row_blocks_kernel, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, ctx->stream, ptr, n, nrb,
+                       rb, d_max);
+    int mx = 0;
+    TE_HIP(ctx, hipMemcpyAsync(&mx, d_max, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    TE_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    hipFree(d_max);
+    *rb_out = rb;
+    *nrb_out = nrb;
+    if (maxlen) *maxlen = mx;
+    return TE_OK;
+}
+
+// Column-split copy (P slices of equal column width) with its own row blocks per slice.
+static int build_parts(te_ctx* ctx, te_matrix* A, int P) {
+    const int64_t n = A->n_rows;
+    const int width = (int)((A->n_cols + P - 1) / P);
+    int64_t *lens = nullptr, *offs = nullptr;
+    TE_HIP(ctx, hipMalloc(&lens, sizeof(int64_t) * (size_t)P * (n + 1)));
+    TE_HIP(ctx, hipMalloc(&offs, sizeof(int64_t) * (size_t)P * (n + 1)));
+    hipLaunchKernelGGL(split_count_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, A->row_ptr,
+                       A->col, n, P, width, lens);
+    for (int p = 0; p < P; p++) {
+        int rc = exclusive_scan64(ctx, lens + (size_t)p * (n + 1), offs + (size_t)p * (n + 1), n + 1);
+        if (rc != TE_OK) return rc;
+    }
+    std::vector<int64_t> tot(P);
+    for (int p = 0; p < P; p++)
+        TE_HIP(ctx, hipMemcpy(&tot[p], offs + (size_t)p * (n + 1) + n, sizeof(int64_t), hipMemcpyDeviceToHost));
+    for (int p = 0; p < P; p++) {
+        CsrPart& q = A->parts[p];
+        q.nnz = tot[p];
+        TE_HIP(ctx, hipMalloc(&q.ptr, sizeof(int) * (n + 1)));
+        TE_HIP(ctx, hipMalloc(&q.col, sizeof(int) * std::max<int64_t>(q.nnz, 1)));
+        TE_HIP(ctx, hipMalloc(&q.val, sizeof(double2) * std::max<int64_t>(q.nnz, 1)));
+    }
+    CsrPart* d_parts = nullptr;
+    TE_HIP(ctx, hipMalloc(&d_parts, sizeof(CsrPart) * P));
+    TE_HIP(ctx, hipMemcpy(d_parts, A->parts, sizeof(CsrPart) * P, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(split_fill_kernel, dim3((unsigned)((n + 256) / 256)), dim3(256), 0, ctx->stream, A->row_ptr,
+                       A->col, A->val, n, P, offs, d_parts);
+    TE_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    hipFree(d_parts);
+    hipFree(lens);
+    hipFree(offs);
+    for (int p = 0; p < P; p++) {
+        CsrPart& q = A->parts[p];
+        int rc = row_blocks(ctx, q.ptr, n, q.nnz, &q.row_block, &q.n_row_blocks, nullptr);
+        if (rc != TE_OK) return rc;
+    }
+    TE_HIP(ctx, hipMalloc(&A->acc, sizeof(double2) * n));
+    A->n_parts = P;
+    return TE_OK;
+}
+
+void free_parts(te_matrix* A) {
+    for (int p = 0; p < kMaxParts; p++) {
+        CsrPart& q = A->parts[p];
+        if (q.ptr) hipFree(q.ptr);
+        if (q.col) hipFree(q.col);
+        if (q.val) hipFree(q.val);
+        if (q.row_block) hipFree(q.row_block);
+        q = CsrPart{};
+    }
+    if (A->acc) hipFree(A->acc);
+    A->acc = nullptr;
+    A->n_parts = 0;
 }
 
 int prepare_spmv(te_ctx* ctx, te_matrix* A) {
@@ -420,6 +605,22 @@ int prepare_spmv(te_ctx* ctx, te_matrix* A) {
         A->shape.kind = 1;
         const int cap = std::max(8, ctx->num_cus * 8);
         A->shape.grid = (int)((std::min(nrb, cap) + 7) / 8 * 8);
+        // slices of x no larger than the budget (default 128 MiB, env TE_SPMV_PART_MB)
+        const char* mb = std::getenv("TE_SPMV_PART_MB");
+        const double budget = (mb ? std::atof(mb) : 128.0) * 1048576.0;
+        const double xbytes = 16.0 * (double)A->n_cols;
+        int P = budget > 0 ? (int)std::ceil(xbytes / budget) : 1;
+        P = std::min(P, kMaxParts);
+        if (P > 1) {
+            free_parts(A);
+            int rc = build_parts(ctx, A, P);
+            if (rc != TE_OK) return rc;
+            for (int p = 0; p < P; p++)              // every pass must visit every row
+                if (A->parts[p].n_row_blocks == 0) {
+                    free_parts(A);
+                    break;
+                }
+        }
     }
     return TE_OK;
 }
