@@ -67,7 +67,8 @@ def build_cpp(verbose=False):
     mk = os.path.join(HERE, "cpp", "Makefile")
     if not os.path.exists(mk):
         return None
-    r = subprocess.run(["make", "-s", "-C", os.path.dirname(mk)], capture_output=True, text=True)
+    r = subprocess.run(["make", "-s", "-C", os.path.dirname(mk), "libtimeevolver.so", "examples/oscillators"],
+                       capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("C++ layer build failed:\n%s%s" % (r.stdout, r.stderr))
     return os.path.join(HERE, "cpp", "libtimeevolver.so")

@@ -35,7 +35,7 @@ struct SpmvShape {
     int grid;             // blocks of the launch
 };
 
-// Row-block ("CSR-stream") SpMV: block b holds the rows whose first entry lies in
+// Row-block ("CSR-stream") SpMV: block b holds the rows r with ptr[r] + r in
 // [b * kStreamChunk, (b+1) * kStreamChunk); its entries (at most kStreamChunk + longest row - 1)
 // are multiplied into LDS with every load issued up front, then reduced row by row.
 constexpr int kStreamChunk = 2048;

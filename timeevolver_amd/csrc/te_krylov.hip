@@ -25,9 +25,10 @@
 
 namespace te {
 
-constexpr int kCtl = 256;            // threads of the step controller
-constexpr int kPts = 256;            // integrand points per evaluation pass
+constexpr int kCtl = 1024;           // threads of the step controller (16 wavefronts)
+constexpr int kPts = 1024;           // integrand points per evaluation batch
 constexpr int kGrp = 8;              // lanes per integrand point
+constexpr int kQB = 32;              // integrals per batch
 
 struct CtlShared {
     double Q[kMaxKrylov * kMaxKrylov];  // eigenvectors: column k at Q[k * 64 + row]
@@ -36,11 +37,23 @@ struct CtlShared {
     double p[kMaxKrylov], g[kMaxKrylov], s[kMaxKrylov];
     double2 ph[kMaxKrylov];
     double xs[kPts], fx[kPts], wt[kPts];
-    double bc[4];                       // broadcast slots
+    // a batch of independent error integrals
+    double qa[kQB], qb[kQB], qc[kQB], qd[kQB];
+    double qI1[kQB], qL1[kQB], qerr[kQB];
+    double qret[kQB];
+    int qmethod[kQB], qdone[kQB], qfin[kQB], qok[kQB];
+    int act[kQB];
+    int nact;
     double h;
     int m;
     int info;
 };
+
+// sqrt(a^2 + b^2) without the scaling steps of hypot when no overflow or underflow can occur.
+__device__ __forceinline__ double fast_hypot(double a, double b) {
+    const double s = a * a + b * b;
+    return (s > 1e-290 && s < 1e290) ? sqrt(s) : hypot(a, b);
+}
 
 // ---------------------------------------------------------------------------------------
 // Implicit QL with Wilkinson shifts on a symmetric tridiagonal matrix (Golub & Van Loan,
@@ -63,14 +76,14 @@ __device__ int wave_tridiag_ql(int n, double* d, double* e, double* Q) {
             if (mm == l) break;
             if (++iter > 60) return 1;
             double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-            double r = hypot(g, 1.0);
+            double r = fast_hypot(g, 1.0);
             g = d[mm] - d[l] + e[l] / (g + copysign(r, g));
             double s = 1.0, c = 1.0, p = 0.0;
             bool early = false;
             for (int i = mm - 1; i >= l; i--) {
                 const double f = s * e[i];
                 const double bb = c * e[i];
-                r = hypot(f, g);
+                r = fast_hypot(f, g);
                 e[i + 1] = r;
                 if (r == 0.0) {
                     d[i + 1] -= p;
@@ -78,8 +91,9 @@ __device__ int wave_tridiag_ql(int n, double* d, double* e, double* Q) {
                     early = true;
                     break;
                 }
-                s = f / r;
-                c = g / r;
+                const double ir = 1.0 / r;
+                s = f * ir;
+                c = g * ir;
                 g = d[i + 1] - p;
                 r = (d[i] - g) * s + 2.0 * c * bb;
                 p = s * r;
@@ -128,145 +142,231 @@ __device__ void eval_points(CtlShared& S, int npts) {
     __syncthreads();
 }
 
-// Weighted sums over slots [first, first+cnt) in slot order (thread 0), broadcast to all.
-__device__ void weighted_sums(CtlShared& S, int first, int cnt, double& s, double& as) {
-    if (threadIdx.x == 0) {
-        double a = s, b = as;
-        for (int i = first; i < first + cnt; i++) {
-            a += S.fx[i] * S.wt[i];
-            b += fabs(S.fx[i]) * S.wt[i];
-        }
-        S.bc[0] = a;
-        S.bc[1] = b;
-    }
-    __syncthreads();
-    s = S.bc[0];
-    as = S.bc[1];
-    __syncthreads();
-}
+// ---------------------------------------------------------------------------------------
+// Batched error integrals. The integrals q < nb over [qa[q], qb[q]] (method qmethod[q]: 0
+// Gauss-Legendre 15, 1 Gauss-Legendre 7, 2 tanh-sinh) are independent, so their integrand
+// points are evaluated together; each integral keeps exactly the summation order and the
+// level schedule it would have alone. Results: qret[q] and qok[q] (integrateError's value and
+// success flag, core/krylovTimeEvolver.cpp:584-618).
+// ---------------------------------------------------------------------------------------
 
-// One tanh-sinh row: nodes [first, end) of row `row`, right then left point of each node;
-// points that round onto an end point get weight 0. When fc != nullptr the centre c is
-// evaluated in the same pass and returned in *fc.
-__device__ void ts_row(CtlShared& S, const StepConsts& C, int row, int first, double a, double b, double c,
-                       double d, double& s, double& as, double* fc) {
-    const int off0 = C.ts_off[row] + first, off1 = C.ts_off[row + 1];
-    const int nn = off1 - off0;
-    s = 0.0;
-    as = 0.0;
-    int done = 0;
-    bool first_chunk = true;
-    while (done < nn || (first_chunk && fc != nullptr)) {
-        const int slot = (first_chunk && fc != nullptr) ? 1 : 0;
-        int cnt = (kPts - slot) / 2;
-        if (cnt > nn - done) cnt = nn - done;
-        const int tid = threadIdx.x;
-        if (slot == 1 && tid == 0) {
-            S.xs[0] = c;
-            S.wt[0] = 0.0;
+// Tanh-sinh (Takahasi & Mori) with Boost's driver structure: h = 1 on row 0, halving per row,
+// error |I_k - I_{k-1}|, stop when error <= tol * L1, at most ts_rows - 1 refinements. Points
+// that round onto an end point get weight 0.
+__device__ void ts_batch(CtlShared& S, const StepConsts& C, double tol) {
+    const double half_pi = 1.5707963267948966;
+    const int tid = threadIdx.x;
+    // level 0: centre, then (right, left) for nodes t = 1, 2, ...
+    {
+        const int n0 = C.ts_off[1] - C.ts_off[0];
+        const int per = 1 + 2 * (n0 - 1);
+        const int G = kPts / per;
+        for (int g0 = 0; g0 < S.nact; g0 += G) {
+            const int cnt = min(G, S.nact - g0);
+            for (int i = tid; i < cnt * per; i += kCtl) {
+                const int q = S.act[g0 + i / per], j = i % per;
+                if (j == 0) {
+                    S.xs[i] = S.qc[q];
+                    S.wt[i] = 0.0;
+                } else {
+                    const double2 nd = C.ts_nodes[C.ts_off[0] + (j + 1) / 2];
+                    const double off = S.qd[q] * nd.x;
+                    const bool right = ((j - 1) & 1) == 0;
+                    const double x = right ? S.qb[q] - off : S.qa[q] + off;
+                    S.xs[i] = x;
+                    S.wt[i] = (x != (right ? S.qb[q] : S.qa[q])) ? nd.y : 0.0;
+                }
+            }
+            __syncthreads();
+            eval_points(S, cnt * per);
+            if (tid < cnt) {
+                const int q = S.act[g0 + tid], base = tid * per;
+                const double y0 = S.fx[base];
+                double s = 0.0, as = 0.0;
+                for (int j = 1; j < per; j++) {
+                    s += S.fx[base + j] * S.wt[base + j];
+                    as += fabs(S.fx[base + j]) * S.wt[base + j];
+                }
+                S.qI1[q] = (half_pi * y0 + s) * 1.0;
+                S.qL1[q] = (half_pi * fabs(y0) + as) * 1.0;
+                S.qerr[q] = DBL_MAX;
+                S.qdone[q] = 0;
+                S.qfin[q] = 1;
+            }
+            __syncthreads();
         }
-        if (tid < cnt) {
-            const double2 nd = C.ts_nodes[off0 + done + tid];
-            const double off = d * nd.x;
-            const double xr = b - off, xl = a + off;
-            S.xs[slot + 2 * tid] = xr;
-            S.wt[slot + 2 * tid] = (xr != b) ? nd.y : 0.0;
-            S.xs[slot + 2 * tid + 1] = xl;
-            S.wt[slot + 2 * tid + 1] = (xl != a) ? nd.y : 0.0;
+    }
+    double h = 1.0;
+    for (int k = 1; k < C.ts_rows; k++) {
+        if (tid == 0) {                                    // compact the still-active integrals
+            int na = 0;
+            for (int i = 0; i < S.nact; i++)
+                if (!S.qdone[S.act[i]]) S.act[na++] = S.act[i];
+            S.nact = na;
         }
         __syncthreads();
-        eval_points(S, slot + 2 * cnt);
-        if (slot == 1) *fc = S.fx[0];
-        weighted_sums(S, slot, 2 * cnt, s, as);
-        done += cnt;
-        first_chunk = false;
-    }
-}
-
-// Tanh-sinh with Boost's driver structure (h = 1 on row 0, halving per row, error
-// |I_k - I_{k-1}|, stop when error <= tol * L1, at most 14 refinements).
-__device__ double ts_integrate(CtlShared& S, const StepConsts& C, double a, double b, double tol, double& error,
-                               double& L1, bool& finite) {
-    const double half_pi = 1.5707963267948966;
-    const double c = 0.5 * (a + b), d = 0.5 * (b - a);
-    double h = 1.0;
-    double y0 = 0.0, s, as;
-    ts_row(S, C, 0, 1, a, b, c, d, s, as, &y0);
-    double I0 = half_pi * y0, L0 = half_pi * fabs(y0);
-    I0 = (I0 + s) * h;
-    L0 = (L0 + as) * h;
-    double I1 = I0, L11 = L0, err = DBL_MAX;
-    finite = true;
-    for (int k = 1; k < C.ts_rows; k++) {
-        I0 = I1;
-        L0 = L11;
-        I1 = 0.5 * I0;
-        L11 = 0.5 * L0;
+        const int nact = S.nact;
+        if (nact == 0) break;
         h *= 0.5;
-        ts_row(S, C, k, 0, a, b, c, d, s, as, nullptr);
-        I1 += s * h;
-        L11 += as * h;
-        err = fabs(I0 - I1);
-        if (!isfinite(I1)) {
-            finite = false;
-            break;
+        const int off0 = C.ts_off[k];
+        const int nk = C.ts_off[k + 1] - off0;
+        const int per = 2 * nk;
+        if (per <= kPts) {
+            const int G = kPts / per;
+            for (int g0 = 0; g0 < nact; g0 += G) {
+                const int cnt = min(G, nact - g0);
+                for (int i = tid; i < cnt * per; i += kCtl) {
+                    const int q = S.act[g0 + i / per], j = i % per;
+                    const double2 nd = C.ts_nodes[off0 + j / 2];
+                    const double off = S.qd[q] * nd.x;
+                    const bool right = (j & 1) == 0;
+                    const double x = right ? S.qb[q] - off : S.qa[q] + off;
+                    S.xs[i] = x;
+                    S.wt[i] = (x != (right ? S.qb[q] : S.qa[q])) ? nd.y : 0.0;
+                }
+                __syncthreads();
+                eval_points(S, cnt * per);
+                if (tid < cnt) {
+                    const int q = S.act[g0 + tid], base = tid * per;
+                    double s = 0.0, as = 0.0;
+                    for (int j = 0; j < per; j++) {
+                        s += S.fx[base + j] * S.wt[base + j];
+                        as += fabs(S.fx[base + j]) * S.wt[base + j];
+                    }
+                    const double I0 = S.qI1[q], L0 = S.qL1[q];
+                    double I1 = 0.5 * I0, L11 = 0.5 * L0;
+                    I1 += s * h;
+                    L11 += as * h;
+                    const double err = fabs(I0 - I1);
+                    S.qI1[q] = I1;
+                    S.qL1[q] = L11;
+                    S.qerr[q] = err;
+                    if (!isfinite(I1)) {
+                        S.qfin[q] = 0;
+                        S.qdone[q] = 1;
+                    } else if (err <= tol * L11) {
+                        S.qdone[q] = 1;
+                    }
+                }
+                __syncthreads();
+            }
+        } else {
+            // rows wider than the point buffer: one integral at a time, points in chunks
+            for (int ia = 0; ia < nact; ia++) {
+                const int q = S.act[ia];
+                double s = 0.0, as = 0.0;
+                for (int done = 0; done < per; done += kPts) {
+                    const int cnt = min(kPts, per - done);
+                    for (int i = tid; i < cnt; i += kCtl) {
+                        const int j = done + i;
+                        const double2 nd = C.ts_nodes[off0 + j / 2];
+                        const double off = S.qd[q] * nd.x;
+                        const bool right = (j & 1) == 0;
+                        const double x = right ? S.qb[q] - off : S.qa[q] + off;
+                        S.xs[i] = x;
+                        S.wt[i] = (x != (right ? S.qb[q] : S.qa[q])) ? nd.y : 0.0;
+                    }
+                    __syncthreads();
+                    eval_points(S, cnt);
+                    if (tid == 0)
+                        for (int i = 0; i < cnt; i++) {
+                            s += S.fx[i] * S.wt[i];
+                            as += fabs(S.fx[i]) * S.wt[i];
+                        }
+                    __syncthreads();
+                }
+                if (tid == 0) {
+                    const double I0 = S.qI1[q], L0 = S.qL1[q];
+                    double I1 = 0.5 * I0, L11 = 0.5 * L0;
+                    I1 += s * h;
+                    L11 += as * h;
+                    const double err = fabs(I0 - I1);
+                    S.qI1[q] = I1;
+                    S.qL1[q] = L11;
+                    S.qerr[q] = err;
+                    if (!isfinite(I1)) {
+                        S.qfin[q] = 0;
+                        S.qdone[q] = 1;
+                    } else if (err <= tol * L11) {
+                        S.qdone[q] = 1;
+                    }
+                }
+                __syncthreads();
+            }
         }
-        if (err <= tol * L11) break;
     }
-    error = err * d;
-    L1 = L11 * d;
-    return I1 * d;
 }
 
-// Gauss-Legendre: centre first (odd n), then (f(c + d x_i) + f(c - d x_i)) w_i.
-__device__ double gl_integrate(CtlShared& S, const double2* nodes, int nn, int n, double a, double b) {
-    const double c = 0.5 * (a + b), d = 0.5 * (b - a);
+// Gauss-Legendre for the active integrals: centre first (odd n), then
+// (f(c + d x_i) + f(c - d x_i)) w_i (Boost's gauss<double, N>::integrate order).
+__device__ void gl_batch(CtlShared& S, const double2* nodes, int nn, int n) {
     const int tid = threadIdx.x;
-    const bool odd = (n & 1) != 0;
-    const int first = odd ? 1 : 0;
-    if (tid == 0 && odd) S.xs[0] = c;
-    if (tid >= first && tid < nn) {
-        const double2 nd = nodes[tid];
-        const int k = first + 2 * (tid - first);
-        S.xs[k] = c + d * nd.x;
-        S.xs[k + 1] = c - d * nd.x;
-    }
-    __syncthreads();
-    eval_points(S, first + 2 * (nn - first));
-    if (tid == 0) {
-        double s = odd ? S.fx[0] * nodes[0].y : 0.0;
-        for (int i = first; i < nn; i++) {
-            const int k = first + 2 * (i - first);
-            s += (S.fx[k] + S.fx[k + 1]) * nodes[i].y;
+    const int first = (n & 1) ? 1 : 0;
+    const int per = first + 2 * (nn - first);
+    const int G = kPts / per;
+    for (int g0 = 0; g0 < S.nact; g0 += G) {
+        const int cnt = min(G, S.nact - g0);
+        for (int i = tid; i < cnt * per; i += kCtl) {
+            const int q = S.act[g0 + i / per], j = i % per;
+            if (first && j == 0) {
+                S.xs[i] = S.qc[q];
+            } else {
+                const int node = first + (j - first) / 2;
+                const double dx = S.qd[q] * nodes[node].x;
+                S.xs[i] = ((j - first) & 1) == 0 ? S.qc[q] + dx : S.qc[q] - dx;
+            }
         }
-        S.bc[2] = s * d;
+        __syncthreads();
+        eval_points(S, cnt * per);
+        if (tid < cnt) {
+            const int q = S.act[g0 + tid], base = tid * per;
+            double s = first ? S.fx[base] * nodes[0].y : 0.0;
+            for (int i = first; i < nn; i++) {
+                const int k = base + first + 2 * (i - first);
+                s += (S.fx[k] + S.fx[k + 1]) * nodes[i].y;
+            }
+            S.qret[q] = s * S.qd[q];
+            S.qok[q] = 1;
+        }
+        __syncthreads();
     }
-    __syncthreads();
-    const double r = S.bc[2];
-    __syncthreads();
-    return r;
 }
 
-// integrateError (core/krylovTimeEvolver.cpp:584-618); method 0 GL15, 1 GL7, 2 tanh-sinh.
-__device__ double integrate_error(CtlShared& S, const StepConsts& C, double a, double b, int method,
-                                  bool& successful) {
+// integrateError for the nb integrals set up in S.qa / S.qb / S.qmethod.
+__device__ void integrate_batch(CtlShared& S, const StepConsts& C, int nb) {
     const double termination = 1e-3;                                 // :54
-    double ret;
-    bool success = true;
-    if (method == 0) ret = gl_integrate(S, C.gl15, C.gl15_n, 15, a, b);
-    else if (method == 1) ret = gl_integrate(S, C.gl7, C.gl7_n, 7, a, b);
-    else {
-        double error = 0.0, L1 = 0.0;
-        bool finite = true;
-        ret = ts_integrate(S, C, a, b, termination, error, L1, finite);
-        if (!finite) {                                               // Boost throws (:602-605)
-            ret = C.tol * 10;
-            error = 0.0;
-        }
-        if (error / ret > termination && ret > termination * (b - a) * C.tolRate) success = false;   // :607
+    const int tid = threadIdx.x;
+    if (tid < nb) {
+        S.qc[tid] = 0.5 * (S.qa[tid] + S.qb[tid]);
+        S.qd[tid] = 0.5 * (S.qb[tid] - S.qa[tid]);
     }
-    successful = successful && success;
-    return ret;
+    for (int method = 0; method < 3; method++) {
+        if (tid == 0) {
+            int na = 0;
+            for (int q = 0; q < nb; q++)
+                if (S.qmethod[q] == method) S.act[na++] = q;
+            S.nact = na;
+        }
+        __syncthreads();
+        if (S.nact == 0) continue;
+        if (method == 0) gl_batch(S, C.gl15, C.gl15_n, 15);
+        else if (method == 1) gl_batch(S, C.gl7, C.gl7_n, 7);
+        else {
+            ts_batch(S, C, termination);
+            if (tid < nb && S.qmethod[tid] == 2) {
+                const int q = tid;
+                const double d = S.qd[q];
+                double ret = S.qI1[q] * d, error = S.qerr[q] * d;
+                if (!S.qfin[q]) {                                    // Boost throws (:602-605)
+                    ret = C.tol * 10;
+                    error = 0.0;
+                }
+                S.qret[q] = ret;
+                S.qok[q] = (error / ret > termination && ret > termination * (S.qb[q] - S.qa[q]) * C.tolRate) ? 0 : 1;
+            }
+            __syncthreads();
+        }
+    }
 }
 
 // out[j] = s_j [exp(x H) e1]_j  (expKrylov, :685-702, scaled for the unnormalised basis).
@@ -298,7 +398,23 @@ __device__ void exp_e1_scaled(CtlShared& S, double x, double2* out) {
     __syncthreads();
 }
 
-// findMaximalStepSize (core/krylovTimeEvolver.cpp:153-254). Returns the status code, or -1
+// Queue of precomputed integrals consumed in the reference's sequential order: `next` is the
+// first unconsumed entry, `count` the number computed.
+struct IntQueue {
+    int next, count;
+};
+
+__device__ __forceinline__ void queue_set(CtlShared& S, int q, double a, double b, int method) {
+    if (threadIdx.x == 0) {
+        S.qa[q] = a;
+        S.qb[q] = b;
+        S.qmethod[q] = method;
+    }
+}
+
+// findMaximalStepSize (core/krylovTimeEvolver.cpp:153-254). The decisions are the reference's,
+// taken in its order; only the integrals are computed ahead in batches (speculative ones whose
+// result the loop never consumes do not touch the success flag). Returns the status code, or -1
 // where the reference exits after GO_MAX halvings.
 __device__ int find_step(CtlShared& S, const StepConsts& C, double t_stepSuggestion, double t_step_max,
                          int n_substeps, bool increaseStep, int mLong, int mShort, double* t_stepRet,
@@ -314,7 +430,33 @@ __device__ int find_step(CtlShared& S, const StepConsts& C, double t_stepSuggest
         t_step = t_step_max;
         skipSubsteps = true;
     }
-    double err_step = integrate_error(S, C, 0, t_step, mLong, okInt);
+    // batch 0: [0, t_step], then either the doubling candidates or the first substeps (valid if
+    // no halving follows)
+    int nb = 0;
+    queue_set(S, nb++, 0, t_step, mLong);
+    int spec_sub = 0;                       // substeps 1..spec_sub speculated with t_step unchanged
+    if (increaseStep && !skipSubsteps) {
+        double v = t_step;
+        while (nb < kQB) {
+            v *= 2;
+            if (v > t_step_max) v = t_step_max;
+            queue_set(S, nb++, 0, v, mLong);
+            if (v >= t_step_max) break;
+        }
+    } else if (!skipSubsteps) {
+        const double s = t_step / n_substeps;
+        for (int n = 1; n <= 8 && nb < kQB; n++) {
+            if (t_step + (n - 1) * s >= t_step_max) break;
+            queue_set(S, nb++, t_step + (n - 1) * s, t_step + n * s, mShort);
+            spec_sub = n;
+        }
+    }
+    __syncthreads();
+    integrate_batch(S, C, nb);
+    IntQueue Q{1, nb};
+    double err_step = S.qret[0];
+    okInt = okInt && S.qok[0];
+
     if (increaseStep && !skipSubsteps) {
         double err_new = err_step, t_new = t_step;
         while (err_new < tolRate * t_new && t_step < t_step_max) {
@@ -326,25 +468,80 @@ __device__ int find_step(CtlShared& S, const StepConsts& C, double t_stepSuggest
             }
             t_new *= 2;
             if (t_new > t_step_max) t_new = t_step_max;
-            err_new = integrate_error(S, C, 0, t_new, mLong, okInt);
+            if (Q.next >= Q.count) {                     // refill with the next doublings
+                int k = 0;
+                double v = t_new;
+                queue_set(S, k++, 0, v, mLong);
+                while (k < kQB && v < t_step_max) {
+                    v *= 2;
+                    if (v > t_step_max) v = t_step_max;
+                    queue_set(S, k++, 0, v, mLong);
+                }
+                __syncthreads();
+                integrate_batch(S, C, k);
+                Q.next = 0;
+                Q.count = k;
+            }
+            err_new = S.qret[Q.next];
+            okInt = okInt && S.qok[Q.next];
+            Q.next++;
         }
+        spec_sub = 0;
     }
-    while (err_step > tolRate * t_step) {
-        nbReductions++;
-        t_step = t_step / 2.0;
-        err_step = integrate_error(S, C, 0, t_step, mLong, okInt);
-        if (nbReductions == GO_MAX) return -1;
-        skipSubsteps = false;
+    bool halved = false;
+    if (err_step > tolRate * t_step) {
+        halved = true;
+        Q.next = Q.count = 0;
+        while (err_step > tolRate * t_step) {
+            nbReductions++;
+            t_step = t_step / 2.0;
+            if (Q.next >= Q.count) {                     // refill with the next halvings
+                int k = 0;
+                double v = t_step;
+                while (k < kQB && nbReductions + k <= GO_MAX) {
+                    queue_set(S, k++, 0, v, mLong);
+                    v = v / 2.0;
+                }
+                __syncthreads();
+                integrate_batch(S, C, k);
+                Q.next = 0;
+                Q.count = k;
+            }
+            err_step = S.qret[Q.next];
+            okInt = okInt && S.qok[Q.next];
+            Q.next++;
+            if (nbReductions == GO_MAX) return -1;
+            skipSubsteps = false;
+        }
     }
     const double s = t_step / n_substeps;
     int n_s = 0;
     double deltaError = 0;
     if (!skipSubsteps) {
+        // substeps speculated in batch 0 hold for the unchanged t_step only
+        int sub_next = 1, sub_count = halved ? 0 : spec_sub;   // substep n is at queue slot offset
+        int sub_slot0 = halved ? 0 : 1 + 0;                    // batch 0: slot 0 is [0, t_step]
+        int batch = 8;
         while (err_step + deltaError < tolRate * (t_step + n_s * s)) {
             err_step += deltaError;
             n_s++;
             if (t_step + (n_s - 1) * s >= t_step_max) break;
-            deltaError = integrate_error(S, C, t_step + (n_s - 1) * s, t_step + n_s * s, mShort, okInt);
+            if (n_s > sub_count) {                       // refill: substeps n_s, n_s + 1, ...
+                batch = min(kQB, batch * 2);
+                int k = 0;
+                for (int n = n_s; k < batch; n++) {
+                    if (n > n_s && t_step + (n - 1) * s >= t_step_max) break;
+                    queue_set(S, k++, t_step + (n - 1) * s, t_step + n * s, mShort);
+                }
+                __syncthreads();
+                integrate_batch(S, C, k);
+                sub_next = n_s;
+                sub_count = n_s + k - 1;
+                sub_slot0 = 0;
+            }
+            const int slot = sub_slot0 + (n_s - sub_next);
+            deltaError = S.qret[slot];
+            okInt = okInt && S.qok[slot];
         }
         t_step += (n_s - 1) * s;
     }

@@ -338,7 +338,10 @@ __global__ __launch_bounds__(kBlock) void lanczos_apply_stream_kernel(LanczosArg
     }
 }
 
-// rb[b] = first row whose first entry is >= b * kStreamChunk (rb[nrb] = n); max row length.
+// Row blocks on the merge-path coordinate ptr[r] + r (entries plus rows before row r):
+// rb[b] = first row with ptr[r] + r >= b * kStreamChunk, rb[nrb] = n. A block then holds at
+// most kStreamChunk rows and at most kStreamChunk - 1 + (longest row) entries, so neither
+// long rows nor long runs of empty rows unbalance the blocks. Also returns the longest row.
 __global__ void row_blocks_kernel(const int* __restrict__ ptr, int64_t n, int nrb, int* rb, int* maxlen) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
@@ -350,10 +353,10 @@ __global__ void row_blocks_kernel(const int* __restrict__ ptr, int64_t n, int nr
             rb[i] = (int)n;
         } else {
             const int64_t target = i * (int64_t)kStreamChunk;
-            int64_t lo = 0, hi = n;                 // first r in [0, n] with ptr[r] >= target
+            int64_t lo = 0, hi = n;                 // first r in [0, n] with ptr[r] + r >= target
             while (lo < hi) {
                 const int64_t mid = (lo + hi) / 2;
-                if (ptr[mid] >= target) hi = mid;
+                if ((int64_t)ptr[mid] + mid >= target) hi = mid;
                 else lo = mid + 1;
             }
             rb[i] = (int)lo;
@@ -506,7 +509,7 @@ static int exclusive_scan64(te_ctx* ctx, const int64_t* in, int64_t* out, int64_
 }
 
 static int row_blocks(te_ctx* ctx, const int* ptr, int64_t n, int64_t nnz, int** rb_out, int* nrb_out, int* maxlen) {
-    const int nrb = (int)((nnz + kStreamChunk - 1) / kStreamChunk);
+    const int nrb = (int)((nnz + n + kStreamChunk - 1) / kStreamChunk);
     int* rb = nullptr;
     TE_HIP(ctx, hipMalloc(&rb, sizeof(int) * (nrb + 1)));
     int* d_max = nullptr;
@@ -583,7 +586,7 @@ void free_parts(te_matrix* A) {
 
 int prepare_spmv(te_ctx* ctx, te_matrix* A) {
     A->shape = choose_shape(ctx, A->n_rows, A->nnz);
-    const int nrb = (int)((A->nnz + kStreamChunk - 1) / kStreamChunk);
+    const int nrb = (int)((A->nnz + A->n_rows + kStreamChunk - 1) / kStreamChunk);
     if (A->row_block) hipFree(A->row_block);
     A->row_block = nullptr;
     TE_HIP(ctx, hipMalloc(&A->row_block, sizeof(int) * (nrb + 1)));
