@@ -9,6 +9,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 
@@ -41,7 +42,11 @@ int fail(te_ctx* ctx, int code, const char* fmt, ...) {
 }
 
 static int grid_for(const te_ctx* ctx, int64_t work_items) {
-    const int cap = std::max(8, ctx->num_cus * 8);
+    static const int per_cu = [] {
+        const char* v = std::getenv("TE_STREAM_BLOCKS_PER_CU");      // launch-shape sweeps
+        return v ? std::max(1, std::atoi(v)) : 8;
+    }();
+    const int cap = std::max(8, ctx->num_cus * per_cu);
     int64_t g = (work_items + kBlock - 1) / kBlock;
     g = std::max<int64_t>(8, std::min<int64_t>(g, cap));
     return (int)((g + 7) / 8 * 8);

@@ -435,9 +435,13 @@ __device__ int find_step(CtlShared& S, const StepConsts& C, double t_stepSuggest
     int nb = 0;
     queue_set(S, nb++, 0, t_step, mLong);
     int spec_sub = 0;                       // substeps 1..spec_sub speculated with t_step unchanged
+    // Speculation windows are kept small: a candidate far beyond the point where the
+    // sequential loop stops (a doubling up to t_step_max) can cost more levels than every
+    // consumed integral together.
+    constexpr int kDoubleAhead = 2, kHalveAhead = 3;
     if (increaseStep && !skipSubsteps) {
         double v = t_step;
-        while (nb < kQB) {
+        while (nb < 1 + kDoubleAhead) {
             v *= 2;
             if (v > t_step_max) v = t_step_max;
             queue_set(S, nb++, 0, v, mLong);
@@ -445,7 +449,7 @@ __device__ int find_step(CtlShared& S, const StepConsts& C, double t_stepSuggest
         }
     } else if (!skipSubsteps) {
         const double s = t_step / n_substeps;
-        for (int n = 1; n <= 8 && nb < kQB; n++) {
+        for (int n = 1; n <= 4 && nb < kQB; n++) {
             if (t_step + (n - 1) * s >= t_step_max) break;
             queue_set(S, nb++, t_step + (n - 1) * s, t_step + n * s, mShort);
             spec_sub = n;
@@ -472,7 +476,7 @@ __device__ int find_step(CtlShared& S, const StepConsts& C, double t_stepSuggest
                 int k = 0;
                 double v = t_new;
                 queue_set(S, k++, 0, v, mLong);
-                while (k < kQB && v < t_step_max) {
+                while (k < kDoubleAhead && v < t_step_max) {
                     v *= 2;
                     if (v > t_step_max) v = t_step_max;
                     queue_set(S, k++, 0, v, mLong);
@@ -498,7 +502,7 @@ __device__ int find_step(CtlShared& S, const StepConsts& C, double t_stepSuggest
             if (Q.next >= Q.count) {                     // refill with the next halvings
                 int k = 0;
                 double v = t_step;
-                while (k < kQB && nbReductions + k <= GO_MAX) {
+                while (k < kHalveAhead && nbReductions + k <= GO_MAX) {
                     queue_set(S, k++, 0, v, mLong);
                     v = v / 2.0;
                 }
@@ -521,7 +525,7 @@ __device__ int find_step(CtlShared& S, const StepConsts& C, double t_stepSuggest
         // substeps speculated in batch 0 hold for the unchanged t_step only
         int sub_next = 1, sub_count = halved ? 0 : spec_sub;   // substep n is at queue slot offset
         int sub_slot0 = halved ? 0 : 1 + 0;                    // batch 0: slot 0 is [0, t_step]
-        int batch = 8;
+        int batch = 4;
         while (err_step + deltaError < tolRate * (t_step + n_s * s)) {
             err_step += deltaError;
             n_s++;

@@ -606,11 +606,14 @@ int prepare_spmv(te_ctx* ctx, te_matrix* A) {
     const bool fits = mx + kStreamChunk - 1 <= kStreamCap && nrb > 0;
     if (fits && !(force && force[0] == '0')) {
         A->shape.kind = 1;
-        const int cap = std::max(8, ctx->num_cus * 8);
+        const char* gmul = std::getenv("TE_SPMV_BLOCKS_PER_CU");   // launch-shape sweeps
+        const int cap = std::max(8, ctx->num_cus * (gmul ? std::max(1, std::atoi(gmul)) : 8));
         A->shape.grid = (int)((std::min(nrb, cap) + 7) / 8 * 8);
-        // slices of x no larger than the budget (default 128 MiB, env TE_SPMV_PART_MB)
+        // Column slices of x no larger than TE_SPMV_PART_MB (MiB). Off by default: measured on
+        // the L = N = 14 ring (x = 321 MB), 2, 3 and 5 slices were 10-30 % slower than one
+        // pass (the slices are not retained by the MALL under this access pattern).
         const char* mb = std::getenv("TE_SPMV_PART_MB");
-        const double budget = (mb ? std::atof(mb) : 128.0) * 1048576.0;
+        const double budget = (mb ? std::atof(mb) : 0.0) * 1048576.0;
         const double xbytes = 16.0 * (double)A->n_cols;
         int P = budget > 0 ? (int)std::ceil(xbytes / budget) : 1;
         P = std::min(P, kMaxParts);
