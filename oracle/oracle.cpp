@@ -25,6 +25,7 @@
 #include <complex>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <map>
@@ -489,7 +490,8 @@ struct TanhSinh {
     // Returns the integral; *error and *L1 as Boost reports them. Throws never;
     // *finite is false if a sum became non-finite (Boost raises an evaluation error).
     template <class F>
-    double integrate(F f, double a, double b, double tol, double* error, double* L1, bool* finite) const {
+    double integrate(F f, double a, double b, double tol, double* error, double* L1, bool* finite,
+                     double abs_floor = 0.0, double value_cap = 0.0) const {
         double c = 0.5 * (a + b), d = 0.5 * (b - a);
         auto level_sum = [&](const std::vector<TSNode>& row, size_t first, double& s, double& as) {
             s = 0.0; as = 0.0;
@@ -509,7 +511,9 @@ struct TanhSinh {
         L0 = (L0 + as) * h;
         double I1 = I0, L11 = L0, err = std::numeric_limits<double>::max();
         *finite = true;
+        int lev = 0;
         for (int k = 1; k < max_levels; k++) {
+            lev = k;
             I0 = I1; L0 = L11;
             I1 = 0.5 * I0; L11 = 0.5 * L0;
             h *= 0.5;
@@ -519,9 +523,15 @@ struct TanhSinh {
             err = std::fabs(I0 - I1);
             if (!std::isfinite(I1)) { *finite = false; break; }
             if (err <= tol * L11) break;
+            // successive estimates agree to the integrand's rounding noise, and the value is too small
+            // for its error to be judged (see the success rule of integrate_error)
+            if (err * d <= abs_floor && std::fabs(I1) * d <= value_cap) break;
         }
         *error = err * d;
         *L1 = L11 * d;
+        static const bool trace = std::getenv("ORACLE_TRACE_QUAD") != nullptr;   // per-integral log
+        if (trace)
+            std::fprintf(stderr, "quad a=%.6e b=%.6e levels=%d I=%.6e err=%.3e\n", a, b, lev, I1 * d, err * d);
         return I1 * d;
     }
     static double half_pi() { return 1.5707963267948966; }
@@ -587,7 +597,13 @@ static double integrate_error(const KrylovSpace& ks, double a, double b, double 
     else if (method == 1) ret = gauss7().integrate(f, a, b);
     else {
         bool finite = true;
-        ret = tanh_sinh().integrate(f, a, b, termination, &error, &L1, &finite);
+        // The integrand h |sum_k p_k e^{-i x lambda_k}| is bounded by h (sum_k |p_k| <= 1), so two
+        // estimates that agree to eps h (b-a) agree to the rounding unit of that bound: refinement
+        // stops there, but only below termination (b-a) tolRate, where the reference counts the
+        // integral as successful whatever its error (:607-608).
+        const double floor_ = DBL_EPSILON * h * (b - a);
+        const double cap = termination * (b - a) * tolRate;
+        ret = tanh_sinh().integrate(f, a, b, termination, &error, &L1, &finite, floor_, cap);
         if (!finite) { ret = tol * 10; error = 0.0; }        // Boost throws; caught at :602-605
         if (error / ret > termination && ret > termination * (b - a) * tolRate) success = false;  // :607-608
     }

@@ -44,7 +44,7 @@ int fail(te_ctx* ctx, int code, const char* fmt, ...) {
 static int grid_for(const te_ctx* ctx, int64_t work_items) {
     static const int per_cu = [] {
         const char* v = std::getenv("TE_STREAM_BLOCKS_PER_CU");      // launch-shape sweeps
-        return v ? std::max(1, std::atoi(v)) : 8;
+        return v ? std::max(1, std::atoi(v)) : 4;
     }();
     const int cap = std::max(8, ctx->num_cus * per_cu);
     int64_t g = (work_items + kBlock - 1) / kBlock;

@@ -40,7 +40,7 @@ struct CtlShared {
     // a batch of independent error integrals
     double qa[kQB], qb[kQB], qc[kQB], qd[kQB];
     double qI1[kQB], qL1[kQB], qerr[kQB];
-    double qret[kQB];
+    double qret[kQB], qfloor[kQB], qcap[kQB];
     int qmethod[kQB], qdone[kQB], qfin[kQB], qok[kQB];
     int act[kQB];
     int nact;
@@ -244,7 +244,8 @@ __device__ void ts_batch(CtlShared& S, const StepConsts& C, double tol) {
                     if (!isfinite(I1)) {
                         S.qfin[q] = 0;
                         S.qdone[q] = 1;
-                    } else if (err <= tol * L11) {
+                    } else if (err <= tol * L11 ||
+                               (err * S.qd[q] <= S.qfloor[q] && fabs(I1) * S.qd[q] <= S.qcap[q])) {
                         S.qdone[q] = 1;
                     }
                 }
@@ -287,7 +288,8 @@ __device__ void ts_batch(CtlShared& S, const StepConsts& C, double tol) {
                     if (!isfinite(I1)) {
                         S.qfin[q] = 0;
                         S.qdone[q] = 1;
-                    } else if (err <= tol * L11) {
+                    } else if (err <= tol * L11 ||
+                               (err * S.qd[q] <= S.qfloor[q] && fabs(I1) * S.qd[q] <= S.qcap[q])) {
                         S.qdone[q] = 1;
                     }
                 }
@@ -339,6 +341,12 @@ __device__ void integrate_batch(CtlShared& S, const StepConsts& C, int nb) {
     if (tid < nb) {
         S.qc[tid] = 0.5 * (S.qa[tid] + S.qb[tid]);
         S.qd[tid] = 0.5 * (S.qb[tid] - S.qa[tid]);
+        // The integrand h |sum_k p_k e^{-i x lambda_k}| is bounded by h (sum_k |p_k| <= 1):
+        // tanh-sinh refinement stops once successive estimates agree to eps h (b-a), the rounding
+        // unit of that bound, provided the value is below termination (b-a) tolRate, where the
+        // integral counts as successful whatever its error (core/krylovTimeEvolver.cpp:607-608).
+        S.qfloor[tid] = DBL_EPSILON * S.h * (S.qb[tid] - S.qa[tid]);
+        S.qcap[tid] = termination * (S.qb[tid] - S.qa[tid]) * C.tolRate;
     }
     for (int method = 0; method < 3; method++) {
         if (tid == 0) {
