@@ -108,6 +108,132 @@ def test_spmv_vs_oracle(gpu, oracle_lib):
         assert np.linalg.norm(y - yr) <= 1e-14 * np.linalg.norm(yr)
 
 
+def _spmv_check(oracle_lib, M, D, v, c, r, seed=5):
+    x = random_state(D, seed)
+    for alpha in (1.0, -0.3j, 0.2 - 0.7j):
+        y = M.spmv(x, alpha)
+        yr = oracle_lib.spmv(D, v, c, r, x, alpha)
+        assert np.linalg.norm(y - yr) <= 1e-14 * np.linalg.norm(yr)
+
+
+def _distinct_offdiag(v, c, r):
+    off = c != r
+    bits = np.ascontiguousarray(v[off]).view(np.uint64).reshape(-1, 2)
+    return len(np.unique(bits, axis=0)), int(off.sum())
+
+
+@pytest.mark.parametrize("L", [6, 8, 10])
+def test_value_dictionary_storage(gpu, oracle_lib, monkeypatch, L):
+    """Ring operators are stored as a value dictionary whose table holds exactly the distinct
+    off-diagonal values (counted bit for bit on the host): in the basis order for a COO matrix,
+    in the lexicographic locality order for a built operator. SpMV matches the oracle and the
+    plain CSR row-block kernel, and an evolution takes the same steps with either."""
+    terms = as_tuples(_ring_terms(L))
+    ob = oracle_lib.Basis.basis(L, L)
+    v, c, r = oracle_lib.build_operator(ob, terms)
+    n_distinct, n_off = _distinct_offdiag(v, c, r)
+    gb = capi.Basis.basis(gpu, L, L)
+    Hg = capi.operator_build(gpu, gb, terms)
+    A = capi.Matrix.from_coo(gpu, v, c, r, ob.D)
+    for M, want in ((A, 2), (Hg, 3)):
+        fmt, nv, nbytes = M.storage()
+        assert fmt == want and nv == n_distinct
+        assert nbytes == 6 * n_off + 4 * (ob.D + 1) + 16 * ob.D
+        _spmv_check(oracle_lib, M, ob.D, v, c, r)
+    monkeypatch.setenv("TE_SPMV_VD", "0")
+    P = capi.Matrix.from_coo(gpu, v, c, r, ob.D)
+    monkeypatch.delenv("TE_SPMV_VD")
+    assert P.storage()[0] == 1 and P.storage()[2] == 20 * len(v) + 4 * (ob.D + 1)
+    x = random_state(ob.D, 3)
+    yv, yp = A.spmv(x, -1j), P.spmv(x, -1j)
+    assert np.linalg.norm(yv - yp) <= 1e-14 * np.linalg.norm(yp)
+    if L == 8:
+        rv = capi.Evolver(gpu, A, fock_state(ob.D), 3.0, 0.1, [], tol=1e-8).run()
+        rp = capi.Evolver(gpu, P, fock_state(ob.D), 3.0, 0.1, [], tol=1e-8).run()
+        assert rv["n_steps"] == rp["n_steps"] and rv["statusCode"] == rp["statusCode"]
+        assert np.linalg.norm(rv["state"] - rp["state"]) <= 1e-12
+        assert abs(rv["err"] - rp["err"]) <= 1e-12
+
+
+def test_locality_order_evolution(gpu, oracle_lib, monkeypatch):
+    """An operator built on a basis evolves in the lexicographic order of the states; every
+    observable kind (diagonal, non-diagonal sparse, vector, host callback), the sampled state
+    and the returned state come back in the basis order: same results as the basis-order
+    evolver (to rounding) and as the oracle (within the error bound)."""
+    L = N = 6
+    terms = as_tuples(_ring_terms(L))
+    ob = oracle_lib.Basis.basis(N, L)
+    Hc = oracle_lib.build_operator(ob, terms)
+    psi = random_state(ob.D, 21)
+    proj = random_state(ob.D, 22)
+    n2 = oracle_lib.build_operator(ob, as_tuples([createNumberOperator(2, 1)]))
+    ro = oracle_lib.evolve(ob.D, Hc, psi, 2.0, 0.1, [("sparse", Hc), ("vector", proj), ("sparse", n2)], tol=1e-8)
+    diag2 = np.zeros(ob.D)
+    diag2[n2[2]] = n2[0].real
+    res = {}
+    for order in ("1", "0"):
+        monkeypatch.setenv("TE_SPMV_ORDER", order)
+        gb = capi.Basis.basis(gpu, N, L)
+        H = capi.operator_build(gpu, gb, terms)
+        N2 = capi.operator_build(gpu, gb, [createNumberOperator(2, 1)])
+        assert H.storage()[0] == (3 if order == "1" else 2)
+        host = ("host", lambda st: float(np.sum(diag2 * np.abs(st) ** 2)))
+        res[order] = capi.Evolver(gpu, H, psi, 2.0, 0.1, [H, ("vector", proj), host, N2], tol=1e-8).run()
+    monkeypatch.delenv("TE_SPMV_ORDER")
+    rg, rb = res["1"], res["0"]
+    assert rg["n_steps"] == rb["n_steps"] and rg["statusCode"] == rb["statusCode"]
+    assert np.linalg.norm(rg["state"] - rb["state"]) <= 1e-12
+    assert np.abs(rg["observables"] - rb["observables"]).max() <= 1e-12
+    bound = ro["err"] + rg["err"]
+    assert np.linalg.norm(rg["state"] - ro["state"]) <= bound + 1e-10
+    for k, ko in ((0, 0), (1, 1), (2, 2), (3, 2)):
+        assert np.abs(rg["observables"][k] - ro["observables"][ko]).max() <= 1e-9
+
+
+def test_value_dictionary_fallbacks(gpu, oracle_lib):
+    """Matrices the dictionary cannot hold keep the plain CSR kernels, with correct products:
+    too many distinct values, repeated diagonal entries (COO duplicates are summed), rows longer
+    than a row block, a diagonal-only matrix (the evolver reads those as dense diagonals);
+    rows without a diagonal entry and a table of one value do qualify."""
+    rng = np.random.default_rng(0)
+    n = 3000
+    # random values: ~20000 distinct
+    r = rng.integers(0, n, 20000).astype(np.uint64)
+    c = rng.integers(0, n, 20000).astype(np.uint64)
+    v = rng.normal(size=20000) + 1j * rng.normal(size=20000)
+    M = capi.Matrix.from_coo(gpu, v, c, r, n)
+    assert M.storage()[0] == 1
+    _spmv_check(oracle_lib, M, n, v, c, r)
+    # few distinct values, but every diagonal entry given twice
+    idx = np.arange(n, dtype=np.uint64)
+    nb = (idx + 1) % n
+    r2 = np.concatenate([idx, idx, idx, nb])
+    c2 = np.concatenate([idx, idx, nb, idx])
+    v2 = np.concatenate([np.full(n, 0.5), np.full(n, 0.25), np.full(n, -1.0 + 0.5j), np.full(n, -1.0 - 0.5j)])
+    M = capi.Matrix.from_coo(gpu, v2, c2, r2, n)
+    assert M.storage()[0] == 1
+    _spmv_check(oracle_lib, M, n, v2, c2, r2)
+    # one row with 400 off-diagonal entries
+    r3 = np.concatenate([idx, np.zeros(400, np.uint64)])
+    c3 = np.concatenate([idx, np.arange(1, 401, dtype=np.uint64)])
+    v3 = np.concatenate([np.linspace(-1, 1, n), np.full(400, 0.125j)])
+    M = capi.Matrix.from_coo(gpu, v3, c3, r3, n)
+    assert M.storage()[0] == 0
+    _spmv_check(oracle_lib, M, n, v3, c3, r3)
+    # diagonal only
+    vv = np.linspace(-2, 3, n) + 0.5j
+    M = capi.Matrix.from_coo(gpu, vv, idx, idx, n)
+    assert M.storage()[0] == 1
+    _spmv_check(oracle_lib, M, n, vv, idx, idx)
+    # off-diagonal band with no diagonal in half of the rows
+    rr, cc = np.concatenate([idx[::2], idx, nb]), np.concatenate([idx[::2], nb, idx])
+    vv = np.concatenate([np.arange(n // 2) * 0.01, np.full(n, 0.3), np.full(n, 0.3)])
+    M = capi.Matrix.from_coo(gpu, vv, cc, rr, n)
+    fmt, nv, _ = M.storage()
+    assert fmt == 2 and nv == _distinct_offdiag(vv, cc, rr)[0] == 1
+    _spmv_check(oracle_lib, M, n, vv, cc, rr)
+
+
 # ---------------------------------------------------------------------------------------- evolution
 def _gpu_model(gpu, basis_kind, args, terms, obs_terms):
     b = capi.Basis.tensor(gpu, *args) if basis_kind == "tensor" else capi.Basis.basis(gpu, *args)
@@ -259,10 +385,13 @@ def test_column_split_spmv_and_evolution(gpu, oracle_lib, monkeypatch, part_mb):
     ob = oracle_lib.Basis.basis(N, L)
     v, c, r = oracle_lib.build_operator(ob, terms)
     monkeypatch.setenv("TE_SPMV_PART_MB", part_mb)
+    monkeypatch.setenv("TE_SPMV_VD", "0")            # the split applies to the plain CSR copy
     A = capi.Matrix.from_coo(gpu, v, c, r, ob.D)
     gb = capi.Basis.basis(gpu, N, L)
     Hg = capi.operator_build(gpu, gb, terms)
     monkeypatch.delenv("TE_SPMV_PART_MB")
+    monkeypatch.delenv("TE_SPMV_VD")
+    assert A.storage()[0] == 1 and Hg.storage()[0] == 1
     x = random_state(ob.D, 5)
     for M in (A, Hg):
         y = M.spmv(x, -0.3j)

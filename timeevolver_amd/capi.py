@@ -58,6 +58,7 @@ _SIGS = {
     "te_matrix_from_coo": (_c.c_int, [_p, _dp, _szp, _szp, _c.c_size_t, _c.c_size_t, _c.c_size_t, _pp]),
     "te_matrix_destroy": (None, [_p]),
     "te_matrix_info": (_c.c_int, [_p, _szp, _szp, _szp]),
+    "te_matrix_storage": (_c.c_int, [_p, _ip, _szp, _szp]),
     "te_matrix_norm1": (_c.c_int, [_p, _dp]),
     "te_matrix_to_coo": (_c.c_int, [_p, _dp, _szp, _szp]),
     "te_matrix_to_csr": (_c.c_int, [_p, _dp, _i64p, _ip]),
@@ -168,6 +169,13 @@ class Matrix:
                                            _ptr(r, _c.c_size_t), len(c), n_rows, n_cols, _c.byref(h)),
                   "te_matrix_from_coo")
         return cls(ctx, h)
+
+    def storage(self):
+        """(format, n_values, stream_bytes): 0 CSR lanes-per-row, 1 CSR row blocks,
+        2 value-dictionary row blocks; bytes of matrix data one product reads."""
+        f, nv, b = _c.c_int(), _c.c_size_t(), _c.c_size_t()
+        self.ctx.check(self.ctx.L.te_matrix_storage(self.h, _c.byref(f), _c.byref(nv), _c.byref(b)), "storage")
+        return f.value, nv.value, b.value
 
     def norm1(self):
         out = _c.c_double()

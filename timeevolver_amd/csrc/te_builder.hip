@@ -24,6 +24,9 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
+#include <climits>
+#include <cstdint>
 #include <vector>
 
 #include "te_internal.h"
@@ -57,6 +60,10 @@ struct te_basis {
     int* occ = nullptr;
     te::BasisDev dev{};
     std::vector<void*> allocations;
+    // Lexicographic order of the states (computed on first use by an operator build): the row
+    // order of the operators' value-dictionary copies, new -> old. Null if not applicable.
+    mutable int* lex_perm = nullptr;
+    mutable bool lex_done = false;
 };
 
 namespace te {
@@ -597,6 +604,7 @@ extern "C" int te_tensor_basis_create(te_ctx* ctx, int N1, int K1, int N2, int K
 extern "C" void te_basis_destroy(te_basis* B) {
     if (!B) return;
     if (B->occ) hipFree(B->occ);
+    if (B->lex_perm) hipFree(B->lex_perm);
     for (void* p : B->allocations) hipFree(p);
     delete B;
 }
@@ -638,6 +646,76 @@ extern "C" int te_basis_find(const te_basis* B, const int* occ, size_t count, in
     hipFree(d_out);
     return TE_OK;
 }
+
+namespace te {
+
+__global__ void lex_key_kernel(const int* __restrict__ occ, int64_t D, int K, unsigned long long radix,
+                               unsigned long long* keys, int* idx) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= D) return;
+    unsigned long long k = 0;
+    for (int m = 0; m < K; m++) k = k * radix + (unsigned long long)occ[i * K + m];
+    keys[i] = k;
+    idx[i] = (int)i;
+}
+
+// Lexicographic order of the basis states (mode 0 most significant, mixed radix max_occ + 1),
+// as new -> old. Neighbouring states that differ in the trailing modes get neighbouring
+// indices, so a product's gathers from x stay within a few thousand entries for the transitions
+// among those modes. Null (identity) when the key does not fit 64 bits or TE_SPMV_ORDER=0.
+static int basis_locality_order(te_ctx* ctx, const te_basis* B, const int** out) {
+    *out = nullptr;
+    const char* e = std::getenv("TE_SPMV_ORDER");
+    if (e && e[0] == '0') return TE_OK;
+    if (B->lex_done) {
+        *out = B->lex_perm;
+        return TE_OK;
+    }
+    B->lex_done = true;
+    if (B->D < 2 || B->D > INT32_MAX) return TE_OK;
+    const unsigned long long radix = (unsigned long long)B->max_occ + 1;
+    unsigned __int128 span = 1;
+    for (int m = 0; m < B->K; m++) {
+        span *= radix;
+        if (span > ((unsigned __int128)1 << 64)) return TE_OK;
+    }
+    int bits = 0;
+    while (bits < 64 && ((unsigned __int128)1 << bits) < span) bits++;
+    const int64_t D = B->D;
+    hipStream_t s = ctx->stream;
+    unsigned long long *k_in = nullptr, *k_out = nullptr;
+    int *i_in = nullptr, *perm = nullptr;
+    void* tmp = nullptr;
+    size_t bytes = 0;
+    int rc = TE_OK;
+    if (hipMalloc(&k_in, sizeof(unsigned long long) * D) != hipSuccess ||
+        hipMalloc(&k_out, sizeof(unsigned long long) * D) != hipSuccess ||
+        hipMalloc(&i_in, sizeof(int) * D) != hipSuccess || hipMalloc(&perm, sizeof(int) * D) != hipSuccess)
+        rc = fail(ctx, TE_ERR_ALLOC, "locality order allocation");
+    if (rc == TE_OK) {
+        hipLaunchKernelGGL(lex_key_kernel, dim3(blocks_for(D)), dim3(256), 0, s, B->occ, D, B->K, radix, k_in, i_in);
+        if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k_in, k_out, i_in, perm, (int)D, 0, bits, s) !=
+                hipSuccess ||
+            hipMalloc(&tmp, std::max<size_t>(bytes, 1)) != hipSuccess)
+            rc = fail(ctx, TE_ERR_HIP, "locality order sort setup");
+    }
+    if (rc == TE_OK && (hipcub::DeviceRadixSort::SortPairs(tmp, bytes, k_in, k_out, i_in, perm, (int)D, 0, bits, s) !=
+                            hipSuccess ||
+                        hipStreamSynchronize(s) != hipSuccess))
+        rc = fail(ctx, TE_ERR_HIP, "locality order sort");
+    void* frees[] = {k_in, k_out, i_in, tmp};
+    for (void* p : frees)
+        if (p) hipFree(p);
+    if (rc != TE_OK) {
+        if (perm) hipFree(perm);
+        return rc;
+    }
+    B->lex_perm = perm;
+    *out = perm;
+    return TE_OK;
+}
+
+}  // namespace te
 
 extern "C" int te_operator_build(te_ctx* ctx, const te_basis* B, const te_op_term* terms, int n_terms, te_matrix** out) {
     if (!ctx || !B || !out || n_terms < 0 || (n_terms > 0 && !terms)) return TE_ERR_ARG;
@@ -749,7 +827,9 @@ extern "C" int te_operator_build(te_ctx* ctx, const te_basis* B, const te_op_ter
         else if (hipStreamSynchronize(s) != hipSuccess) rc = fail(ctx, TE_ERR_HIP, "operator build failed");
     }
     if (rc == TE_OK) rc = finalize_matrix(ctx, A);
-    if (rc == TE_OK) rc = prepare_spmv(ctx, A);
+    const int* order = nullptr;
+    if (rc == TE_OK && !A->diagonal) rc = basis_locality_order(ctx, B, &order);
+    if (rc == TE_OK) rc = prepare_spmv(ctx, A, order);
     void* frees[] = {d_off, d_ops, d_modes, d_caps, d_coef, d_sqrt, ws, counts, rp64};
     for (void* p : frees)
         if (p) hipFree(p);

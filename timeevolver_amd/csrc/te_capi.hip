@@ -276,7 +276,21 @@ extern "C" void te_matrix_destroy(te_matrix* A) {
     if (A->diag_re) hipFree(A->diag_re);
     if (A->row_block) hipFree(A->row_block);
     free_parts(A);
+    free_vd(A);
     delete A;
+}
+
+extern "C" int te_matrix_storage(const te_matrix* A, int* format, size_t* n_values, size_t* stream_bytes) {
+    if (!A) return TE_ERR_ARG;
+    const int f = A->use_vd ? (A->vd.perm ? 3 : 2) : A->shape.kind;
+    if (format) *format = f;
+    if (n_values) *n_values = A->use_vd ? (size_t)A->vd.n_values : 0;
+    if (stream_bytes) {
+        const size_t n = (size_t)A->n_rows;
+        *stream_bytes = A->use_vd ? 6 * (size_t)A->vd.nnz + 4 * (n + 1) + 16 * n
+                                  : 20 * (size_t)A->nnz + 4 * (n + 1);
+    }
+    return TE_OK;
 }
 
 extern "C" int te_matrix_info(const te_matrix* A, size_t* n_rows, size_t* n_cols, size_t* nnz) {
@@ -409,6 +423,13 @@ struct te_evolver {
     ReduceScratch rs;
     int grid_stream = 8;
     std::vector<double2> e1;
+    // Locality order of H's value-dictionary copy (null: the basis order). psi0, U, the
+    // sampled states, tmp, diag_all and vec_dev are then in that order; tmp_ref and state_out
+    // hold states mapped back to the basis order for callers.
+    const int* perm = nullptr;      // new -> old
+    const int* iperm = nullptr;     // old -> new
+    double2* tmp_ref = nullptr;
+    double2* state_out = nullptr;
     // optional kernel timing: HIP event pairs on the context stream, per kernel kind
     bool profiling = false;
     std::vector<hipEvent_t> prof_ev[4];
@@ -424,7 +445,8 @@ static void evolver_free(te_evolver* ev) {
     hipSetDevice(ev->ctx->device);
     for (auto& v : ev->prof_ev)
         for (hipEvent_t e : v) hipEventDestroy(e);
-    double2* ptrs[] = {ev->psi0, ev->U, ev->sampled, ev->tmp, ev->tmp2, ev->wcoef, ev->Y, ev->ts_nodes, ev->gl15, ev->gl7};
+    double2* ptrs[] = {ev->psi0,    ev->U,   ev->sampled,  ev->tmp, ev->tmp2,    ev->wcoef,
+                       ev->Y,       ev->ts_nodes, ev->gl15, ev->gl7, ev->tmp_ref, ev->state_out};
     for (double2* p : ptrs)
         if (p) hipFree(p);
     if (ev->ks) hipFree(ev->ks);
@@ -522,7 +544,7 @@ static int evolver_create(te_ctx* ctx, te_matrix* H, const void* psi0, bool psi0
     ev->trace_cap = 1 << 20;
     TE_TRY(dmalloc(ctx, &ev->trace, ev->trace_cap));
     TE_TRY(dmalloc(ctx, &ev->obs_dev, (size_t)std::max(n_obs, 1) * ev->n_samples));
-    TE_TRY(reduce_scratch_alloc(ctx, ev->rs, std::max(ev->grid_stream, H->shape.grid)));
+    TE_TRY(reduce_scratch_alloc(ctx, ev->rs, std::max({ev->grid_stream, H->shape.grid, H->vd.grid})));
 
     if (psi0_on_device)
         TE_TRY(hipMemcpy(ev->psi0, psi0, sizeof(double2) * n, hipMemcpyDeviceToDevice) == hipSuccess ? TE_OK
@@ -540,6 +562,16 @@ static int evolver_create(te_ctx* ctx, te_matrix* H, const void* psi0, bool psi0
     if (std::fabs(std::sqrt(r.x) - 1.0) > params->tol) {
         evolver_free(ev);
         return fail(ctx, TE_ERR_FATAL, "Initial vector is not normalized");
+    }
+    if (H->use_vd && H->vd.perm) {            // work in the order of H's value-dictionary copy
+        ev->perm = H->vd.perm;
+        ev->iperm = H->vd.iperm;
+        TE_TRY(dmalloc(ctx, &ev->state_out, n));
+        TE_TRY(hipMemcpyAsync(ev->sampled, ev->psi0, sizeof(double2) * n, hipMemcpyDeviceToDevice, ctx->stream) ==
+                       hipSuccess
+                   ? TE_OK
+                   : TE_ERR_HIP);
+        launch_gather(ev->sampled, ev->perm, ev->psi0, n, ctx->stream);
     }
 
     // observables
@@ -566,9 +598,13 @@ static int evolver_create(te_ctx* ctx, te_matrix* H, const void* psi0, bool psi0
                 return fail(ctx, TE_ERR_ARG, "null vector observable");
             }
             TE_TRY(dmalloc(ctx, &ev->vec_dev[i], n));
-            TE_TRY(hipMemcpy(ev->vec_dev[i], o.vector, sizeof(double2) * n, hipMemcpyHostToDevice) == hipSuccess
+            double2* dst = ev->perm ? ev->sampled : ev->vec_dev[i];
+            TE_TRY(hipMemcpyAsync(dst, o.vector, sizeof(double2) * n, hipMemcpyHostToDevice, ctx->stream) ==
+                           hipSuccess
                        ? TE_OK
                        : TE_ERR_HIP);
+            if (ev->perm) launch_gather(ev->sampled, ev->perm, ev->vec_dev[i], n, ctx->stream);
+            TE_TRY(hipStreamSynchronize(ctx->stream) == hipSuccess ? TE_OK : TE_ERR_HIP);
             ev->generic_obs.push_back(i);
         } else if (o.kind == TE_OBS_HOST) {
             if (!o.callback) {
@@ -584,15 +620,21 @@ static int evolver_create(te_ctx* ctx, te_matrix* H, const void* psi0, bool psi0
     if (!diag_m.empty()) {
         TE_TRY(dmalloc(ctx, &ev->diag_all, diag_m.size() * (size_t)n));
         TE_TRY(dmalloc(ctx, &ev->diag_vals, diag_m.size() * (size_t)ev->n_samples));
-        for (size_t i = 0; i < diag_m.size(); i++)
-            TE_TRY(hipMemcpy(ev->diag_all + i * n, diag_m[i]->diag_re, sizeof(double) * n, hipMemcpyDeviceToDevice) ==
-                           hipSuccess
-                       ? TE_OK
-                       : TE_ERR_HIP);
+        for (size_t i = 0; i < diag_m.size(); i++) {
+            if (ev->perm)
+                launch_gather_re(diag_m[i]->diag_re, ev->perm, ev->diag_all + i * n, n, ctx->stream);
+            else
+                TE_TRY(hipMemcpyAsync(ev->diag_all + i * n, diag_m[i]->diag_re, sizeof(double) * n,
+                                      hipMemcpyDeviceToDevice, ctx->stream) == hipSuccess
+                           ? TE_OK
+                           : TE_ERR_HIP);
+        }
+        TE_TRY(hipStreamSynchronize(ctx->stream) == hipSuccess ? TE_OK : TE_ERR_HIP);
     }
     if (!ev->generic_obs.empty()) {
         TE_TRY(dmalloc(ctx, &ev->tmp, n));
         TE_TRY(dmalloc(ctx, &ev->tmp2, n));
+        if (ev->perm) TE_TRY(dmalloc(ctx, &ev->tmp_ref, n));
     }
 
     // quadrature tables
@@ -633,19 +675,25 @@ static int eval_generic(te_evolver* ev, long long sample) {
     te_ctx* ctx = ev->ctx;
     hipStream_t s = ctx->stream;
     bool host_copied = false;
+    // sparse observables and host callbacks see the state in the basis order
+    const double2* ref = ev->tmp;
+    if (ev->perm) {
+        launch_gather(ev->tmp, ev->iperm, ev->tmp_ref, ev->n, s);
+        ref = ev->tmp_ref;
+    }
     for (int i : ev->generic_obs) {
         const te_observable& o = ev->obs[i];
         double* dst = ev->obs_dev + (size_t)i * ev->n_samples + sample;
         if (o.kind == TE_OBS_SPARSE) {
-            launch_spmv(o.matrix, make_double2(1.0, 0.0), ev->tmp, ev->tmp2, s);
-            launch_zdotc(ev->tmp, ev->tmp2, ev->n, ev->rs, 1, s);
+            launch_spmv(o.matrix, make_double2(1.0, 0.0), ref, ev->tmp2, s);
+            launch_zdotc(ref, ev->tmp2, ev->n, ev->rs, 1, s);
             launch_store_obs(ev->rs.result, 1, 0, dst, s);
         } else if (o.kind == TE_OBS_VECTOR) {
             launch_zdotc(ev->tmp, ev->vec_dev[i], ev->n, ev->rs, 1, s);
             launch_store_obs(ev->rs.result, 1, 1, dst, s);
         } else {
             if (!host_copied) {
-                TE_HIP(ctx, hipMemcpyAsync(ev->host_state.data(), ev->tmp, sizeof(double2) * ev->n,
+                TE_HIP(ctx, hipMemcpyAsync(ev->host_state.data(), ref, sizeof(double2) * ev->n,
                                            hipMemcpyDeviceToHost, s));
                 TE_HIP(ctx, hipStreamSynchronize(s));
                 host_copied = true;
@@ -746,7 +794,7 @@ extern "C" int te_evolver_num_samples(const te_evolver* ev, size_t* n_samples) {
 
 extern "C" int te_evolver_state_device(te_evolver* ev, const void** state_dev) {
     if (!ev || !state_dev) return TE_ERR_ARG;
-    *state_dev = ev->sampled;
+    *state_dev = ev->perm ? ev->state_out : ev->sampled;
     return TE_OK;
 }
 
@@ -956,8 +1004,13 @@ extern "C" int te_evolver_run(te_evolver* ev, double* obs_values, double* evolve
                                        ev->diag_vals + k * ev->n_samples, sizeof(double) * ev->n_samples,
                                        hipMemcpyDeviceToHost, s));
     }
+    const double2* state = ev->sampled;
+    if (ev->perm) {                                   // back to the basis order
+        launch_gather(ev->sampled, ev->iperm, ev->state_out, n, s);
+        state = ev->state_out;
+    }
     if (evolved_state)
-        TE_HIP(ctx, hipMemcpyAsync(evolved_state, ev->sampled, sizeof(double2) * n, hipMemcpyDeviceToHost, s));
+        TE_HIP(ctx, hipMemcpyAsync(evolved_state, state, sizeof(double2) * n, hipMemcpyDeviceToHost, s));
     TE_HIP(ctx, hipStreamSynchronize(s));
     prof_collect(ev);
     if (obs_values)

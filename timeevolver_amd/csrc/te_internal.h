@@ -54,6 +54,45 @@ struct CsrPart {
     int64_t nnz = 0;
 };
 
+// Value-dictionary copy of a matrix (lossless): the off-diagonal entries keep their CSR order
+// with 32-bit columns and 16-bit indices into a table of the distinct values (compared bit for
+// bit), the diagonal is a dense vector. It streams 6 bytes per off-diagonal entry instead of 20.
+// Row blocks as in the stream kernel, with a smaller chunk so that the products (kVdCap entries)
+// and the table (kVdMaxValues entries) fit 40 KB of LDS with the block's flags: four blocks
+// per CU (160 KB).
+constexpr int kVdChunk = 1280;
+constexpr int kVdCap = 1536;        // rows up to kVdCap - kVdChunk + 1 = 257 off-diagonal entries
+constexpr int kVdMaxValues = 1008;
+constexpr int kVdPad = 256;         // zero entries after col/vidx: clamped loads stay in bounds
+struct VdCsr {
+    int* ptr = nullptr;             // n_rows + 1, off-diagonal entries
+    int* col = nullptr;
+    unsigned short* vidx = nullptr;
+    double2* vtab = nullptr;        // n_values distinct off-diagonal values
+    int n_values = 0;
+    double2* diag = nullptr;        // n_rows, zero where the row stores no diagonal entry
+    int2* blocks = nullptr;         // n_row_blocks + 1: (first row, its first entry)
+    int n_row_blocks = 0;
+    int grid = 0;
+    int64_t nnz = 0;
+    // Optional locality order of the rows and columns: the copy holds P A P^T with
+    // perm[new] = old, iperm[old] = new. Vectors the Lanczos kernels see are then in that order.
+    int* perm = nullptr;
+    int* iperm = nullptr;
+};
+
+// What the kernels read of a VdCsr.
+struct VdView {
+    const int* ptr;
+    const int* col;
+    const unsigned short* vidx;
+    const double2* vtab;
+    int n_values;
+    const double2* diag;
+    const int2* blocks;
+    int n_row_blocks;
+};
+
 }  // namespace te
 
 struct te_ctx {
@@ -81,6 +120,8 @@ struct te_matrix {
     int n_parts = 0;            // > 1: column-split copy in parts[] used by the stream kernels
     te::CsrPart parts[te::kMaxParts];
     double2* acc = nullptr;     // n_rows partial row sums between the passes of a split SpMV
+    bool use_vd = false;        // the stream kernels read the value-dictionary copy
+    te::VdCsr vd;
 };
 
 namespace te {
@@ -88,9 +129,15 @@ namespace te {
 // SpMV launch shape for a matrix (lanes per row from the mean row length, grid from the CUs).
 SpmvShape choose_shape(const te_ctx* ctx, int64_t n_rows, int64_t nnz);
 // Derives the row blocks of the stream kernel on the device and picks the kernel (call after
-// row_ptr is on the device).
-int prepare_spmv(te_ctx* ctx, te_matrix* A);
+// row_ptr is on the device). perm (device, n_rows entries, new -> old), when given, is the row
+// and column order of the value-dictionary copy the Lanczos kernels read; it is copied.
+int prepare_spmv(te_ctx* ctx, te_matrix* A, const int* perm = nullptr);
+// Vector gathers between the reference order and a matrix's locality order.
+// dst[i] = src[idx[i]] for i < n.
+void launch_gather(const double2* src, const int* idx, double2* dst, int64_t n, hipStream_t s);
+void launch_gather_re(const double* src, const int* idx, double* dst, int64_t n, hipStream_t s);
 void free_parts(te_matrix* A);
+void free_vd(te_matrix* A);
 
 // ---- kernels launched across translation units (te_spmv.hip) ----
 void launch_spmv(const te_matrix* A, double2 alpha, const double2* x, double2* y, hipStream_t s);

@@ -33,6 +33,7 @@ struct LanczosArgs {
     double2* partials;
     unsigned int* ticket;
     const double2* acc_in;   // partial row sums of the preceding column slices (or null)
+    VdView vd;               // value-dictionary copy (set by launch_lanczos_apply when used)
 };
 
 // Persistent control state of timeEvolve (core/krylovTimeEvolver.cpp:261-512), updated by

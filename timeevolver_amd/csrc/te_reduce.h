@@ -76,4 +76,28 @@ __device__ __forceinline__ int xcd_block_id() {
     return (b & 7) * (g >> 3) + (b >> 3);
 }
 
+// Workgroup barrier for LDS traffic only: waits for this wave's LDS operations (lgkmcnt), not
+// for its outstanding global loads, which __syncthreads() also drains. Loads issued for later
+// work stay in flight across it. The memory clobber keeps the compiler from moving memory
+// accesses across the barrier.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// Row-block schedule of the stream kernels: XCD x (blocks b with b % 8 == x) owns the
+// contiguous eighth [x nrb / 8, (x+1) nrb / 8) of the row blocks and its blocks take them
+// round-robin, so they sweep that range together and the neighbours that rows a few thousand
+// apart gather from x are still in the XCD's L2. Without a grid multiple of 8: round-robin.
+__device__ __forceinline__ void xcd_sweep(int nrb, int& first, int& end, int& stride) {
+    const int g = gridDim.x, b = blockIdx.x;
+    if ((g & 7) != 0) {
+        first = b;
+        end = nrb;
+        stride = g;
+        return;
+    }
+    const int x = b & 7;
+    first = (int)((long long)nrb * x / 8) + (b >> 3);
+    end = (int)((long long)nrb * (x + 1) / 8);
+    stride = g >> 3;
+}
+
 }  // namespace te

@@ -210,12 +210,10 @@ __device__ __forceinline__ void stream_rows(const int* __restrict__ rb, int nrb,
                                             const int* __restrict__ col, const double2* __restrict__ val,
                                             const double2* __restrict__ x, double2* prod, Epi& epi,
                                             const double2* __restrict__ acc_in) {
-    const int per = (nrb + gridDim.x - 1) / gridDim.x;
-    const int lb = xcd_block_id();
-    const int b0 = lb * per;
-    const int b1 = b0 + per < nrb ? b0 + per : nrb;
+    int b0, b1, bstride;
+    xcd_sweep(nrb, b0, b1, bstride);
     const int t = threadIdx.x;
-    for (int b = b0; b < b1; b++) {
+    for (int b = b0; b < b1; b += bstride) {
         const int r0 = rb[b], r1 = rb[b + 1];
         const int k0 = ptr[r0];
         const int nk = ptr[r1] - k0;
@@ -338,11 +336,198 @@ __global__ __launch_bounds__(kBlock) void lanczos_apply_stream_kernel(LanczosArg
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Value-dictionary stream kernels. Same row blocks as above (chunk kVdChunk). Row r is
+//   y_r = diag_r x_r + sum_k vtab[vidx_k] x[col_k]   (diagonal first, then column order)
+// with the table staged in LDS once per block.
+//
+// Software pipeline over a block's row blocks: when the iteration for row block b starts, its
+// columns and value indices are in registers and the bounds of the next one are known. The
+// iteration issues b's gathers from x and its row loads (row pointers, diagonal, x_r and the
+// epilogue's stream), the next block's column loads and the bounds of the one after, all before
+// waiting on any of them, so a row block costs about one memory round trip instead of four.
+// ---------------------------------------------------------------------------------------
+constexpr int kVdNI = kVdCap / kBlock;
+constexpr int kVdLds = kVdCap + kVdMaxValues;      // double2 entries: products, then the table
+
+// Branch-free: slots past the block's end load entry k0 (kVdPad zero entries follow the last
+// block, so k0 is in bounds also for an empty block at the end); their products are not stored.
+__device__ __forceinline__ void vd_load_cols(const VdView& m, int k0, int nk, int* c, unsigned* vi) {
+#pragma unroll
+    for (int i = 0; i < kVdNI; i++) {
+        const int k = threadIdx.x + i * kBlock;
+        const int kk = k0 + (k < nk ? k : 0);
+        c[i] = __builtin_nontemporal_load(&m.col[kk]);
+        vi[i] = __builtin_nontemporal_load(&m.vidx[kk]);
+    }
+}
+
+template <class Epi>
+__device__ __forceinline__ void stream_rows_vd(const VdView& m, const double2* __restrict__ x, double2* lds, Epi& epi) {
+    double2* prod = lds;
+    double2* vt = lds + kVdCap;
+    const int t = threadIdx.x;
+    int b, end, stride;
+    xcd_sweep(m.n_row_blocks, b, end, stride);
+    int2 lo = make_int2(0, 0), hi = lo, nlo = lo, nhi = lo;
+    int c[kVdNI];
+    unsigned vi[kVdNI];
+    if (b < end) {
+        lo = m.blocks[b];
+        hi = m.blocks[b + 1];
+    }
+    vd_load_cols(m, lo.y, hi.y - lo.y, c, vi);
+    if (b + stride < end) {
+        nlo = m.blocks[b + stride];
+        nhi = m.blocks[b + stride + 1];
+    }
+    for (int i = t; i < m.n_values; i += kBlock) vt[i] = m.vtab[i];
+    __syncthreads();
+    for (; b < end; b += stride) {
+        const int k0 = lo.y, nk = hi.y - lo.y;
+        const int r0 = lo.x, nrows = hi.x - lo.x;
+        double2 xv[kVdNI];
+#pragma unroll
+        for (int i = 0; i < kVdNI; i++) xv[i] = x[c[i]];
+        // this thread's first row
+        const int r = r0 + t;
+        int pa = 0, pe = 0;
+        double2 d = make_double2(0.0, 0.0), xr = d;
+        typename Epi::Pre pre{};
+        if (t < nrows) {
+            pa = m.ptr[r] - k0;
+            pe = m.ptr[r + 1] - k0;
+            d = load_stream(&m.diag[r]);
+            xr = x[r];
+            pre = epi.load(r);
+        }
+        // the next row block's columns, the bounds of the one after it
+        const int nb = b + stride;
+        int cn[kVdNI];
+        unsigned vin[kVdNI];
+        vd_load_cols(m, nlo.y, nb < end ? nhi.y - nlo.y : 0, cn, vin);
+        int2 nnlo = nlo, nnhi = nhi;
+        if (nb + stride < end) {
+            nnlo = m.blocks[nb + stride];
+            nnhi = m.blocks[nb + stride + 1];
+        }
+#pragma unroll
+        for (int i = 0; i < kVdNI; i++) {
+            const int k = t + i * kBlock;
+            if (k < nk) prod[k] = cmul(vt[vi[i]], xv[i]);
+        }
+        lds_barrier();
+        if (t < nrows) {
+            double sr = d.x * xr.x - d.y * xr.y, si = d.x * xr.y + d.y * xr.x;
+            int k = pa;
+            for (; k + 4 <= pe; k += 4) {        // four LDS reads in flight, sums in entry order
+                const double2 p0 = prod[k], p1 = prod[k + 1], p2 = prod[k + 2], p3 = prod[k + 3];
+                sr += p0.x;
+                si += p0.y;
+                sr += p1.x;
+                si += p1.y;
+                sr += p2.x;
+                si += p2.y;
+                sr += p3.x;
+                si += p3.y;
+            }
+            for (; k < pe; k++) {
+                sr += prod[k].x;
+                si += prod[k].y;
+            }
+            epi.row(r, make_double2(sr, si), xr, pre);
+        }
+        for (int rr = r + kBlock; rr < r0 + nrows; rr += kBlock) {     // row blocks of many short rows
+            const int a = m.ptr[rr] - k0, e = m.ptr[rr + 1] - k0;
+            const double2 x2 = x[rr];
+            const double2 d2 = load_stream(&m.diag[rr]);
+            double sr = d2.x * x2.x - d2.y * x2.y, si = d2.x * x2.y + d2.y * x2.x;
+            for (int k = a; k < e; k++) {
+                sr += prod[k].x;
+                si += prod[k].y;
+            }
+            epi.row(rr, make_double2(sr, si), x2, epi.load(rr));
+        }
+        lds_barrier();
+        lo = nlo;
+        hi = nhi;
+        nlo = nnlo;
+        nhi = nnhi;
+#pragma unroll
+        for (int i = 0; i < kVdNI; i++) {
+            c[i] = cn[i];
+            vi[i] = vin[i];
+        }
+    }
+}
+
+struct SpmvEpiVd {
+    using Pre = int;
+    double2 alpha;
+    double2* y;
+    __device__ __forceinline__ Pre load(int) const { return 0; }
+    __device__ __forceinline__ void row(int r, double2 s, double2, Pre) { y[r] = cmul(alpha, s); }
+};
+
+__global__ __launch_bounds__(kBlock) void spmv_vd_kernel(VdView m, const double2* __restrict__ x,
+                                                         double2* __restrict__ y, double2 alpha) {
+    __shared__ double2 lds[kVdLds];
+    SpmvEpiVd epi{alpha, y};
+    stream_rows_vd(m, x, lds, epi);
+}
+
+// As ApplyEpi, with u_j[r] (= x[r]) and u_{j-1}[r] loaded ahead by the row loop.
+struct ApplyEpiVd {
+    using Pre = double2;
+    double2 c1;
+    double c2;
+    const double2* up;
+    double2* w;
+    double2 dot;
+    __device__ __forceinline__ Pre load(int r) const {
+        return up ? load_stream(&up[r]) : make_double2(0.0, 0.0);
+    }
+    __device__ __forceinline__ void row(int r, double2 s, double2 u, Pre p) {
+        double2 wr = cmul(c1, s);
+        if (up) {
+            wr.x += c2 * p.x;
+            wr.y += c2 * p.y;
+        }
+        store_stream(&w[r], wr);
+        dot.x += u.x * wr.x + u.y * wr.y;
+        dot.y += u.x * wr.y - u.y * wr.x;
+    }
+};
+
+__global__ __launch_bounds__(kBlock) void lanczos_apply_vd_kernel(LanczosArgs a) {
+    KrylovScalars* ks = a.ks;
+    if (ks->broken) return;
+    __shared__ double2 lds[kVdLds];
+    const int j = a.j;
+    const double sj = ks->s[j];
+    ApplyEpiVd epi;
+    epi.c1 = make_double2(a.alpha.x * sj, a.alpha.y * sj);
+    epi.c2 = j > 0 ? ks->beta[j - 1] * ks->s[j - 1] : 0.0;
+    epi.up = j > 0 ? a.U + (size_t)(j - 1) * a.ld : nullptr;
+    epi.w = a.U + (size_t)(j + 1) * a.ld;
+    epi.dot = make_double2(0.0, 0.0);
+    stream_rows_vd(a.vd, a.U + (size_t)j * a.ld, lds, epi);
+    double2 bs = block_sum2<kBlock>(epi.dot, lds);      // the products' LDS is free again
+    if (threadIdx.x == 0) a.partials[blockIdx.x] = bs;
+    if (last_block_arrive(a.ticket)) {
+        double2 tot = reduce_partials<kBlock>(a.partials, gridDim.x, 1, lds);
+        if (threadIdx.x == 0) {
+            ks->a[j] = make_double2(sj * tot.x, sj * tot.y);
+            *a.ticket = 0;
+        }
+    }
+}
+
 // Row blocks on the merge-path coordinate ptr[r] + r (entries plus rows before row r):
 // rb[b] = first row with ptr[r] + r >= b * kStreamChunk, rb[nrb] = n. A block then holds at
 // most kStreamChunk rows and at most kStreamChunk - 1 + (longest row) entries, so neither
 // long rows nor long runs of empty rows unbalance the blocks. Also returns the longest row.
-__global__ void row_blocks_kernel(const int* __restrict__ ptr, int64_t n, int nrb, int* rb, int* maxlen) {
+__global__ void row_blocks_kernel(const int* __restrict__ ptr, int64_t n, int nrb, int chunk, int* rb, int* maxlen) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         const int len = ptr[i + 1] - ptr[i];
@@ -352,7 +537,7 @@ __global__ void row_blocks_kernel(const int* __restrict__ ptr, int64_t n, int nr
         if (i == nrb) {
             rb[i] = (int)n;
         } else {
-            const int64_t target = i * (int64_t)kStreamChunk;
+            const int64_t target = i * (int64_t)chunk;
             int64_t lo = 0, hi = n;                 // first r in [0, n] with ptr[r] + r >= target
             while (lo < hi) {
                 const int64_t mid = (lo + hi) / 2;
@@ -408,7 +593,17 @@ __global__ __launch_bounds__(kBlock) void zdotc_kernel(const double2* __restrict
         default: MACRO(64); break;\
     }
 
+static VdView vd_view(const te_matrix* A) {
+    const VdCsr& v = A->vd;
+    return VdView{v.ptr, v.col, v.vidx, v.vtab, v.n_values, v.diag, v.blocks, v.n_row_blocks};
+}
+
+// y = alpha A x in the reference order (a reordered value-dictionary copy is for the evolver).
 void launch_spmv(const te_matrix* A, double2 alpha, const double2* x, double2* y, hipStream_t s) {
+    if (A->use_vd && !A->vd.perm) {
+        hipLaunchKernelGGL(spmv_vd_kernel, dim3(A->vd.grid), dim3(kBlock), 0, s, vd_view(A), x, y, alpha);
+        return;
+    }
     const int grid = A->shape.grid;
     if (A->shape.kind == 1) {
         if (A->n_parts > 1) {
@@ -434,6 +629,11 @@ void launch_spmv(const te_matrix* A, double2 alpha, const double2* x, double2* y
 }
 
 void launch_lanczos_apply(LanczosArgs a, const te_matrix* A, hipStream_t s) {
+    if (A->use_vd) {
+        a.vd = vd_view(A);
+        hipLaunchKernelGGL(lanczos_apply_vd_kernel, dim3(A->vd.grid), dim3(kBlock), 0, s, a);
+        return;
+    }
     const SpmvShape& sh = A->shape;
     const int grid = sh.grid;
     if (sh.kind == 1) {
@@ -508,8 +708,9 @@ static int exclusive_scan64(te_ctx* ctx, const int64_t* in, int64_t* out, int64_
     return TE_OK;
 }
 
-static int row_blocks(te_ctx* ctx, const int* ptr, int64_t n, int64_t nnz, int** rb_out, int* nrb_out, int* maxlen) {
-    const int nrb = (int)((nnz + n + kStreamChunk - 1) / kStreamChunk);
+static int row_blocks(te_ctx* ctx, const int* ptr, int64_t n, int64_t nnz, int** rb_out, int* nrb_out, int* maxlen,
+                      int chunk = kStreamChunk) {
+    const int nrb = (int)((nnz + n + chunk - 1) / chunk);
     int* rb = nullptr;
     TE_HIP(ctx, hipMalloc(&rb, sizeof(int) * (nrb + 1)));
     int* d_max = nullptr;
@@ -517,7 +718,7 @@ static int row_blocks(te_ctx* ctx, const int* ptr, int64_t n, int64_t nnz, int**
     TE_HIP(ctx, hipMemsetAsync(d_max, 0, sizeof(int), ctx->stream));
     const int64_t items = std::max<int64_t>(n, nrb + 1);
     hipLaunchKernelGGL(row_blocks_kernel, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, ctx->stream, ptr, n, nrb,
-                       rb, d_max);
+                       chunk, rb, d_max);
     int mx = 0;
     TE_HIP(ctx, hipMemcpyAsync(&mx, d_max, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     TE_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -584,7 +785,284 @@ void free_parts(te_matrix* A) {
     A->n_parts = 0;
 }
 
-int prepare_spmv(te_ctx* ctx, te_matrix* A) {
+// ---------------------------------------------------------------------------------------
+// Value-dictionary build: distinct off-diagonal values found with a lock-free open-addressing
+// set of 64-bit keys (every probe loop is bounded by the slot count), then checked bit for
+// bit against the table; any overflow, key collision or repeated diagonal entry keeps the
+// plain CSR kernels.
+// ---------------------------------------------------------------------------------------
+constexpr int kVdSlots = 4096;      // power of two, >= 4 kVdMaxValues
+
+__device__ __forceinline__ unsigned long long vd_bits(double v) { return (unsigned long long)__double_as_longlong(v); }
+
+__device__ __forceinline__ unsigned long long vd_key(double2 v) {
+    unsigned long long h = vd_bits(v.x) * 0x9E3779B97F4A7C15ull;
+    h ^= vd_bits(v.y) + 0x632BE59BD9B4E019ull + (h << 6) + (h >> 2);
+    h ^= h >> 31;
+    h *= 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 29;
+    return h | 1ull;                // 0 marks an empty slot
+}
+
+enum { kVdFlagRepeatedDiag = 1, kVdFlagFull = 2, kVdFlagMismatch = 4 };
+
+// Per row of the copy (old row perm[r], or r): off-diagonal count; flags[2] = longest such row.
+__global__ void vd_count_kernel(const int* __restrict__ ptr, const int* __restrict__ col, const int* __restrict__ perm,
+                                int64_t n, int64_t* offlen, int* flags) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > n) return;
+    if (r == n) {
+        offlen[n] = 0;
+        return;
+    }
+    const int o = perm ? perm[r] : (int)r;
+    int nd = 0;
+    for (int k = ptr[o]; k < ptr[o + 1]; k++) nd += col[k] == o;
+    const int len = ptr[o + 1] - ptr[o] - nd;
+    offlen[r] = len;
+    atomicMax(&flags[2], len);
+    if (nd > 1) atomicOr(flags, (int)kVdFlagRepeatedDiag);
+}
+
+__global__ void vd_insert_kernel(const int* __restrict__ ptr, const int* __restrict__ col,
+                                 const double2* __restrict__ val, int64_t n, unsigned long long* keys,
+                                 double2* slot_val, int* flags) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    for (int k = ptr[r]; k < ptr[r + 1]; k++) {
+        if (col[k] == (int)r) continue;
+        const double2 v = val[k];
+        const unsigned long long key = vd_key(v);
+        int s = (int)(key & (kVdSlots - 1));
+        bool placed = false;
+        for (int probe = 0; probe < kVdSlots; probe++) {
+            unsigned long long cur = __hip_atomic_load(&keys[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (cur == 0ull) {
+                cur = atomicCAS(&keys[s], 0ull, key);
+                if (cur == 0ull) slot_val[s] = v;     // this thread created the slot
+            }
+            if (cur == 0ull || cur == key) {
+                placed = true;
+                break;
+            }
+            s = (s + 1) & (kVdSlots - 1);
+        }
+        if (!placed) atomicOr(flags, (int)kVdFlagFull);
+    }
+}
+
+// Dense table in slot order (one thread: 4096 slots).
+__global__ void vd_compact_kernel(const unsigned long long* keys, const double2* slot_val, int* dense, double2* vtab,
+                                  int* n_values) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    int c = 0;
+    for (int s = 0; s < kVdSlots; s++) {
+        if (keys[s] != 0ull) {
+            dense[s] = c;
+            if (c < kVdMaxValues) vtab[c] = slot_val[s];
+            c++;
+        } else {
+            dense[s] = -1;
+        }
+    }
+    *n_values = c;
+}
+
+// Row r of the copy = old row perm[r] (or r), columns mapped through iperm and sorted.
+__global__ void vd_fill_kernel(const int* __restrict__ ptr, const int* __restrict__ col,
+                               const double2* __restrict__ val, const int* __restrict__ perm,
+                               const int* __restrict__ iperm, int64_t n, const int64_t* __restrict__ offs,
+                               const unsigned long long* __restrict__ keys, const double2* __restrict__ slot_val,
+                               const int* __restrict__ dense, int* vptr, int* vcol, unsigned short* vidx,
+                               double2* diag, int* flags) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > n) return;
+    vptr[r] = (int)offs[r];
+    if (r == n) return;
+    const int o0 = (int)offs[r];
+    int o = o0;
+    const int orow = perm ? perm[r] : (int)r;
+    double2 d = make_double2(0.0, 0.0);
+    for (int k = ptr[orow]; k < ptr[orow + 1]; k++) {
+        const int c = col[k];
+        const double2 v = val[k];
+        if (c == orow) {
+            d = v;
+            continue;
+        }
+        const unsigned long long key = vd_key(v);
+        int s = (int)(key & (kVdSlots - 1)), found = -1;
+        for (int probe = 0; probe < kVdSlots; probe++) {
+            const unsigned long long cur = keys[s];
+            if (cur == key) {
+                found = s;
+                break;
+            }
+            if (cur == 0ull) break;
+            s = (s + 1) & (kVdSlots - 1);
+        }
+        int idx = 0;
+        if (found < 0 || vd_bits(slot_val[found].x) != vd_bits(v.x) || vd_bits(slot_val[found].y) != vd_bits(v.y))
+            atomicOr(flags, (int)kVdFlagMismatch);
+        else
+            idx = dense[found];
+        vcol[o] = iperm ? iperm[c] : c;
+        vidx[o] = (unsigned short)idx;
+        o++;
+    }
+    diag[r] = d;
+    if (iperm) {                         // insertion sort of the row by new column (rows are short)
+        for (int a = o0 + 1; a < o; a++) {
+            const int cc = vcol[a];
+            const unsigned short vv = vidx[a];
+            int b = a - 1;
+            while (b >= o0 && vcol[b] > cc) {
+                vcol[b + 1] = vcol[b];
+                vidx[b + 1] = vidx[b];
+                b--;
+            }
+            vcol[b + 1] = cc;
+            vidx[b + 1] = vv;
+        }
+    }
+}
+
+__global__ void vd_blocks_kernel(const int* __restrict__ rb, const int* __restrict__ ptr, int nrb, int2* blocks) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= nrb) blocks[i] = make_int2(rb[i], ptr[rb[i]]);
+}
+
+__global__ void invert_perm_kernel(const int* __restrict__ perm, int64_t n, int* iperm) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) iperm[perm[i]] = (int)i;
+}
+
+__global__ void gather_kernel(const double2* __restrict__ src, const int* __restrict__ idx, double2* __restrict__ dst,
+                              int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[idx[i]];
+}
+
+__global__ void gather_re_kernel(const double* __restrict__ src, const int* __restrict__ idx, double* __restrict__ dst,
+                                 int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[idx[i]];
+}
+
+static unsigned gather_grid(int64_t n) { return (unsigned)std::min<int64_t>((n + 255) / 256, 8192); }
+
+void launch_gather(const double2* src, const int* idx, double2* dst, int64_t n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(gather_kernel, dim3(gather_grid(n)), dim3(256), 0, s, src, idx, dst, n);
+}
+
+void launch_gather_re(const double* src, const int* idx, double* dst, int64_t n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(gather_re_kernel, dim3(gather_grid(n)), dim3(256), 0, s, src, idx, dst, n);
+}
+
+void free_vd(te_matrix* A) {
+    VdCsr& v = A->vd;
+    if (v.ptr) hipFree(v.ptr);
+    if (v.col) hipFree(v.col);
+    if (v.vidx) hipFree(v.vidx);
+    if (v.vtab) hipFree(v.vtab);
+    if (v.diag) hipFree(v.diag);
+    if (v.blocks) hipFree(v.blocks);
+    if (v.perm) hipFree(v.perm);
+    if (v.iperm) hipFree(v.iperm);
+    v = VdCsr{};
+    A->use_vd = false;
+}
+
+// Builds A->vd from the CSR on the device. Returns TE_OK also when the matrix does not qualify
+// (A->use_vd stays false); errors only for HIP failures.
+static int build_vd(te_ctx* ctx, te_matrix* A, const int* perm) {
+    free_vd(A);
+    const int64_t n = A->n_rows;
+    if (n != A->n_cols || n == 0) return TE_OK;
+    hipStream_t st = ctx->stream;
+    int64_t *offlen = nullptr, *offs = nullptr;
+    unsigned long long* keys = nullptr;
+    double2* slot_val = nullptr;
+    int *dense = nullptr, *flags = nullptr;     // flags[0]: kVdFlag*, flags[1]: n_values, flags[2]: longest row
+    TE_HIP(ctx, hipMalloc(&offlen, sizeof(int64_t) * (n + 1)));
+    TE_HIP(ctx, hipMalloc(&offs, sizeof(int64_t) * (n + 1)));
+    TE_HIP(ctx, hipMalloc(&keys, sizeof(unsigned long long) * kVdSlots));
+    TE_HIP(ctx, hipMalloc(&slot_val, sizeof(double2) * kVdSlots));
+    TE_HIP(ctx, hipMalloc(&dense, sizeof(int) * kVdSlots));
+    TE_HIP(ctx, hipMalloc(&flags, sizeof(int) * 3));
+    VdCsr& v = A->vd;
+    TE_HIP(ctx, hipMalloc(&v.vtab, sizeof(double2) * kVdMaxValues));
+    if (perm) {
+        TE_HIP(ctx, hipMalloc(&v.perm, sizeof(int) * n));
+        TE_HIP(ctx, hipMalloc(&v.iperm, sizeof(int) * n));
+        TE_HIP(ctx, hipMemcpyAsync(v.perm, perm, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(invert_perm_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, v.perm, n, v.iperm);
+    }
+    TE_HIP(ctx, hipMemsetAsync(keys, 0, sizeof(unsigned long long) * kVdSlots, st));
+    TE_HIP(ctx, hipMemsetAsync(flags, 0, sizeof(int) * 3, st));
+    const unsigned g1 = (unsigned)((n + 256) / 256);
+    hipLaunchKernelGGL(vd_count_kernel, dim3(g1), dim3(256), 0, st, A->row_ptr, A->col, v.perm, n, offlen, flags);
+    hipLaunchKernelGGL(vd_insert_kernel, dim3(g1), dim3(256), 0, st, A->row_ptr, A->col, A->val, n, keys, slot_val,
+                       flags);
+    hipLaunchKernelGGL(vd_compact_kernel, dim3(1), dim3(64), 0, st, keys, slot_val, dense, v.vtab, flags + 1);
+    int rc = exclusive_scan64(ctx, offlen, offs, n + 1);
+    int h[3] = {0, 0, 0};
+    int64_t nnz_off = 0;
+    if (rc == TE_OK) {
+        TE_HIP(ctx, hipMemcpyAsync(h, flags, sizeof(int) * 3, hipMemcpyDeviceToHost, st));
+        TE_HIP(ctx, hipMemcpyAsync(&nnz_off, offs + n, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        TE_HIP(ctx, hipStreamSynchronize(st));
+    }
+    // rows longer than a row block can hold are not stored (nor sorted, which is quadratic)
+    bool ok = rc == TE_OK && h[0] == 0 && h[1] <= kVdMaxValues && h[2] + kVdChunk - 1 <= kVdCap;
+    if (ok) {
+        v.n_values = h[1];
+        v.nnz = nnz_off;
+        TE_HIP(ctx, hipMalloc(&v.ptr, sizeof(int) * (n + 1)));
+        TE_HIP(ctx, hipMalloc(&v.col, sizeof(int) * (nnz_off + kVdPad)));
+        TE_HIP(ctx, hipMalloc(&v.vidx, sizeof(unsigned short) * (nnz_off + kVdPad)));
+        TE_HIP(ctx, hipMemsetAsync(v.col + nnz_off, 0, sizeof(int) * kVdPad, st));
+        TE_HIP(ctx, hipMemsetAsync(v.vidx + nnz_off, 0, sizeof(unsigned short) * kVdPad, st));
+        TE_HIP(ctx, hipMalloc(&v.diag, sizeof(double2) * n));
+        hipLaunchKernelGGL(vd_fill_kernel, dim3(g1), dim3(256), 0, st, A->row_ptr, A->col, A->val, v.perm, v.iperm, n,
+                           offs, keys, slot_val, dense, v.ptr, v.col, v.vidx, v.diag, flags);
+        TE_HIP(ctx, hipMemcpyAsync(h, flags, sizeof(int), hipMemcpyDeviceToHost, st));
+        TE_HIP(ctx, hipStreamSynchronize(st));
+        ok = h[0] == 0;
+    }
+    hipFree(offlen);
+    hipFree(offs);
+    hipFree(keys);
+    hipFree(slot_val);
+    hipFree(dense);
+    hipFree(flags);
+    if (rc != TE_OK) return rc;
+    int maxlen = 0;
+    if (ok) {
+        int* rb = nullptr;
+        rc = row_blocks(ctx, v.ptr, n, v.nnz, &rb, &v.n_row_blocks, &maxlen, kVdChunk);
+        if (rc != TE_OK) return rc;
+        ok = v.n_row_blocks > 0 && maxlen + kVdChunk - 1 <= kVdCap;
+        if (ok) {
+            TE_HIP(ctx, hipMalloc(&v.blocks, sizeof(int2) * (v.n_row_blocks + 1)));
+            hipLaunchKernelGGL(vd_blocks_kernel, dim3((unsigned)((v.n_row_blocks + 256) / 256)), dim3(256), 0, st, rb,
+                               v.ptr, v.n_row_blocks, v.blocks);
+            TE_HIP(ctx, hipStreamSynchronize(st));
+        }
+        hipFree(rb);
+    }
+    if (!ok) {
+        free_vd(A);
+        return TE_OK;
+    }
+    const char* gmul = std::getenv("TE_VD_BLOCKS_PER_CU");          // launch-shape sweeps
+    const int cap = std::max(8, ctx->num_cus * (gmul ? std::max(1, std::atoi(gmul)) : 4));
+    v.grid = (int)((std::min(v.n_row_blocks, cap) + 7) / 8 * 8);
+    A->use_vd = true;
+    return TE_OK;
+}
+
+int prepare_spmv(te_ctx* ctx, te_matrix* A, const int* perm) {
     A->shape = choose_shape(ctx, A->n_rows, A->nnz);
     const int nrb = (int)((A->nnz + A->n_rows + kStreamChunk - 1) / kStreamChunk);
     if (A->row_block) hipFree(A->row_block);
@@ -595,7 +1073,7 @@ int prepare_spmv(te_ctx* ctx, te_matrix* A) {
     TE_HIP(ctx, hipMemsetAsync(d_max, 0, sizeof(int), ctx->stream));
     const int64_t items = std::max<int64_t>(A->n_rows, nrb + 1);
     hipLaunchKernelGGL(row_blocks_kernel, dim3((unsigned)((items + 255) / 256)), dim3(256), 0, ctx->stream, A->row_ptr,
-                       (int64_t)A->n_rows, nrb, A->row_block, d_max);
+                       (int64_t)A->n_rows, nrb, kStreamChunk, A->row_block, d_max);
     int mx = 0;
     TE_HIP(ctx, hipMemcpyAsync(&mx, d_max, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     TE_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -609,6 +1087,13 @@ int prepare_spmv(te_ctx* ctx, te_matrix* A) {
         const char* gmul = std::getenv("TE_SPMV_BLOCKS_PER_CU");   // launch-shape sweeps
         const int cap = std::max(8, ctx->num_cus * (gmul ? std::max(1, std::atoi(gmul)) : 8));
         A->shape.grid = (int)((std::min(nrb, cap) + 7) / 8 * 8);
+        // Diagonal matrices (observables of n_k type) are read as dense diagonals by the evolver.
+        const char* vd = std::getenv("TE_SPMV_VD");                    // 0: plain CSR stream kernels
+        if (!(vd && vd[0] == '0') && !A->diagonal) {
+            int rc = build_vd(ctx, A, perm);
+            if (rc != TE_OK) return rc;
+            if (A->use_vd) return TE_OK;
+        }
         // Column slices of x no larger than TE_SPMV_PART_MB (MiB). Off by default: measured on
         // the L = N = 14 ring (x = 321 MB), 2, 3 and 5 slices were 10-30 % slower than one
         // pass (the slices are not retained by the MALL under this access pattern).
