@@ -10,8 +10,9 @@ a "step" is one complete timeEvolve() from psi0 to T with all inputs resident in
 
 Prints one JSON line. `value` is the time-to-T in seconds (lower is better). `roofline` is
 the fused Lanczos SpMV kernel: algorithmic bytes per launch = the matrix bytes its storage
-format reads once (value dictionary: 6 per off-diagonal entry + 4 (D+1) row pointers + 16 D
-diagonal; plain CSR: 20 nnz + 4 (D+1)) + 16 D (x = v_j, read once, also for <v_j, w>) + 16 D
+format reads once (value dictionary: 6 per stored off-diagonal slot, sliced-ELL padding
+included, + row pointers or slice offsets and row lengths + 16 D diagonal; plain CSR:
+20 nnz + 4 (D+1)) + 16 D (x = v_j, read once, also for <v_j, w>) + 16 D
 (w written) + 16 D (v_{j-1}), divided by its mean duration from HIP events recorded around
 every launch on the library's stream during the timed runs. `csr_equivalent_GBps` prices the
 same launches at the reference's CSR bytes (20 nnz + 4 (D+1) + 48 D). `cpu_baseline` times the
@@ -114,7 +115,8 @@ def main():
     b_csr = 20 * H.nnz + 4 * (D + 1) + 48 * D
     achieved = b_apply / t_apply / 1e9
     kname = {0: "te::lanczos_apply_kernel", 1: "te::lanczos_apply_stream_kernel",
-             2: "te::lanczos_apply_vd_kernel", 3: "te::lanczos_apply_vd_kernel"}[fmt]
+             2: "te::lanczos_apply_vd_kernel", 3: "te::lanczos_apply_vd_kernel",
+             4: "te::lanczos_apply_sell_kernel"}[fmt]
     kernel_ms = {k: round(sum(kt[k][0] for kt in kts) / len(kts), 3) for k in kts[0]}
     kernel_launches = {k: kts[0][k][1] for k in kts[0]}
 
@@ -142,7 +144,8 @@ def main():
         "spmv_GBps": round(b_spmv / t_apply / 1e9, 1),
         "csr_equivalent_GBps": round(b_csr / t_apply / 1e9, 1),
         "storage": {"format": ["csr-lanes", "csr-rowblocks", "value-dictionary",
-                               "value-dictionary, lexicographic state order"][fmt], "n_values": n_values,
+                               "value-dictionary, locality order",
+                               "value-dictionary sliced ELL, locality order"][fmt], "n_values": n_values,
                     "matrix_bytes": b_matrix},
         "kernel_ms_per_run": kernel_ms,
         "kernel_launches_per_run": kernel_launches,

@@ -58,9 +58,10 @@ int te_matrix_info(const te_matrix* A, size_t* n_rows, size_t* n_cols, size_t* n
 /* No counterpart in the reference: the device storage the evolver's Lanczos products read
  * for A. format: 0 CSR with lanes per row, 1 CSR row blocks, 2 value-dictionary row blocks
  * (32-bit columns and 16-bit indices into n_values distinct off-diagonal values, dense
- * diagonal), 3 the same over the basis states in lexicographic order (operators from
- * te_operator_build; te_spmv then reads the CSR row blocks in the basis order).
- * stream_bytes: bytes of matrix data one product reads in that format. */
+ * diagonal), 3 the same in a locality order of the rows (lexicographic order of the basis
+ * states for operators from te_operator_build), 4 its sliced-ELL form (64-row slices, rows
+ * sorted by length within windows of 512). With formats 3 and 4 te_spmv reads the CSR row
+ * blocks in the basis order. stream_bytes: bytes of matrix data one product reads. */
 int te_matrix_storage(const te_matrix* A, int* format, size_t* n_values, size_t* stream_bytes);
 /* smatrix::norm1 (core/matrixDataTypes.cpp:67-79): maximal absolute column sum. */
 int te_matrix_norm1(te_matrix* A, double* out);

@@ -132,13 +132,30 @@ def test_value_dictionary_storage(gpu, oracle_lib, monkeypatch, L):
     ob = oracle_lib.Basis.basis(L, L)
     v, c, r = oracle_lib.build_operator(ob, terms)
     n_distinct, n_off = _distinct_offdiag(v, c, r)
+    monkeypatch.setenv("TE_SPMV_SELL", "0")
     gb = capi.Basis.basis(gpu, L, L)
     Hg = capi.operator_build(gpu, gb, terms)
     A = capi.Matrix.from_coo(gpu, v, c, r, ob.D)
+    monkeypatch.delenv("TE_SPMV_SELL")
     for M, want in ((A, 2), (Hg, 3)):
         fmt, nv, nbytes = M.storage()
         assert fmt == want and nv == n_distinct
         assert nbytes == 6 * n_off + 4 * (ob.D + 1) + 16 * ob.D
+        _spmv_check(oracle_lib, M, ob.D, v, c, r)
+    # sliced ELL (default): 64-row slices padded to their longest row, rows in the basis order
+    # (COO matrix) or in the lexicographic order of the states (built operator)
+    ns = (ob.D + 63) // 64
+    rowlen = np.bincount(r[c != r].astype(np.int64), minlength=ob.D)
+    states = ob.states().astype(np.int64)
+    lex = np.lexsort(states.T[::-1])                  # mode 0 most significant
+    for M, order in ((capi.Matrix.from_coo(gpu, v, c, r, ob.D), np.arange(ob.D)),
+                     (capi.operator_build(gpu, gb, terms), lex)):
+        fmt, nv, nbytes = M.storage()
+        assert fmt == 4 and nv == n_distinct
+        lens = np.zeros(ns * 64, np.int64)
+        lens[:ob.D] = rowlen[order]
+        slots = 64 * int(lens.reshape(ns, 64).max(1).sum())
+        assert nbytes == 6 * slots + 128 * ns + 4 * (ns + 1) + 16 * ob.D
         _spmv_check(oracle_lib, M, ob.D, v, c, r)
     monkeypatch.setenv("TE_SPMV_VD", "0")
     P = capi.Matrix.from_coo(gpu, v, c, r, ob.D)
@@ -171,19 +188,24 @@ def test_locality_order_evolution(gpu, oracle_lib, monkeypatch):
     diag2 = np.zeros(ob.D)
     diag2[n2[2]] = n2[0].real
     res = {}
-    for order in ("1", "0"):
-        monkeypatch.setenv("TE_SPMV_ORDER", order)
+    configs = {"sell+lex": ({}, 4), "rows+lex": ({"TE_SPMV_SELL": "0"}, 3),
+               "rows+basis": ({"TE_SPMV_SELL": "0", "TE_SPMV_ORDER": "0"}, 2)}
+    for name, (env, fmt) in configs.items():
+        for k, val in env.items():
+            monkeypatch.setenv(k, val)
         gb = capi.Basis.basis(gpu, N, L)
         H = capi.operator_build(gpu, gb, terms)
         N2 = capi.operator_build(gpu, gb, [createNumberOperator(2, 1)])
-        assert H.storage()[0] == (3 if order == "1" else 2)
+        for k in env:
+            monkeypatch.delenv(k)
+        assert H.storage()[0] == fmt
         host = ("host", lambda st: float(np.sum(diag2 * np.abs(st) ** 2)))
-        res[order] = capi.Evolver(gpu, H, psi, 2.0, 0.1, [H, ("vector", proj), host, N2], tol=1e-8).run()
-    monkeypatch.delenv("TE_SPMV_ORDER")
-    rg, rb = res["1"], res["0"]
-    assert rg["n_steps"] == rb["n_steps"] and rg["statusCode"] == rb["statusCode"]
-    assert np.linalg.norm(rg["state"] - rb["state"]) <= 1e-12
-    assert np.abs(rg["observables"] - rb["observables"]).max() <= 1e-12
+        res[name] = capi.Evolver(gpu, H, psi, 2.0, 0.1, [H, ("vector", proj), host, N2], tol=1e-8).run()
+    rg, rb = res["sell+lex"], res["rows+basis"]
+    for other in (res["rows+lex"], rb):
+        assert rg["n_steps"] == other["n_steps"] and rg["statusCode"] == other["statusCode"]
+        assert np.linalg.norm(rg["state"] - other["state"]) <= 1e-12
+        assert np.abs(rg["observables"] - other["observables"]).max() <= 1e-12
     bound = ro["err"] + rg["err"]
     assert np.linalg.norm(rg["state"] - ro["state"]) <= bound + 1e-10
     for k, ko in ((0, 0), (1, 1), (2, 2), (3, 2)):
@@ -230,8 +252,46 @@ def test_value_dictionary_fallbacks(gpu, oracle_lib):
     vv = np.concatenate([np.arange(n // 2) * 0.01, np.full(n, 0.3), np.full(n, 0.3)])
     M = capi.Matrix.from_coo(gpu, vv, cc, rr, n)
     fmt, nv, _ = M.storage()
-    assert fmt == 2 and nv == _distinct_offdiag(vv, cc, rr)[0] == 1
+    assert fmt == 4 and nv == _distinct_offdiag(vv, cc, rr)[0] == 1
     _spmv_check(oracle_lib, M, n, vv, cc, rr)
+
+
+@pytest.mark.parametrize("storage", ["sell", "rows", "csr"])
+def test_irregular_rows_evolution(gpu, oracle_lib, monkeypatch, storage):
+    """A Hermitian matrix with rows of 0 to 40 off-diagonal entries (few distinct values),
+    evolved with each storage of the Lanczos product: sliced ELL with length-sorted windows and
+    padded slots, value-dictionary row blocks, plain CSR row blocks. All within the error bound
+    of the oracle, and the same steps and states to rounding."""
+    rng = np.random.default_rng(7)
+    n = 4099                                             # not a multiple of the 64-row slice
+    deg = rng.integers(0, 21, n)
+    rows = np.repeat(np.arange(n), deg)
+    cols = rng.integers(0, n, len(rows))
+    keep = rows != cols
+    rows, cols = rows[keep], cols[keep]
+    key = np.unique(rows * n + cols)
+    rows, cols = key // n, key % n
+    vals = np.array([0.3, -0.7, 0.2 + 0.4j, 1.1j])[rng.integers(0, 4, len(rows))]
+    # Hermitian: A + A^H, then a diagonal
+    r = np.concatenate([rows, cols, np.arange(n)])
+    c = np.concatenate([cols, rows, np.arange(n)])
+    v = np.concatenate([vals, np.conj(vals), rng.normal(size=n)])
+    r, c, v = r.astype(np.uint64), c.astype(np.uint64), v.astype(complex)
+    psi = random_state(n, 3)
+    env = {"sell": {}, "rows": {"TE_SPMV_SELL": "0"}, "csr": {"TE_SPMV_VD": "0"}}[storage]
+    for k, val in env.items():
+        monkeypatch.setenv(k, val)
+    H = capi.Matrix.from_coo(gpu, v, c, r, n)
+    for k in env:
+        monkeypatch.delenv(k)
+    fmt = H.storage()[0]
+    assert fmt == {"sell": 4, "rows": 2, "csr": 1}[storage]
+    _spmv_check(oracle_lib, H, n, v, c, r)
+    rg = capi.Evolver(gpu, H, psi, 1.0, 0.1, [("vector", random_state(n, 4))], tol=1e-8, m=30).run()
+    ro = oracle_lib.evolve(n, (v, c, r), psi, 1.0, 0.1, [("vector", random_state(n, 4))], tol=1e-8, m=30)
+    assert rg["n_steps"] == ro["n_steps"] and rg["statusCode"] == ro["statusCode"]
+    assert np.linalg.norm(rg["state"] - ro["state"]) <= rg["err"] + ro["err"] + 1e-10
+    assert np.abs(rg["observables"] - ro["observables"]).max() <= 1e-9
 
 
 # ---------------------------------------------------------------------------------------- evolution

@@ -282,13 +282,14 @@ extern "C" void te_matrix_destroy(te_matrix* A) {
 
 extern "C" int te_matrix_storage(const te_matrix* A, int* format, size_t* n_values, size_t* stream_bytes) {
     if (!A) return TE_ERR_ARG;
-    const int f = A->use_vd ? (A->vd.perm ? 3 : 2) : A->shape.kind;
+    const int f = A->use_vd ? (A->vd.sell ? 4 : A->vd.perm ? 3 : 2) : A->shape.kind;
     if (format) *format = f;
     if (n_values) *n_values = A->use_vd ? (size_t)A->vd.n_values : 0;
     if (stream_bytes) {
-        const size_t n = (size_t)A->n_rows;
-        *stream_bytes = A->use_vd ? 6 * (size_t)A->vd.nnz + 4 * (n + 1) + 16 * n
-                                  : 20 * (size_t)A->nnz + 4 * (n + 1);
+        const size_t n = (size_t)A->n_rows, ns = (size_t)A->vd.n_slices;
+        *stream_bytes = f == 4   ? 6 * (size_t)A->vd.sell_entries + 2 * ns * te::kSlice + 4 * (ns + 1) + 16 * n
+                        : f >= 2 ? 6 * (size_t)A->vd.nnz + 4 * (n + 1) + 16 * n
+                                 : 20 * (size_t)A->nnz + 4 * (n + 1);
     }
     return TE_OK;
 }
@@ -544,7 +545,8 @@ static int evolver_create(te_ctx* ctx, te_matrix* H, const void* psi0, bool psi0
     ev->trace_cap = 1 << 20;
     TE_TRY(dmalloc(ctx, &ev->trace, ev->trace_cap));
     TE_TRY(dmalloc(ctx, &ev->obs_dev, (size_t)std::max(n_obs, 1) * ev->n_samples));
-    TE_TRY(reduce_scratch_alloc(ctx, ev->rs, std::max({ev->grid_stream, H->shape.grid, H->vd.grid})));
+    TE_TRY(reduce_scratch_alloc(ctx, ev->rs,
+                                std::max({ev->grid_stream, H->shape.grid, H->vd.grid, H->vd.sell_grid})));
 
     if (psi0_on_device)
         TE_TRY(hipMemcpy(ev->psi0, psi0, sizeof(double2) * n, hipMemcpyDeviceToDevice) == hipSuccess ? TE_OK

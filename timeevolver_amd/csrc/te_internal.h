@@ -79,6 +79,39 @@ struct VdCsr {
     // perm[new] = old, iperm[old] = new. Vectors the Lanczos kernels see are then in that order.
     int* perm = nullptr;
     int* iperm = nullptr;
+    // Sliced-ELL form of the same entries (when every row has at most 255 off-diagonal entries):
+    // slices of kSlice consecutive rows, each padded to its longest row and stored slot-major,
+    // slot j of row s*kSlice + l at sell_off[s] + j*kSlice + l, so a wavefront reads every slot
+    // of its 64 rows with one coalesced load. A row's entries left of the diagonal fill its
+    // first slots, those right of it its last slots, padding in between: the far neighbours of
+    // the slice's rows, which sit at the same offsets for rows that share their leading modes,
+    // then share slots and their gathers from x share cache lines. Optionally (TE_SELL_SIGMA=1)
+    // rows are ordered by decreasing length within windows of kSellWindow (part of perm): less
+    // padding, worse gathers.
+    bool sell = false;
+    int* sell_off = nullptr;            // n_slices + 1
+    int* sell_col = nullptr;
+    unsigned short* sell_vidx = nullptr;
+    unsigned short* sell_lr = nullptr;  // n_slices * kSlice: left count | right count << 8
+    int n_slices = 0;
+    int64_t sell_entries = 0;
+    int sell_grid = 0;
+};
+
+constexpr int kSlice = 64;
+constexpr int kSellWindow = 512;
+
+// What the sliced-ELL kernels read.
+struct SellView {
+    const int* off;
+    const int* col;
+    const unsigned short* vidx;
+    const unsigned short* lr;
+    const double2* vtab;
+    int n_values;
+    const double2* diag;
+    int n_slices;
+    int64_t n;
 };
 
 // What the kernels read of a VdCsr.

@@ -34,6 +34,7 @@ struct LanczosArgs {
     unsigned int* ticket;
     const double2* acc_in;   // partial row sums of the preceding column slices (or null)
     VdView vd;               // value-dictionary copy (set by launch_lanczos_apply when used)
+    SellView sell;           // its sliced-ELL form (idem)
 };
 
 // Persistent control state of timeEvolve (core/krylovTimeEvolver.cpp:261-512), updated by

@@ -100,4 +100,20 @@ __device__ __forceinline__ void xcd_sweep(int nrb, int& first, int& end, int& st
     stride = g >> 3;
 }
 
+// The same at wavefront granularity, for blocks of kBlock threads: the waves of XCD x take the
+// items of its eighth round-robin.
+__device__ __forceinline__ void xcd_sweep_waves(int n_items, int& first, int& end, int& stride) {
+    const int g = gridDim.x, b = blockIdx.x, w = threadIdx.x >> 6, wpb = blockDim.x >> 6;
+    if ((g & 7) != 0) {
+        first = b * wpb + w;
+        end = n_items;
+        stride = g * wpb;
+        return;
+    }
+    const int x = b & 7;
+    first = (int)((long long)n_items * x / 8) + (b >> 3) * wpb + w;
+    end = (int)((long long)n_items * (x + 1) / 8);
+    stride = (g >> 3) * wpb;
+}
+
 }  // namespace te

@@ -523,6 +523,119 @@ __global__ __launch_bounds__(kBlock) void lanczos_apply_vd_kernel(LanczosArgs a)
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Sliced-ELL value-dictionary kernels: one lane per row, a wavefront per slice of kSlice rows.
+// Row r: y_r = diag_r x_r + sum_j vtab[vidx_j] x[col_j] (diagonal first, then column order: the
+// same sums as the row-block kernel). Slots past a row's length are loaded (col 0, value 0) and
+// not added. Four slots are loaded before any is used, and the wavefronts of a CU work on
+// different slices, so gathers from x overlap without LDS staging or barriers.
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ void sell_load_group(const int* cp, const unsigned short* vp, int* c, unsigned* v) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        c[q] = __builtin_nontemporal_load(cp + q * kSlice);
+        v[q] = __builtin_nontemporal_load(vp + q * kSlice);
+    }
+}
+
+template <class Epi>
+__device__ __forceinline__ void sell_rows(const SellView& m, const double2* __restrict__ x, double2* vt, Epi& epi) {
+    for (int i = threadIdx.x; i < m.n_values; i += blockDim.x) vt[i] = m.vtab[i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    int s, end, stride;
+    xcd_sweep_waves(m.n_slices, s, end, stride);
+    s = __builtin_amdgcn_readfirstlane(s);            // wave-uniform: slice offsets via scalar loads
+    end = __builtin_amdgcn_readfirstlane(end);
+    stride = __builtin_amdgcn_readfirstlane(stride);
+    for (; s < end; s += stride) {
+        const int o = m.off[s];
+        const int ls = (m.off[s + 1] - o) / kSlice;                   // slots of this slice
+        const int ng = (ls + 3) >> 2;                                  // groups of 4 slots
+        const int64_t r = (int64_t)s * kSlice + lane;
+        const unsigned lr = m.lr[r];
+        const int nl = (int)(lr & 255u), rfirst = ls - (int)(lr >> 8);  // slots [0,nl) and [rfirst,ls)
+        double2 xr = make_double2(0.0, 0.0), d = xr;
+        typename Epi::Pre pre{};
+        if (r < m.n) {
+            xr = x[r];
+            d = load_stream(&m.diag[r]);
+            pre = epi.load((int)r);
+        }
+        const int* cp = m.col + o + lane;
+        const unsigned short* vp = m.vidx + o + lane;
+        int c[4];
+        unsigned v[4];
+        sell_load_group(cp, vp, c, v);
+        double sr = d.x * xr.x - d.y * xr.y, si = d.x * xr.y + d.y * xr.x;
+        // Group g's gathers and group g+1's column loads are in flight together (the last group
+        // is peeled so that the loop body is straight-line code for the wait counters). Slots
+        // past the slice end (last group) read the next slice's entries or the zero padding: in
+        // bounds, valid columns, never added.
+        auto accumulate = [&](const unsigned* vv, const double2* xv, int g) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const double2 p = cmul(vt[vv[q]], xv[q]);
+                const int j = 4 * g + q;
+                const bool in = (j < nl) | ((j >= rfirst) & (j < ls));   // selects, not branches
+                sr = in ? sr + p.x : sr;
+                si = in ? si + p.y : si;
+            }
+        };
+        for (int g = 0; g + 1 < ng; g++) {
+            double2 xv[4];
+            unsigned vv[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                xv[q] = x[c[q]];
+                vv[q] = v[q];
+            }
+            cp += 4 * kSlice;
+            vp += 4 * kSlice;
+            sell_load_group(cp, vp, c, v);
+            accumulate(vv, xv, g);
+        }
+        if (ng > 0) {
+            double2 xv[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) xv[q] = x[c[q]];
+            accumulate(v, xv, ng - 1);
+        }
+        if (r < m.n) epi.row((int)r, make_double2(sr, si), xr, pre);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void spmv_sell_kernel(SellView m, const double2* __restrict__ x,
+                                                           double2* __restrict__ y, double2 alpha) {
+    __shared__ double2 vt[kVdMaxValues];
+    SpmvEpiVd epi{alpha, y};
+    sell_rows(m, x, vt, epi);
+}
+
+__global__ __launch_bounds__(kBlock) void lanczos_apply_sell_kernel(LanczosArgs a) {
+    KrylovScalars* ks = a.ks;
+    if (ks->broken) return;
+    __shared__ double2 vt[kVdMaxValues];
+    const int j = a.j;
+    const double sj = ks->s[j];
+    ApplyEpiVd epi;
+    epi.c1 = make_double2(a.alpha.x * sj, a.alpha.y * sj);
+    epi.c2 = j > 0 ? ks->beta[j - 1] * ks->s[j - 1] : 0.0;
+    epi.up = j > 0 ? a.U + (size_t)(j - 1) * a.ld : nullptr;
+    epi.w = a.U + (size_t)(j + 1) * a.ld;
+    epi.dot = make_double2(0.0, 0.0);
+    sell_rows(a.sell, a.U + (size_t)j * a.ld, vt, epi);
+    double2 bs = block_sum2<kBlock>(epi.dot, vt);       // the table's LDS is free again
+    if (threadIdx.x == 0) a.partials[blockIdx.x] = bs;
+    if (last_block_arrive(a.ticket)) {
+        double2 tot = reduce_partials<kBlock>(a.partials, gridDim.x, 1, vt);
+        if (threadIdx.x == 0) {
+            ks->a[j] = make_double2(sj * tot.x, sj * tot.y);
+            *a.ticket = 0;
+        }
+    }
+}
+
 // Row blocks on the merge-path coordinate ptr[r] + r (entries plus rows before row r):
 // rb[b] = first row with ptr[r] + r >= b * kStreamChunk, rb[nrb] = n. A block then holds at
 // most kStreamChunk rows and at most kStreamChunk - 1 + (longest row) entries, so neither
@@ -598,8 +711,17 @@ static VdView vd_view(const te_matrix* A) {
     return VdView{v.ptr, v.col, v.vidx, v.vtab, v.n_values, v.diag, v.blocks, v.n_row_blocks};
 }
 
+static SellView sell_view(const te_matrix* A) {
+    const VdCsr& v = A->vd;
+    return SellView{v.sell_off, v.sell_col, v.sell_vidx, v.sell_lr, v.vtab, v.n_values, v.diag, v.n_slices, A->n_rows};
+}
+
 // y = alpha A x in the reference order (a reordered value-dictionary copy is for the evolver).
 void launch_spmv(const te_matrix* A, double2 alpha, const double2* x, double2* y, hipStream_t s) {
+    if (A->use_vd && !A->vd.perm && A->vd.sell) {
+        hipLaunchKernelGGL(spmv_sell_kernel, dim3(A->vd.sell_grid), dim3(kBlock), 0, s, sell_view(A), x, y, alpha);
+        return;
+    }
     if (A->use_vd && !A->vd.perm) {
         hipLaunchKernelGGL(spmv_vd_kernel, dim3(A->vd.grid), dim3(kBlock), 0, s, vd_view(A), x, y, alpha);
         return;
@@ -629,6 +751,12 @@ void launch_spmv(const te_matrix* A, double2 alpha, const double2* x, double2* y
 }
 
 void launch_lanczos_apply(LanczosArgs a, const te_matrix* A, hipStream_t s) {
+    if (A->use_vd && A->vd.sell) {
+        const VdCsr& v = A->vd;
+        a.sell = sell_view(A);
+        hipLaunchKernelGGL(lanczos_apply_sell_kernel, dim3(v.sell_grid), dim3(kBlock), 0, s, a);
+        return;
+    }
     if (A->use_vd) {
         a.vd = vd_view(A);
         hipLaunchKernelGGL(lanczos_apply_vd_kernel, dim3(A->vd.grid), dim3(kBlock), 0, s, a);
@@ -959,6 +1087,72 @@ void launch_gather_re(const double* src, const int* idx, double* dst, int64_t n,
     if (n > 0) hipLaunchKernelGGL(gather_re_kernel, dim3(gather_grid(n)), dim3(256), 0, s, src, idx, dst, n);
 }
 
+// Within each window of kSellWindow rows of the base order, rows by decreasing off-diagonal
+// count (ties in base order): out[w0 + rank] = base[w0 + i] (or w0 + i).
+__global__ __launch_bounds__(kSellWindow) void sigma_sort_kernel(const int64_t* __restrict__ len, int64_t n,
+                                                                 const int* __restrict__ base, int* out) {
+    __shared__ int sl[kSellWindow];
+    const int64_t w0 = (int64_t)blockIdx.x * kSellWindow;
+    const int i = threadIdx.x;
+    const int64_t r = w0 + i;
+    const int cnt = (int)(n - w0 < kSellWindow ? n - w0 : kSellWindow);
+    sl[i] = r < n ? (int)len[r] : -1;
+    __syncthreads();
+    if (r >= n) return;
+    const int li = sl[i];
+    int rank = 0;
+    for (int j2 = 0; j2 < cnt; j2++) {
+        const int lj = sl[j2];
+        rank += (lj > li) || (lj == li && j2 < i);
+    }
+    out[w0 + rank] = base ? base[r] : (int)r;
+}
+
+__global__ void sell_len_kernel(const int* __restrict__ vptr, int64_t n, int n_slices, int64_t* slen) {
+    const int s2 = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s2 > n_slices) return;
+    if (s2 == n_slices) {
+        slen[s2] = 0;
+        return;
+    }
+    int mx = 0;
+    for (int l = 0; l < kSlice; l++) {
+        const int64_t r = (int64_t)s2 * kSlice + l;
+        if (r < n) mx = max(mx, vptr[r + 1] - vptr[r]);
+    }
+    slen[s2] = (int64_t)mx * kSlice;
+}
+
+// Row r of slice s: entries left of the diagonal in slots [0, nl), right of it in [L - nr, L),
+// both in column order; padding slots gather x[r] (col r, value index 0) and are not added.
+__global__ void sell_fill_kernel(const int* __restrict__ vptr, const int* __restrict__ vcol,
+                                 const unsigned short* __restrict__ vvidx, int64_t n, int n_slices,
+                                 const int64_t* __restrict__ soff, int* sell_off, int* scol, unsigned short* svidx,
+                                 unsigned short* lr) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= (int64_t)n_slices * kSlice) return;
+    const int s2 = (int)(r / kSlice), l = (int)(r % kSlice);
+    if (l == 0) {
+        sell_off[s2] = (int)soff[s2];
+        if (s2 == n_slices - 1) sell_off[n_slices] = (int)soff[n_slices];
+    }
+    const int len = r < n ? vptr[r + 1] - vptr[r] : 0;
+    const int k0 = r < n ? vptr[r] : 0;
+    int nl = 0;
+    while (nl < len && vcol[k0 + nl] < (int)r) nl++;
+    const int nr = len - nl;
+    lr[r] = (unsigned short)(nl | (nr << 8));
+    const int64_t o = soff[s2];
+    const int L = (int)((soff[s2 + 1] - o) / kSlice);
+    const int pad_col = r < n ? (int)r : 0;
+    for (int j = 0; j < L; j++) {
+        const int64_t idx = o + (int64_t)j * kSlice + l;
+        const int k = j < nl ? j : (j >= L - nr ? nl + (j - (L - nr)) : -1);
+        scol[idx] = k >= 0 ? vcol[k0 + k] : pad_col;
+        svidx[idx] = k >= 0 ? vvidx[k0 + k] : (unsigned short)0;
+    }
+}
+
 void free_vd(te_matrix* A) {
     VdCsr& v = A->vd;
     if (v.ptr) hipFree(v.ptr);
@@ -969,8 +1163,64 @@ void free_vd(te_matrix* A) {
     if (v.blocks) hipFree(v.blocks);
     if (v.perm) hipFree(v.perm);
     if (v.iperm) hipFree(v.iperm);
+    if (v.sell_off) hipFree(v.sell_off);
+    if (v.sell_col) hipFree(v.sell_col);
+    if (v.sell_vidx) hipFree(v.sell_vidx);
+    if (v.sell_lr) hipFree(v.sell_lr);
     v = VdCsr{};
     A->use_vd = false;
+}
+
+// Sliced-ELL form of A->vd (rows of at most 255 off-diagonal entries).
+static int build_sell(te_ctx* ctx, te_matrix* A) {
+    VdCsr& v = A->vd;
+    hipStream_t st = ctx->stream;
+    const int64_t n = A->n_rows;
+    const int64_t ns64 = (n + kSlice - 1) / kSlice;
+    if (ns64 >= INT32_MAX / kSlice) return TE_OK;
+    const int ns = (int)ns64;
+    int64_t *slen = nullptr, *soff = nullptr;
+    TE_HIP(ctx, hipMalloc(&slen, sizeof(int64_t) * (ns + 1)));
+    TE_HIP(ctx, hipMalloc(&soff, sizeof(int64_t) * (ns + 1)));
+    hipLaunchKernelGGL(sell_len_kernel, dim3((unsigned)((ns + 256) / 256)), dim3(256), 0, st, v.ptr, n, ns, slen);
+    int rc = exclusive_scan64(ctx, slen, soff, ns + 1);
+    int64_t total = 0;
+    if (rc == TE_OK) {
+        TE_HIP(ctx, hipMemcpyAsync(&total, soff + ns, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        TE_HIP(ctx, hipStreamSynchronize(st));
+    }
+    if (rc == TE_OK && total < INT32_MAX) {
+        TE_HIP(ctx, hipMalloc(&v.sell_off, sizeof(int) * (ns + 1)));
+        // a slice's last group of 4 slots may run up to 3 slots past its end: zero padding
+        const int64_t pad = 4 * kSlice;
+        TE_HIP(ctx, hipMalloc(&v.sell_col, sizeof(int) * (total + pad)));
+        TE_HIP(ctx, hipMalloc(&v.sell_vidx, sizeof(unsigned short) * (total + pad)));
+        TE_HIP(ctx, hipMemsetAsync(v.sell_col + total, 0, sizeof(int) * pad, st));
+        TE_HIP(ctx, hipMemsetAsync(v.sell_vidx + total, 0, sizeof(unsigned short) * pad, st));
+        TE_HIP(ctx, hipMalloc(&v.sell_lr, sizeof(unsigned short) * (size_t)ns * kSlice));
+        hipLaunchKernelGGL(sell_fill_kernel, dim3((unsigned)((ns64 * kSlice + 255) / 256)), dim3(256), 0, st, v.ptr,
+                           v.col, v.vidx, n, ns, soff, v.sell_off, v.sell_col, v.sell_vidx, v.sell_lr);
+        TE_HIP(ctx, hipStreamSynchronize(st));
+        v.n_slices = ns;
+        v.sell_entries = total;
+        // Two resident blocks per CU (8 waves): the waves of an XCD sweep its slices together,
+        // and a narrower window of slices in flight keeps more of x's lines in the XCD's L2.
+        // On the L = N = 14 ring 2 beat 1, 3 and 4 (the occupancy limit).
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lanczos_apply_sell_kernel, kBlock, 0) !=
+                hipSuccess ||
+            per_cu < 1)
+            per_cu = 1;
+        const char* gmul = std::getenv("TE_SELL_BLOCKS_PER_CU");    // launch-shape sweeps
+        const int want = gmul ? std::max(1, std::atoi(gmul)) : 2;
+        const int cap = std::max(8, ctx->num_cus * std::min(want, per_cu));
+        const int blocks = (ns + kBlock / 64 - 1) / (kBlock / 64);
+        v.sell_grid = (int)((std::min(blocks, cap) + 7) / 8 * 8);
+        v.sell = true;
+    }
+    hipFree(slen);
+    hipFree(soff);
+    return rc;
 }
 
 // Builds A->vd from the CSR on the device. Returns TE_OK also when the matrix does not qualify
@@ -992,15 +1242,26 @@ static int build_vd(te_ctx* ctx, te_matrix* A, const int* perm) {
     TE_HIP(ctx, hipMalloc(&flags, sizeof(int) * 3));
     VdCsr& v = A->vd;
     TE_HIP(ctx, hipMalloc(&v.vtab, sizeof(double2) * kVdMaxValues));
-    if (perm) {
+    const char* sell_env = std::getenv("TE_SPMV_SELL");              // 0: row-block kernel only
+    const bool want_sell = !(sell_env && sell_env[0] == '0');
+    const char* sigma_env = std::getenv("TE_SELL_SIGMA");            // 1: length sort in windows
+    const bool want_sigma = want_sell && sigma_env && sigma_env[0] == '1';
+    const unsigned g1 = (unsigned)((n + 256) / 256);
+    if (perm || want_sigma) {
         TE_HIP(ctx, hipMalloc(&v.perm, sizeof(int) * n));
         TE_HIP(ctx, hipMalloc(&v.iperm, sizeof(int) * n));
-        TE_HIP(ctx, hipMemcpyAsync(v.perm, perm, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
+        if (want_sigma) {       // refine the base order: rows by decreasing length within windows
+            TE_HIP(ctx, hipMemsetAsync(flags, 0, sizeof(int) * 3, st));
+            hipLaunchKernelGGL(vd_count_kernel, dim3(g1), dim3(256), 0, st, A->row_ptr, A->col, perm, n, offlen, flags);
+            hipLaunchKernelGGL(sigma_sort_kernel, dim3((unsigned)((n + kSellWindow - 1) / kSellWindow)),
+                               dim3(kSellWindow), 0, st, offlen, n, perm, v.perm);
+        } else {
+            TE_HIP(ctx, hipMemcpyAsync(v.perm, perm, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
+        }
         hipLaunchKernelGGL(invert_perm_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, v.perm, n, v.iperm);
     }
     TE_HIP(ctx, hipMemsetAsync(keys, 0, sizeof(unsigned long long) * kVdSlots, st));
     TE_HIP(ctx, hipMemsetAsync(flags, 0, sizeof(int) * 3, st));
-    const unsigned g1 = (unsigned)((n + 256) / 256);
     hipLaunchKernelGGL(vd_count_kernel, dim3(g1), dim3(256), 0, st, A->row_ptr, A->col, v.perm, n, offlen, flags);
     hipLaunchKernelGGL(vd_insert_kernel, dim3(g1), dim3(256), 0, st, A->row_ptr, A->col, A->val, n, keys, slot_val,
                        flags);
@@ -1059,6 +1320,10 @@ static int build_vd(te_ctx* ctx, te_matrix* A, const int* perm) {
     const int cap = std::max(8, ctx->num_cus * (gmul ? std::max(1, std::atoi(gmul)) : 4));
     v.grid = (int)((std::min(v.n_row_blocks, cap) + 7) / 8 * 8);
     A->use_vd = true;
+    if (want_sell && maxlen <= 255) {
+        rc = build_sell(ctx, A);
+        if (rc != TE_OK) return rc;
+    }
     return TE_OK;
 }
 
