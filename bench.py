@@ -151,7 +151,7 @@ def main():
         "kernel_launches_per_run": kernel_launches,
         "roofline": {"kernel": kname + " (fused SpMV + axpy + dot)", "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": measured_traffic(kname, D),
                      "bytes_per_launch": b_apply, "mean_launch_us": round(t_apply * 1e6, 2),
                      "launches_timed": apply_n},
     }
@@ -159,6 +159,21 @@ def main():
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(np, capi, H, obs, D, args, r["n_steps"])
     print(json.dumps(line), flush=True)
+
+
+def measured_traffic(kernel, D):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
+    (profiles/traffic.json: FETCH_SIZE and WRITE_SIZE scaled by a plain-copy calibration), when
+    it was measured on this workload; None otherwise."""
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(p) as f:
+            t = json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+    if not t or ("D=%d" % D) not in t.get("workload", ""):
+        return None
+    return t["fetch_bytes"] + t["write_bytes"]
 
 
 def copy_calibration():

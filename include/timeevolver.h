@@ -175,6 +175,33 @@ int te_evolver_step_trace(te_evolver* ev, double* t_steps, size_t cap, size_t* n
 int te_evolver_set_profiling(te_evolver* ev, int enable);
 int te_evolver_kernel_time(const te_evolver* ev, int kind, double* total_ms, size_t* launches);
 
+/* ---- batched evolution (no reference counterpart: the reference runs one krylovTimeEvolver
+ * per instance, core/krylovTimeEvolver.cpp:26-84 and :261-512) ----------------------------- */
+typedef struct te_batch te_batch;
+/* n_inst independent time evolutions advanced together on the device, every one exactly the
+ * plain method of te_evolver_create/te_evolver_run (same Lanczos recurrence, step control,
+ * sampling and status rules) with its own Hamiltonian H[i] (square, rows of at most 257
+ * entries), initial state psi0[i] (H[i] dimension complex128, host memory, copied) and n_obs
+ * observables obs[i * n_obs + o] (diagonal TE_OBS_SPARSE matrices only; n_obs <= 4). One
+ * parameter set for all. Validation per instance as the constructor: TE_ERR_FATAL names the
+ * instance whose state is not normalised (:45-48). Matrices are borrowed (must outlive the
+ * batch). */
+int te_batch_create(te_ctx* ctx, int n_inst, te_matrix* const* H, const double* const* psi0,
+                    const te_evolve_params* params, const te_observable* obs, int n_obs, te_batch** out);
+void te_batch_destroy(te_batch* b);
+/* Runs every instance from its psi0 to t. obs_values: n_inst x n_obs x n_samples doubles (may be
+ * NULL); states: n_inst pointers (the array or any entry may be NULL) receiving each instance's
+ * last sampled state; results: n_inst entries. An instance that hits a fatal condition of the
+ * reference (no small enough step, :203-207) stops alone with status_code -1 and the call
+ * returns TE_ERR_FATAL naming the first such instance; the others' results are complete. */
+int te_batch_run(te_batch* b, double* obs_values, double* const* states, te_result* results);
+/* Rows of the packed vectors, samples per observable, and rounds of the last run (one round is
+ * one Krylov step of every live instance). */
+int te_batch_info(const te_batch* b, size_t* packed_rows, size_t* n_samples, long long* rounds);
+/* 1 (default): each round is replayed from a captured HIP graph; 0: launched kernel by kernel.
+ * The results are identical. */
+int te_batch_set_graph(te_batch* b, int enable);
+
 /* Log level filter (krylovLogger::loggingLevel, core/krylovLogger.h:17: DEBUG, INFO, WARNING,
  * ERROR, FATAL = 0..4; default 0 prints everything, as the unfiltered Boost.Log core does).
  * Messages go to stderr. */

@@ -1,0 +1,133 @@
+"""GPU parity of the batched driver (te_batch_*): many uneven ring Bose-Hubbard instances in
+one batch, each checked against the oracle's restatement of the reference evolver and against
+the plain single-instance GPU path.
+
+Bars (as for the plain path): ||psi_batch - psi_oracle|| <= err_batch + err_oracle + 1e-10,
+observables within 2 ||O|| (err_batch + err_oracle) + 1e-10, identical sample counts and status
+codes; graph replay and kernel-by-kernel launches bitwise identical; repeated runs bitwise
+identical.
+"""
+import numpy as np
+import pytest
+
+from timeevolver_amd import capi
+from timeevolver_amd.hamiltonian import as_tuples, createNumberOperator
+from timeevolver_amd.workloads import batch_instance, fock_state, random_state, ring_bose_hubbard_terms
+
+pytestmark = pytest.mark.gpu
+
+
+def _instances(ctx, specs, obs_modes=(0,)):
+    """specs: (L, N, eps, J, U, psi) tuples -> device matrices, observables and host terms."""
+    out = []
+    for L, N, eps, J, U, psi in specs:
+        terms = as_tuples(ring_bose_hubbard_terms(L, eps, J, U))
+        b = capi.Basis.basis(ctx, N, L)
+        H = capi.operator_build(ctx, b, terms)
+        obs = [capi.operator_build(ctx, b, as_tuples([createNumberOperator(k, 1)])) for k in obs_modes]
+        out.append((b, H, obs, terms, psi(b.D) if callable(psi) else psi))
+    return out
+
+
+def _seeded_specs(base, count, L_choices):
+    specs = []
+    for i in range(count):
+        L, N, eps, J, U, sseed = batch_instance(base, i, L_choices)
+        specs.append((L, N, eps, J, U, (lambda D, s=sseed: random_state(D, s))))
+    return specs
+
+
+def _oracle_run(oracle, N, L, terms, psi, T, step, obs_modes, tol, m):
+    ob = oracle.Basis.basis(N, L)
+    Hc = oracle.build_operator(ob, terms)
+    oobs = [("sparse", oracle.build_operator(ob, as_tuples([createNumberOperator(k, 1)]))) for k in obs_modes]
+    return oracle.evolve(ob.D, Hc, psi, T, step, oobs, tol=tol, m=m)
+
+
+def _check(rb, ro, obs_norm):
+    bound = rb["err"] + ro["err"]
+    dpsi = np.linalg.norm(rb["state"] - ro["state"])
+    dobs = np.abs(rb["observables"] - ro["observables"]).max()
+    print("D=%d steps batch/oracle %d/%d err %.3e/%.3e |dpsi| %.3e max|dobs| %.3e status %d/%d" %
+          (rb["dim"], rb["n_steps"], ro["n_steps"], rb["err"], ro["err"], dpsi, dobs, rb["statusCode"],
+           ro["statusCode"]))
+    assert dpsi <= bound + 1e-10
+    assert dobs <= 2 * obs_norm * bound + 1e-10
+    assert rb["numSamples"] == ro["numSamples"] == ro["n_samples"]
+    assert rb["statusCode"] == ro["statusCode"]
+    assert abs(rb["evolvedTime"] - ro["evolvedTime"]) <= 1e-12
+
+
+def test_batch_uneven_vs_oracle(gpu, oracle_lib):
+    """12 seeded instances, L in {3..7} (D 10 to 1716), random states, n_0 and n_1, T = 2."""
+    specs = _seeded_specs(101, 12, (3, 4, 5, 6, 7))
+    inst = _instances(gpu, specs, (0, 1))
+    T, step, tol, m = 2.0, 0.25, 1e-8, 30
+    be = capi.BatchEvolver(gpu, [x[1] for x in inst], [x[4] for x in inst], T, step, [x[2] for x in inst],
+                           tol=tol, m=m)
+    rb = be.run()
+    assert be.rounds() == max(r["n_steps"] for r in rb)
+    for (L, N, *_), (b, H, obs, terms, psi), r in zip(specs, inst, rb):
+        ro = _oracle_run(oracle_lib, N, L, terms, psi, T, step, (0, 1), tol, m)
+        _check(r, ro, N)
+
+
+def test_batch_matches_plain_gpu(gpu):
+    """The batch and the single-instance evolver take the same steps on the same instances."""
+    specs = _seeded_specs(7, 6, (6, 7, 8))
+    inst = _instances(gpu, specs)
+    T, step, tol, m = 1.0, 0.25, 1e-8, 30
+    rb = capi.BatchEvolver(gpu, [x[1] for x in inst], [x[4] for x in inst], T, step, [x[2] for x in inst],
+                           tol=tol, m=m).run()
+    for (b, H, obs, terms, psi), r in zip(inst, rb):
+        rp = capi.Evolver(gpu, H, psi, T, step, obs, tol=tol, m=m).run()
+        assert r["n_steps"] == rp["n_steps"] and r["statusCode"] == rp["statusCode"]
+        assert np.linalg.norm(r["state"] - rp["state"]) <= r["err"] + rp["err"] + 1e-10
+        assert np.abs(r["observables"] - rp["observables"]).max() <= 2 * 8 * (r["err"] + rp["err"]) + 1e-10   # n_0 <= N <= 8
+
+
+def test_batch_graph_and_repeat_bitwise(gpu):
+    specs = _seeded_specs(55, 5, (4, 5, 6))
+    inst = _instances(gpu, specs)
+    be = capi.BatchEvolver(gpu, [x[1] for x in inst], [x[4] for x in inst], 1.5, 0.25, [x[2] for x in inst],
+                           tol=1e-8, m=20)
+    r1 = be.run()
+    r2 = be.run()
+    be.set_graph(False)
+    r3 = be.run()
+    for a, b, c in zip(r1, r2, r3):
+        for x in (b, c):
+            assert np.array_equal(a["state"].view(np.uint64), x["state"].view(np.uint64))
+            assert np.array_equal(a["observables"].view(np.uint64), x["observables"].view(np.uint64))
+            assert a["n_steps"] == x["n_steps"] and a["err"] == x["err"]
+
+
+def test_batch_edge_cases(gpu, oracle_lib):
+    """A dimension below m (m0 = D, lucky breakdown), a Fock state, a dimension-1 instance, and
+    an instance identical to another (independence of the per-instance reductions)."""
+    eps2, J2 = [0.3, 0.7], [0.5, -0.2]
+    specs = [(2, 2, eps2, J2, 0.5, fock_state),                 # D = 3 < m
+             (4, 4, [0.1, 0.2, 0.3, 0.4], [0.5, -0.5, 0.25, 1.0], 0.6, fock_state),
+             (3, 0, [0.2, 0.4, 0.6], [0.3, 0.3, 0.3], 0.5, fock_state),   # N = 0: D = 1
+             (4, 4, [0.1, 0.2, 0.3, 0.4], [0.5, -0.5, 0.25, 1.0], 0.6, fock_state)]
+    inst = _instances(gpu, specs)
+    T, step, tol, m = 3.0, 0.5, 1e-8, 10
+    rb = capi.BatchEvolver(gpu, [x[1] for x in inst], [x[4] for x in inst], T, step, [x[2] for x in inst],
+                           tol=tol, m=m).run()
+    for (L, N, *_), (b, H, obs, terms, psi), r in zip(specs, inst, rb):
+        ro = _oracle_run(oracle_lib, N, L, terms, psi, T, step, (0,), tol, m)
+        _check(r, ro, max(N, 1))
+    assert np.array_equal(rb[1]["state"], rb[3]["state"])
+
+
+def test_batch_errors(gpu):
+    specs = _seeded_specs(3, 2, (3, 4))
+    inst = _instances(gpu, specs)
+    bad = inst[1][4] * 1.01
+    with pytest.raises(capi.TEError) as e:
+        capi.BatchEvolver(gpu, [x[1] for x in inst], [inst[0][4], bad], 1.0, 0.25, [x[2] for x in inst], tol=1e-8)
+    assert e.value.code == capi.TE_ERR_FATAL and "instance 1" in str(e.value)
+    with pytest.raises(capi.TEError) as e:      # a non-diagonal observable
+        capi.BatchEvolver(gpu, [x[1] for x in inst], [x[4] for x in inst], 1.0, 0.25, [[x[1]] for x in inst],
+                          tol=1e-8)
+    assert e.value.code == capi.TE_ERR_UNSUPPORTED

@@ -23,6 +23,7 @@ SOURCES = {
     "te_spmv.hip": [],
     "te_krylov.hip": ["-ffp-contract=off"],
     "te_builder.hip": ["-ffp-contract=off"],
+    "te_batch.hip": [],
 }
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang",

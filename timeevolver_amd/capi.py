@@ -82,6 +82,12 @@ _SIGS = {
     "te_evolver_step_trace": (_c.c_int, [_p, _dp, _c.c_size_t, _szp]),
     "te_evolver_set_profiling": (_c.c_int, [_p, _c.c_int]),
     "te_evolver_kernel_time": (_c.c_int, [_p, _c.c_int, _dp, _szp]),
+    "te_batch_create": (_c.c_int, [_p, _c.c_int, _pp, _c.POINTER(_dp), _c.POINTER(te_evolve_params),
+                                   _c.POINTER(te_observable), _c.c_int, _pp]),
+    "te_batch_destroy": (None, [_p]),
+    "te_batch_run": (_c.c_int, [_p, _dp, _c.POINTER(_dp), _c.POINTER(te_result)]),
+    "te_batch_info": (_c.c_int, [_p, _szp, _szp, _c.POINTER(_c.c_longlong)]),
+    "te_batch_set_graph": (_c.c_int, [_p, _c.c_int]),
 }
 
 _lib = None
@@ -375,6 +381,79 @@ class Evolver:
     def close(self):
         if self.h:
             self.ctx.L.te_evolver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BatchEvolver:
+    """Many independent krylovTimeEvolver runs advanced together (te_batch_*). Hs: list of
+    Matrix; psi0s: list of complex vectors; observables: per instance a list of diagonal Matrix
+    observables (same count for every instance)."""
+
+    def __init__(self, ctx, Hs, psi0s, t, sampling_step, observables=None, exp_factor=1.0, tol=1e-6, m=40,
+                 fast_integration=False):
+        self.ctx = ctx
+        n = len(Hs)
+        if observables is None:
+            observables = [[] for _ in range(n)]
+        n_obs = len(observables[0]) if n else 0
+        assert all(len(o) == n_obs for o in observables)
+        self._keep = [list(Hs), [o for ol in observables for o in ol]]
+        self.psis = [np.ascontiguousarray(p, dtype=np.complex128) for p in psi0s]
+        Harr = (_p * n)(*[H.h.value if isinstance(H.h, _p) else H.h for H in Hs])
+        Parr = (_dp * n)(*[_ptr(p.view(np.float64), _c.c_double) for p in self.psis])
+        obs = (te_observable * max(n * n_obs, 1))()
+        for i in range(n):
+            for o in range(n_obs):
+                obs[i * n_obs + o] = te_observable(TE_OBS_SPARSE, observables[i][o].h, None, OBS_CALLBACK(0), None)
+        params = te_evolve_params(t, sampling_step, exp_factor, tol, m, int(fast_integration))
+        h = _p()
+        ctx.check(ctx.L.te_batch_create(ctx.h, n, Harr, Parr, _c.byref(params), obs, n_obs, _c.byref(h)),
+                  "te_batch_create")
+        self.h = h
+        self.n = n
+        self.n_obs = n_obs
+        self.dims = [H.n_rows for H in Hs]
+        rows, ns = _c.c_size_t(), _c.c_size_t()
+        ctx.L.te_batch_info(h, _c.byref(rows), _c.byref(ns), None)
+        self.packed_rows = rows.value
+        self.n_samples = ns.value
+
+    def set_graph(self, on):
+        self.ctx.check(self.ctx.L.te_batch_set_graph(self.h, int(on)), "te_batch_set_graph")
+
+    def rounds(self):
+        r = _c.c_longlong()
+        self.ctx.L.te_batch_info(self.h, None, None, _c.byref(r))
+        return r.value
+
+    def run(self, want_states=True, want_obs=True, check=True):
+        """Returns a list of per-instance dicts (the keys of Evolver.run) and the return code."""
+        R = (te_result * self.n)()
+        obs = np.zeros(max(self.n * self.n_obs * self.n_samples, 1))
+        states = [np.zeros(d, dtype=np.complex128) for d in self.dims] if want_states else None
+        Sarr = (_dp * self.n)(*[_ptr(s.view(np.float64), _c.c_double) for s in states]) if want_states else None
+        rc = self.ctx.L.te_batch_run(self.h, _ptr(obs, _c.c_double) if want_obs else None, Sarr, R)
+        if check:
+            self.ctx.check(rc, "te_batch_run")
+        O = obs[:self.n * self.n_obs * self.n_samples].reshape(self.n, self.n_obs, self.n_samples)
+        out = []
+        for i in range(self.n):
+            r = R[i]
+            out.append({"state": states[i] if want_states else None, "observables": O[i], "err": r.err,
+                        "evolvedTime": r.evolved_time, "numSamples": r.num_samples, "n_steps": r.n_steps,
+                        "dim": r.dim, "krylovDim": r.krylov_dim, "statusCode": r.status_code,
+                        "n_samples": self.n_samples})
+        return (out, rc) if not check else out
+
+    def close(self):
+        if self.h:
+            self.ctx.L.te_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -1,0 +1,794 @@
+// Batched Krylov time evolution: many independent instances of the plain method (each with its
+// own Hamiltonian, dimension, initial state and diagonal observables, all with one parameter
+// set) advanced together. It has no counterpart in the reference, whose cost for such a batch
+// is one krylovTimeEvolver::timeEvolve (core/krylovTimeEvolver.cpp:261-512) per instance.
+//
+// Every instance keeps the reference's algorithm unchanged: the same Lanczos recurrence, step
+// controller, sample schedule and status rules as the single-instance evolver. What changes is
+// how the work reaches the device:
+//  * the Krylov bases of all instances are packed into one array of (m_max + 1) columns, an
+//    instance owning rows [off_i, off_i + D_i) of every column;
+//  * one launch per kernel kind covers every instance: the Lanczos apply walks SpMV units (up
+//    to kBatchRB row blocks of one instance), the project/combine passes walk vector units (up
+//    to vrows rows of one instance); each unit first reads its instance's flags and skips the
+//    instance when it is finished, broken down, or past its Krylov dimension;
+//  * reductions are per instance and deterministic: a unit writes its block partial, the last
+//    unit of the instance to arrive (per-instance ticket) sums the instance's partials in unit
+//    order and resets the ticket;
+//  * the step controller runs one block per instance (launch_batch_step_control);
+//  * one round (m_max Lanczos iterations, controller, combine, observable sums, live count) is
+//    captured once as a HIP graph and replayed until no instance is live. The graph holds
+//    kernel nodes only (tickets are reset by the kernels themselves).
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "te_internal.h"
+#include "te_krylov.h"
+#include "te_reduce.h"
+
+namespace te {
+
+constexpr int kBatchRB = 4;          // row blocks (kStreamChunk entries + rows each) per SpMV unit
+constexpr int kBatchVecRows = 4096;  // rows per vector unit (grown when the observable partials would be large)
+constexpr int kBatchSC = 8;          // samples per pass of the combine kernel
+constexpr int kBatchMaxObs = 4;      // diagonal observables per instance
+constexpr int kBatchTraceCap = 4096; // recorded step sizes per instance
+
+struct BatchInst {
+    int64_t off;            // first row of the instance in the packed vectors
+    int n;                  // dimension
+    int nsu, su0;           // SpMV units [su0, su0 + nsu)
+    int nvu, vu0;           // vector units [vu0, vu0 + nvu)
+    const int* ptr;         // the instance's CSR (rows and columns local to the instance)
+    const int* col;
+    const double2* val;
+    const int* rb;          // its row blocks (kStreamChunk merge-path chunks)
+};
+
+struct BatchDev {
+    const BatchInst* inst;
+    const int4* sunit;      // (instance, first row block, end row block, 0)
+    int n_sunits;
+    const int4* vunit;      // (instance, first row, end row, 0), rows local to the instance
+    int n_vunits;
+    double2* U;             // packed Krylov bases, column-major, ld rows per column
+    int64_t ld;
+    double2 alpha;          // -i expFactor
+    double tolRate;
+    KrylovScalars* ks;      // per instance
+    StepState* st;
+    BatchCtl* bc;
+    double2* spart;         // per SpMV unit
+    unsigned int* sticket;  // per instance
+    double2* vpart;         // per vector unit
+    unsigned int* vticket;  // per instance
+    const double2* wcoef;   // per instance, kMaxKrylov
+    const double2* Y;       // per instance, n_samples x kMaxKrylov
+    long long n_samples;
+    const double* diag;     // nobs x ld: Re of the diagonal observables, packed like the vectors
+    int nobs;
+    double* opart;          // n_vunits x nobs x n_samples
+    double* obs;            // n_inst x nobs x n_samples
+    double2* sampled;       // packed last sampled states
+    double* nrm;            // per instance |sampled|^2 (final pass)
+    int n_inst;
+    int* n_live;
+};
+
+// The last block of an instance's units to arrive (ticket) returns true in all its threads.
+__device__ __forceinline__ bool last_unit_arrive(unsigned int* ticket, unsigned int count) {
+    __shared__ bool last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned int t = atomicAdd(ticket, 1u);
+        last = (t == count - 1);
+    }
+    __syncthreads();
+    if (last) __threadfence();
+    return last;
+}
+
+// Whether instance i takes part in Lanczos iteration j of this round (uniform in a block).
+__device__ __forceinline__ bool lanczos_active(const BatchDev& a, int i, int j) {
+    return a.bc[i].live && !a.ks[i].broken && j < a.st[i].m;
+}
+
+struct BatchApplyEpi {
+    double2 c1;
+    double c2;
+    const double2* up;
+    const double2* uj;
+    double2* w;
+    double2 dot;
+    __device__ __forceinline__ void row(int r, double2 s) {
+        double2 wr = cmul(c1, s);
+        if (up) {
+            const double2 p = load_stream(&up[r]);
+            wr.x += c2 * p.x;
+            wr.y += c2 * p.y;
+        }
+        store_stream(&w[r], wr);
+        const double2 u = uj[r];
+        dot.x += u.x * wr.x + u.y * wr.y;
+        dot.y += u.x * wr.y - u.y * wr.x;
+    }
+};
+
+// Lanczos apply for iteration j of every active instance: w = (alpha s_j) A u_j +
+// (beta_{j-1} s_{j-1}) u_{j-1}, a_j = s_j <u_j, w> (the single-instance apply kernel's
+// arithmetic, per row block).
+__global__ __launch_bounds__(kBlock) void batch_apply_kernel(BatchDev a, int j) {
+    __shared__ double2 prod[kStreamCap];
+    __shared__ double2 sh[kBlock / 64];
+    int u0, u1, us;
+    xcd_sweep(a.n_sunits, u0, u1, us);
+    for (int u = u0; u < u1; u += us) {
+        const int4 su = a.sunit[u];
+        const int i = su.x;
+        if (!lanczos_active(a, i, j)) continue;
+        const BatchInst I = a.inst[i];
+        KrylovScalars* ks = a.ks + i;
+        const double sj = ks->s[j];
+        BatchApplyEpi epi;
+        epi.c1 = make_double2(a.alpha.x * sj, a.alpha.y * sj);
+        epi.c2 = j > 0 ? ks->beta[j - 1] * ks->s[j - 1] : 0.0;
+        epi.up = j > 0 ? a.U + (size_t)(j - 1) * a.ld + I.off : nullptr;
+        epi.uj = a.U + (size_t)j * a.ld + I.off;
+        epi.w = a.U + (size_t)(j + 1) * a.ld + I.off;
+        epi.dot = make_double2(0.0, 0.0);
+        for (int b = su.y; b < su.z; b++)
+            stream_row_block(I.rb[b], I.rb[b + 1], I.ptr, I.col, I.val, epi.uj, prod, epi, nullptr);
+        const double2 bs = block_sum2<kBlock>(epi.dot, sh);
+        if (threadIdx.x == 0) a.spart[u] = bs;
+        if (last_unit_arrive(a.sticket + i, (unsigned)I.nsu)) {
+            const double2 tot = reduce_partials<kBlock>(a.spart + I.su0, I.nsu, 1, sh);
+            if (threadIdx.x == 0) {
+                ks->a[j] = make_double2(sj * tot.x, sj * tot.y);
+                a.sticket[i] = 0;
+            }
+        }
+    }
+}
+
+// Lanczos project for iteration j: w' = w - (a_j s_j) u_j, beta_j = |w'|, stored in place
+// unless j + 1 is the instance's Krylov dimension; breakdown when beta_j < tolRate
+// (core/krylovTimeEvolver.cpp:551-567).
+__global__ __launch_bounds__(kBlock) void batch_project_kernel(BatchDev a, int j) {
+    __shared__ double2 sh[kBlock / 64];
+    int u0, u1, us;
+    xcd_sweep(a.n_vunits, u0, u1, us);
+    for (int u = u0; u < u1; u += us) {
+        const int4 vu = a.vunit[u];
+        const int i = vu.x;
+        if (!lanczos_active(a, i, j)) continue;
+        const BatchInst I = a.inst[i];
+        KrylovScalars* ks = a.ks + i;
+        const int m = a.st[i].m;
+        const double sj = ks->s[j];
+        const double2 aj = ks->a[j];
+        const double2 c3 = make_double2(aj.x * sj, aj.y * sj);
+        const double2* __restrict__ uj = a.U + (size_t)j * a.ld + I.off;
+        double2* __restrict__ w = a.U + (size_t)(j + 1) * a.ld + I.off;
+        const bool store = j + 1 < m;
+        double nrm = 0.0;
+        int row = vu.y + threadIdx.x;
+        for (; row + 3 * kBlock < vu.z; row += 4 * kBlock) {
+            double2 uu[4], wr[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                uu[q] = uj[row + q * kBlock];
+                wr[q] = w[row + q * kBlock];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                wr[q].x -= c3.x * uu[q].x - c3.y * uu[q].y;
+                wr[q].y -= c3.x * uu[q].y + c3.y * uu[q].x;
+                if (store) w[row + q * kBlock] = wr[q];
+                nrm += wr[q].x * wr[q].x + wr[q].y * wr[q].y;
+            }
+        }
+        for (; row < vu.z; row += kBlock) {
+            const double2 uu = uj[row];
+            double2 wr = w[row];
+            wr.x -= c3.x * uu.x - c3.y * uu.y;
+            wr.y -= c3.x * uu.y + c3.y * uu.x;
+            if (store) w[row] = wr;
+            nrm += wr.x * wr.x + wr.y * wr.y;
+        }
+        const double2 bs = block_sum2<kBlock>(make_double2(nrm, 0.0), sh);
+        if (threadIdx.x == 0) a.vpart[u] = bs;
+        if (last_unit_arrive(a.vticket + i, (unsigned)I.nvu)) {
+            const double2 tot = reduce_partials<kBlock>(a.vpart + I.vu0, I.nvu, 1, sh);
+            if (threadIdx.x == 0) {
+                const double beta = sqrt(tot.x);
+                if (beta < a.tolRate) {
+                    ks->broken = 1;
+                    ks->m_hbd = j + 1;
+                    ks->h = beta;
+                } else if (j + 1 == m) {
+                    ks->h = beta;
+                } else {
+                    ks->beta[j] = beta;
+                    ks->s[j + 1] = 1.0 / beta;
+                }
+                a.vticket[i] = 0;
+            }
+        }
+    }
+}
+
+// After the controller: for every instance that stepped, currentVec = U w written over column
+// 0 when the step asks for it (:382), the step's samples U y_s (:409-421), the last of them
+// kept as the sampled state, and per-unit partial sums of the diagonal observables
+// sum_r Re(O_rr) |psi_s(r)|^2. Samples go kBatchSC per pass over U; the pass that covers the
+// last sample is the one that overwrites column 0, after every pass has read it.
+template <int NOB>
+__global__ __launch_bounds__(kBlock) void batch_combine_kernel(BatchDev a) {
+    __shared__ double2 cf[(1 + kBatchSC) * kMaxKrylov];
+    __shared__ double2 sh[kBlock / 64];
+    int u0, u1, us;
+    xcd_sweep(a.n_vunits, u0, u1, us);
+    for (int u = u0; u < u1; u += us) {
+        const int4 vu = a.vunit[u];
+        const int i = vu.x;
+        if (!a.bc[i].stepped) continue;
+        const int m = a.bc[i].cm;
+        const StepState* st = a.st + i;
+        const int S = st->S;
+        const long long y0 = st->sample_base;
+        const int write_w = st->write_w;
+        const int64_t off = a.inst[i].off;
+        const double2* Yi = a.Y + (size_t)i * a.n_samples * kMaxKrylov;
+        const double2* wc = a.wcoef + (size_t)i * kMaxKrylov;
+        for (int c0 = 0; c0 < S || (c0 == 0 && write_w); c0 += kBatchSC) {
+            const int sc = min(kBatchSC, S - c0);      // samples of this pass (may be <= 0)
+            const bool lastpass = c0 + kBatchSC >= S;
+            const bool ww = lastpass && write_w;
+            for (int q = threadIdx.x; q < (1 + kBatchSC) * kMaxKrylov; q += kBlock) {
+                const int c = q / kMaxKrylov, jj = q % kMaxKrylov;
+                double2 v = make_double2(0.0, 0.0);
+                if (jj < m) {
+                    if (c == 0) {
+                        if (ww) v = wc[jj];
+                    } else if (c - 1 < sc) {
+                        v = Yi[(size_t)(y0 + c0 + c - 1) * kMaxKrylov + jj];
+                    }
+                }
+                cf[q] = v;
+            }
+            __syncthreads();
+            double acc[NOB > 0 ? NOB * kBatchSC : 1];
+#pragma unroll
+            for (int q = 0; q < (NOB > 0 ? NOB * kBatchSC : 1); q++) acc[q] = 0.0;
+            for (int row = vu.y + threadIdx.x; row < vu.z; row += kBlock) {
+                const size_t gr = (size_t)off + row;
+                double2 aw = make_double2(0.0, 0.0);
+                double2 as[kBatchSC];
+#pragma unroll
+                for (int s = 0; s < kBatchSC; s++) as[s] = make_double2(0.0, 0.0);
+                for (int jj = 0; jj < m; jj++) {
+                    const double2 uv = a.U[(size_t)jj * a.ld + gr];
+                    if (ww) {
+                        const double2 t = cmul(uv, cf[jj]);
+                        aw.x += t.x;
+                        aw.y += t.y;
+                    }
+#pragma unroll
+                    for (int s = 0; s < kBatchSC; s++) {
+                        if (s < sc) {
+                            const double2 t = cmul(uv, cf[(1 + s) * kMaxKrylov + jj]);
+                            as[s].x += t.x;
+                            as[s].y += t.y;
+                        }
+                    }
+                }
+                if (ww) a.U[gr] = aw;
+                if (lastpass && sc > 0) {
+                    double2 v = as[0];
+#pragma unroll
+                    for (int s = 1; s < kBatchSC; s++)
+                        if (s == sc - 1) v = as[s];
+                    a.sampled[gr] = v;
+                }
+#pragma unroll
+                for (int s = 0; s < kBatchSC; s++) {
+                    const double pr = as[s].x * as[s].x + as[s].y * as[s].y;
+#pragma unroll
+                    for (int o = 0; o < NOB; o++)
+                        if (o < a.nobs && s < sc) acc[o * kBatchSC + s] += a.diag[(size_t)o * a.ld + gr] * pr;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < NOB * kBatchSC; q++) {
+                const int o = q / kBatchSC, s = q % kBatchSC;
+                if (o < a.nobs && s < sc) {          // uniform in the block
+                    const double2 t = block_sum2<kBlock>(make_double2(acc[q], 0.0), sh);
+                    if (threadIdx.x == 0)
+                        a.opart[((size_t)u * a.nobs + o) * a.n_samples + y0 + c0 + s] = t.x;
+                }
+            }
+            __syncthreads();                     // cf is reloaded by the next pass
+        }
+    }
+}
+
+// One block per instance that stepped: its observable values for the step's samples, summed
+// over its vector units in unit order.
+__global__ __launch_bounds__(kBlock) void batch_obs_finish_kernel(BatchDev a) {
+    const int i = blockIdx.x;
+    if (!a.bc[i].stepped || a.nobs == 0) return;
+    const int S = a.st[i].S;
+    const long long y0 = a.st[i].sample_base;
+    const BatchInst I = a.inst[i];
+    for (int p = threadIdx.x; p < a.nobs * S; p += kBlock) {
+        const int o = p / S, s = p % S;
+        double t = 0.0;
+        for (int q = 0; q < I.nvu; q++) t += a.opart[((size_t)(I.vu0 + q) * a.nobs + o) * a.n_samples + y0 + s];
+        a.obs[((size_t)i * a.nobs + o) * a.n_samples + y0 + s] = t;
+    }
+}
+
+// Number of live instances, read by the host after each round.
+__global__ __launch_bounds__(kBlock) void batch_count_kernel(BatchDev a) {
+    __shared__ double2 sh[kBlock / 64];
+    double c = 0.0;
+    for (int i = threadIdx.x; i < a.n_inst; i += kBlock) c += a.bc[i].live ? 1.0 : 0.0;
+    const double2 t = block_sum2<kBlock>(make_double2(c, 0.0), sh);
+    if (threadIdx.x == 0) *a.n_live = (int)t.x;
+}
+
+// |sampled_i|^2 per instance (one block each, rows in order) for the final norm check.
+__global__ __launch_bounds__(kBlock) void batch_norm_kernel(BatchDev a) {
+    __shared__ double2 sh[kBlock / 64];
+    const int i = blockIdx.x;
+    const BatchInst I = a.inst[i];
+    double s = 0.0;
+    for (int r = threadIdx.x; r < I.n; r += kBlock) {
+        const double2 v = a.sampled[(size_t)I.off + r];
+        s += v.x * v.x + v.y * v.y;
+    }
+    const double2 t = block_sum2<kBlock>(make_double2(s, 0.0), sh);
+    if (threadIdx.x == 0) a.nrm[i] = t.x;
+}
+
+}  // namespace te
+
+using namespace te;
+
+struct te_batch {
+    te_ctx* ctx = nullptr;
+    te_evolve_params P{};
+    int n_inst = 0, nobs = 0, m_max = 0;
+    long long n_samples = 0;
+    int64_t ld = 0;
+    double tolRate = 0;
+    std::vector<int64_t> off;
+    std::vector<int> dim, m0;
+    std::vector<double> numErrEst;
+    BatchDev dev{};
+    int grid_s = 8, grid_v = 8;
+    // device buffers
+    BatchInst* inst = nullptr;
+    int4 *sunit = nullptr, *vunit = nullptr;
+    double2 *U = nullptr, *psi0 = nullptr, *sampled = nullptr, *wcoef = nullptr, *Y = nullptr;
+    double2 *spart = nullptr, *vpart = nullptr;
+    unsigned int *sticket = nullptr, *vticket = nullptr;
+    KrylovScalars* ks = nullptr;
+    StepState* st = nullptr;
+    BatchCtl* bc = nullptr;
+    StepConsts* consts = nullptr;
+    double *diag = nullptr, *opart = nullptr, *obs = nullptr, *nrm = nullptr, *trace = nullptr;
+    int* n_live = nullptr;
+    int* n_live_host = nullptr;     // pinned
+    DevQuad quad;
+    // one round as a graph
+    bool use_graph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    long long rounds = 0;
+    double round_ms = 0;
+};
+
+namespace te {
+
+static void batch_free(te_batch* b) {
+    if (!b) return;
+    hipSetDevice(b->ctx->device);
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    void* ptrs[] = {b->inst, b->sunit,   b->vunit,  b->U,  b->psi0, b->sampled, b->wcoef, b->Y,
+                    b->spart, b->vpart,  b->sticket, b->vticket, b->ks, b->st, b->bc, b->consts,
+                    b->diag, b->opart,   b->obs,    b->nrm, b->trace, b->n_live};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    if (b->n_live_host) hipHostFree(b->n_live_host);
+    free_quadrature(b->quad);
+    delete b;
+}
+
+template <class T>
+static int balloc(te_ctx* ctx, T** p, size_t count) {
+    TE_HIP(ctx, hipMalloc((void**)p, sizeof(T) * std::max<size_t>(count, 1)));
+    return TE_OK;
+}
+
+// Enqueues one round on the context's stream.
+static void batch_round(te_batch* b, hipStream_t s) {
+    const BatchDev& d = b->dev;
+    for (int j = 0; j < b->m_max; j++) {
+        hipLaunchKernelGGL(batch_apply_kernel, dim3(b->grid_s), dim3(kBlock), 0, s, d, j);
+        hipLaunchKernelGGL(batch_project_kernel, dim3(b->grid_v), dim3(kBlock), 0, s, d, j);
+    }
+    launch_batch_step_control(b->consts, b->st, b->ks, b->bc, b->n_inst, b->n_samples, s);
+}
+
+static void batch_combine(te_batch* b, hipStream_t s) {
+    const BatchDev& d = b->dev;
+    if (b->nobs == 0) hipLaunchKernelGGL(batch_combine_kernel<0>, dim3(b->grid_v), dim3(kBlock), 0, s, d);
+    else if (b->nobs == 1) hipLaunchKernelGGL(batch_combine_kernel<1>, dim3(b->grid_v), dim3(kBlock), 0, s, d);
+    else if (b->nobs == 2) hipLaunchKernelGGL(batch_combine_kernel<2>, dim3(b->grid_v), dim3(kBlock), 0, s, d);
+    else hipLaunchKernelGGL(batch_combine_kernel<4>, dim3(b->grid_v), dim3(kBlock), 0, s, d);
+    hipLaunchKernelGGL(batch_obs_finish_kernel, dim3(b->n_inst), dim3(kBlock), 0, s, d);
+    hipLaunchKernelGGL(batch_count_kernel, dim3(1), dim3(kBlock), 0, s, d);
+}
+
+}  // namespace te
+
+#undef TE_TRY
+#define TE_TRY(expr)             \
+    do {                         \
+        int rc_ = (expr);        \
+        if (rc_ != TE_OK) {      \
+            te::batch_free(b);   \
+            return rc_;          \
+        }                        \
+    } while (0)
+#define TE_TRYH(call) TE_TRY((call) == hipSuccess ? TE_OK : te::fail(ctx, TE_ERR_HIP, "%s failed", #call))
+
+extern "C" int te_batch_create(te_ctx* ctx, int n_inst, te_matrix* const* H, const double* const* psi0,
+                               const te_evolve_params* params, const te_observable* obs, int n_obs,
+                               te_batch** out) {
+    using namespace te;
+    if (!ctx || n_inst <= 0 || !H || !psi0 || !params || !out || n_obs < 0 || (n_obs > 0 && !obs))
+        return TE_ERR_ARG;
+    *out = nullptr;
+    if (n_obs > kBatchMaxObs)
+        return fail(ctx, TE_ERR_UNSUPPORTED, "batched evolution supports at most %d observables per instance",
+                    kBatchMaxObs);
+    if (!(params->t > 0) || !(params->sampling_step > 0)) return fail(ctx, TE_ERR_ARG, "t and samplingStep must be > 0");
+    if (params->m < 1) return fail(ctx, TE_ERR_ARG, "Krylov dimension must be >= 1");
+    hipSetDevice(ctx->device);
+    te_batch* b = new te_batch();
+    b->ctx = ctx;
+    b->P = *params;
+    b->n_inst = n_inst;
+    b->nobs = n_obs;
+    b->n_samples = (long long)std::floor(params->t / params->sampling_step) + 1;       // :57
+    b->tolRate = params->tol / params->t;                                              // :287
+    b->off.resize(n_inst);
+    b->dim.resize(n_inst);
+    b->m0.resize(n_inst);
+    b->numErrEst.resize(n_inst);
+
+    // instances: dimensions, Krylov dimensions, unit tables
+    std::vector<BatchInst> hinst(n_inst);
+    std::vector<int4> hsu, hvu;
+    int64_t total = 0;
+    for (int i = 0; i < n_inst; i++) {
+        te_matrix* A = H[i];
+        if (!A || !psi0[i]) {
+            batch_free(b);
+            return fail(ctx, TE_ERR_ARG, "instance %d: null matrix or state", i);
+        }
+        if (A->ctx != ctx) {
+            batch_free(b);
+            return fail(ctx, TE_ERR_ARG, "instance %d: matrix belongs to another context", i);
+        }
+        if (A->n_rows == 0) {
+            batch_free(b);
+            return fail(ctx, TE_ERR_FATAL, "instance %d: Invalid Hilbertspace dimension", i);     // :39-43
+        }
+        if (A->n_cols != A->n_rows) {
+            batch_free(b);
+            return fail(ctx, TE_ERR_ARG, "instance %d: Hamiltonian must be square", i);
+        }
+        if (A->max_row_len + kStreamChunk - 1 > kStreamCap || A->n_row_blocks <= 0) {
+            batch_free(b);
+            return fail(ctx, TE_ERR_UNSUPPORTED, "instance %d: rows longer than %d entries", i,
+                        kStreamCap - kStreamChunk + 1);
+        }
+        for (int o = 0; o < n_obs; o++) {
+            const te_observable& ob = obs[(size_t)i * n_obs + o];
+            if (ob.kind != TE_OBS_SPARSE || !ob.matrix || !ob.matrix->diagonal) {
+                batch_free(b);
+                return fail(ctx, TE_ERR_UNSUPPORTED, "instance %d: batched evolution takes diagonal sparse "
+                                                     "observables only", i);
+            }
+            if (ob.matrix->n_rows != A->n_rows || ob.matrix->n_cols != A->n_cols) {
+                batch_free(b);
+                return fail(ctx, TE_ERR_FATAL, "instance %d: Incompatible dimensions", i);
+            }
+        }
+        const int n = (int)A->n_rows;
+        b->dim[i] = n;
+        b->m0[i] = std::min(params->m, n);                                                   // :35
+        if (b->m0[i] > kMaxKrylov) {
+            batch_free(b);
+            return fail(ctx, TE_ERR_UNSUPPORTED, "Krylov dimension %d exceeds %d", b->m0[i], kMaxKrylov);
+        }
+        b->m_max = std::max(b->m_max, b->m0[i]);
+        double nrm1 = 0;
+        TE_TRY(te_matrix_norm1(A, &nrm1));                                                   // :33
+        b->numErrEst[i] = (double)n * nrm1 * DBL_EPSILON;                                    // :289
+        b->off[i] = total;
+        // 64-row alignment keeps every instance's columns on whole cache lines
+        total += ((int64_t)n + 63) / 64 * 64;
+        BatchInst& I = hinst[i];
+        I.off = b->off[i];
+        I.n = n;
+        I.ptr = A->row_ptr;
+        I.col = A->col;
+        I.val = A->val;
+        I.rb = A->row_block;
+        I.su0 = (int)hsu.size();
+        for (int r = 0; r < A->n_row_blocks; r += kBatchRB)
+            hsu.push_back(make_int4(i, r, std::min(r + kBatchRB, A->n_row_blocks), 0));
+        I.nsu = (int)hsu.size() - I.su0;
+    }
+    b->ld = total;
+    // vector units: kBatchVecRows rows, grown so the observable partials stay under 256 MiB
+    int vrows = kBatchVecRows;
+    while (true) {
+        int64_t nvu = 0;
+        for (int i = 0; i < n_inst; i++) nvu += (b->dim[i] + vrows - 1) / vrows;
+        if ((double)nvu * n_obs * (double)b->n_samples * 8.0 <= 256.0 * 1048576.0 || vrows >= (1 << 28)) break;
+        vrows *= 2;
+    }
+    for (int i = 0; i < n_inst; i++) {
+        hinst[i].vu0 = (int)hvu.size();
+        for (int r = 0; r < b->dim[i]; r += vrows) hvu.push_back(make_int4(i, r, std::min(r + vrows, b->dim[i]), 0));
+        hinst[i].nvu = (int)hvu.size() - hinst[i].vu0;
+    }
+
+    const size_t ns = (size_t)b->n_samples;
+    TE_TRY(balloc(ctx, &b->inst, n_inst));
+    TE_TRY(balloc(ctx, &b->sunit, hsu.size()));
+    TE_TRY(balloc(ctx, &b->vunit, hvu.size()));
+    TE_TRY(balloc(ctx, &b->U, (size_t)total * (b->m_max + 1)));
+    TE_TRY(balloc(ctx, &b->psi0, (size_t)total));
+    TE_TRY(balloc(ctx, &b->sampled, (size_t)total));
+    TE_TRY(balloc(ctx, &b->wcoef, (size_t)n_inst * kMaxKrylov));
+    TE_TRY(balloc(ctx, &b->Y, (size_t)n_inst * ns * kMaxKrylov));
+    TE_TRY(balloc(ctx, &b->spart, hsu.size()));
+    TE_TRY(balloc(ctx, &b->vpart, hvu.size()));
+    TE_TRY(balloc(ctx, &b->sticket, n_inst));
+    TE_TRY(balloc(ctx, &b->vticket, n_inst));
+    TE_TRY(balloc(ctx, &b->ks, n_inst));
+    TE_TRY(balloc(ctx, &b->st, n_inst));
+    TE_TRY(balloc(ctx, &b->bc, n_inst));
+    TE_TRY(balloc(ctx, &b->consts, n_inst));
+    TE_TRY(balloc(ctx, &b->diag, (size_t)std::max(n_obs, 1) * total));
+    TE_TRY(balloc(ctx, &b->opart, hvu.size() * std::max(n_obs, 1) * ns));
+    TE_TRY(balloc(ctx, &b->obs, (size_t)n_inst * std::max(n_obs, 1) * ns));
+    TE_TRY(balloc(ctx, &b->nrm, n_inst));
+    TE_TRY(balloc(ctx, &b->trace, (size_t)n_inst * kBatchTraceCap));
+    TE_TRY(balloc(ctx, &b->n_live, 1));
+    TE_TRYH(hipHostMalloc((void**)&b->n_live_host, sizeof(int)));
+    TE_TRY(upload_quadrature(ctx, b->quad));
+
+    hipStream_t s = ctx->stream;
+    TE_TRYH(hipMemcpyAsync(b->inst, hinst.data(), sizeof(BatchInst) * n_inst, hipMemcpyHostToDevice, s));
+    TE_TRYH(hipMemcpyAsync(b->sunit, hsu.data(), sizeof(int4) * hsu.size(), hipMemcpyHostToDevice, s));
+    TE_TRYH(hipMemcpyAsync(b->vunit, hvu.data(), sizeof(int4) * hvu.size(), hipMemcpyHostToDevice, s));
+    TE_TRYH(hipMemsetAsync(b->sticket, 0, sizeof(unsigned) * n_inst, s));
+    TE_TRYH(hipMemsetAsync(b->vticket, 0, sizeof(unsigned) * n_inst, s));
+    TE_TRYH(hipMemsetAsync(b->psi0, 0, sizeof(double2) * total, s));
+    TE_TRYH(hipMemsetAsync(b->diag, 0, sizeof(double) * std::max(n_obs, 1) * total, s));
+    std::vector<StepConsts> hc(n_inst);
+    for (int i = 0; i < n_inst; i++) {
+        TE_TRYH(hipMemcpyAsync(b->psi0 + b->off[i], psi0[i], sizeof(double2) * b->dim[i], hipMemcpyHostToDevice, s));
+        for (int o = 0; o < n_obs; o++)
+            TE_TRYH(hipMemcpyAsync(b->diag + (size_t)o * total + b->off[i], obs[(size_t)i * n_obs + o].matrix->diag_re,
+                                   sizeof(double) * b->dim[i], hipMemcpyDeviceToDevice, s));
+        StepConsts C = step_consts(b->P, b->tolRate, b->numErrEst[i], b->n_samples, b->quad);
+        C.Y = b->Y + (size_t)i * ns * kMaxKrylov;
+        C.wcoef = b->wcoef + (size_t)i * kMaxKrylov;
+        C.trace = b->trace + (size_t)i * kBatchTraceCap;
+        C.trace_cap = kBatchTraceCap;
+        hc[i] = C;
+    }
+    TE_TRYH(hipMemcpyAsync(b->consts, hc.data(), sizeof(StepConsts) * n_inst, hipMemcpyHostToDevice, s));
+    TE_TRYH(hipStreamSynchronize(s));
+    // |‖psi0‖ - 1| <= tol (:45-48), per instance
+    b->dev.inst = b->inst;
+    b->dev.sampled = b->psi0;
+    b->dev.nrm = b->nrm;
+    hipLaunchKernelGGL(batch_norm_kernel, dim3(n_inst), dim3(kBlock), 0, s, b->dev);
+    std::vector<double> n2(n_inst);
+    TE_TRYH(hipMemcpyAsync(n2.data(), b->nrm, sizeof(double) * n_inst, hipMemcpyDeviceToHost, s));
+    TE_TRYH(hipStreamSynchronize(s));
+    for (int i = 0; i < n_inst; i++)
+        if (std::fabs(std::sqrt(n2[i]) - 1.0) > params->tol) {
+            batch_free(b);
+            return fail(ctx, TE_ERR_FATAL, "instance %d: Initial vector is not normalized", i);
+        }
+
+    BatchDev& d = b->dev;
+    d.inst = b->inst;
+    d.sunit = b->sunit;
+    d.n_sunits = (int)hsu.size();
+    d.vunit = b->vunit;
+    d.n_vunits = (int)hvu.size();
+    d.U = b->U;
+    d.ld = total;
+    d.alpha = make_double2(0.0, -1.0 * params->exp_factor);                               // :31
+    d.tolRate = b->tolRate;
+    d.ks = b->ks;
+    d.st = b->st;
+    d.bc = b->bc;
+    d.spart = b->spart;
+    d.sticket = b->sticket;
+    d.vpart = b->vpart;
+    d.vticket = b->vticket;
+    d.wcoef = b->wcoef;
+    d.Y = b->Y;
+    d.n_samples = b->n_samples;
+    d.diag = b->diag;
+    d.nobs = n_obs;
+    d.opart = b->opart;
+    d.obs = b->obs;
+    d.sampled = b->sampled;
+    d.nrm = b->nrm;
+    d.n_inst = n_inst;
+    d.n_live = b->n_live;
+    // LDS of the apply kernel (36 KB) allows four blocks per CU
+    auto grid = [&](int items, int per_cu) {
+        int g = std::min(items, ctx->num_cus * per_cu);
+        return std::max(8, (g + 7) / 8 * 8);
+    };
+    b->grid_s = grid(d.n_sunits, 4);
+    b->grid_v = grid(d.n_vunits, 4);
+    *out = b;
+    return TE_OK;
+}
+
+extern "C" void te_batch_destroy(te_batch* b) { te::batch_free(b); }
+
+extern "C" int te_batch_info(const te_batch* b, size_t* packed_rows, size_t* n_samples, long long* rounds) {
+    if (!b) return TE_ERR_ARG;
+    if (packed_rows) *packed_rows = (size_t)b->ld;
+    if (n_samples) *n_samples = (size_t)b->n_samples;
+    if (rounds) *rounds = b->rounds;
+    return TE_OK;
+}
+
+extern "C" int te_batch_set_graph(te_batch* b, int enable) {
+    if (!b) return TE_ERR_ARG;
+    b->use_graph = enable != 0;
+    return TE_OK;
+}
+
+#undef TE_TRY
+#undef TE_TRYH
+
+extern "C" int te_batch_run(te_batch* b, double* obs_values, double* const* states, te_result* results) {
+    using namespace te;
+    if (!b || !results) return TE_ERR_ARG;
+    te_ctx* ctx = b->ctx;
+    hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    const int n_inst = b->n_inst;
+    const size_t ns = (size_t)b->n_samples;
+
+    // initial state: U[:,0] = psi0, the sample at t = 0 (:305-313) through the combine pass
+    std::vector<StepState> hst(n_inst);
+    std::vector<BatchCtl> hbc(n_inst);
+    std::vector<KrylovScalars> hks(n_inst);
+    for (int i = 0; i < n_inst; i++) {
+        StepState init{};
+        init.t_step = 1.0 / 10.0;                                                      // :285
+        init.m = b->m0[i];
+        init.index_samples = 1;
+        init.S = 1;
+        init.sample_base = 0;
+        init.write_w = 0;
+        hst[i] = init;
+        hbc[i].live = b->n_samples > 1 ? 1 : 0;
+        hbc[i].stepped = 1;
+        hbc[i].cm = 1;
+        hbc[i].stall = 0;
+        KrylovScalars k{};
+        k.s[0] = 1.0;
+        hks[i] = k;
+    }
+    std::vector<double2> e1((size_t)n_inst * kMaxKrylov, make_double2(0.0, 0.0));
+    for (int i = 0; i < n_inst; i++) e1[(size_t)i * kMaxKrylov] = make_double2(1.0, 0.0);
+    TE_HIP(ctx, hipMemcpyAsync(b->st, hst.data(), sizeof(StepState) * n_inst, hipMemcpyHostToDevice, s));
+    TE_HIP(ctx, hipMemcpyAsync(b->bc, hbc.data(), sizeof(BatchCtl) * n_inst, hipMemcpyHostToDevice, s));
+    TE_HIP(ctx, hipMemcpyAsync(b->ks, hks.data(), sizeof(KrylovScalars) * n_inst, hipMemcpyHostToDevice, s));
+    for (int i = 0; i < n_inst; i++)
+        TE_HIP(ctx, hipMemcpyAsync(b->Y + (size_t)i * ns * kMaxKrylov, e1.data() + (size_t)i * kMaxKrylov,
+                                   sizeof(double2) * kMaxKrylov, hipMemcpyHostToDevice, s));
+    TE_HIP(ctx, hipMemcpyAsync(b->U, b->psi0, sizeof(double2) * b->ld, hipMemcpyDeviceToDevice, s));
+    TE_HIP(ctx, hipMemsetAsync(b->obs, 0, sizeof(double) * n_inst * std::max(b->nobs, 1) * ns, s));
+    batch_combine(b, s);
+    TE_HIP(ctx, hipGetLastError());
+    if (b->P.fast_integration)
+        log_msg(2, "Please note that a less accurate method for evaluating the error integral was used. We "
+                   "recommend recomputing with accurate integration to establish the validity of the result.");
+
+    // rounds until no instance is live
+    if (b->use_graph && !b->exec) {
+        TE_HIP(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        batch_round(b, s);
+        batch_combine(b, s);
+        hipGraph_t g = nullptr;
+        TE_HIP(ctx, hipStreamEndCapture(s, &g));
+        b->graph = g;
+        TE_HIP(ctx, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    b->rounds = 0;
+    TE_HIP(ctx, hipMemcpyAsync(b->n_live_host, b->n_live, sizeof(int), hipMemcpyDeviceToHost, s));
+    TE_HIP(ctx, hipStreamSynchronize(s));
+    const long long max_rounds = 1LL << 24;
+    while (*b->n_live_host > 0) {
+        if (b->rounds >= max_rounds) return fail(ctx, TE_ERR_FATAL, "batched evolution makes no progress");
+        if (b->use_graph) {
+            TE_HIP(ctx, hipGraphLaunch(b->exec, s));
+        } else {
+            batch_round(b, s);
+            batch_combine(b, s);
+        }
+        TE_HIP(ctx, hipGetLastError());
+        TE_HIP(ctx, hipMemcpyAsync(b->n_live_host, b->n_live, sizeof(int), hipMemcpyDeviceToHost, s));
+        TE_HIP(ctx, hipStreamSynchronize(s));
+        b->rounds++;
+    }
+
+    // results (:445-504, :662-676)
+    hipLaunchKernelGGL(batch_norm_kernel, dim3(n_inst), dim3(kBlock), 0, s, b->dev);
+    std::vector<double> n2(n_inst);
+    TE_HIP(ctx, hipMemcpyAsync(hst.data(), b->st, sizeof(StepState) * n_inst, hipMemcpyDeviceToHost, s));
+    TE_HIP(ctx, hipMemcpyAsync(n2.data(), b->nrm, sizeof(double) * n_inst, hipMemcpyDeviceToHost, s));
+    if (obs_values && b->nobs > 0)
+        TE_HIP(ctx, hipMemcpyAsync(obs_values, b->obs, sizeof(double) * n_inst * b->nobs * ns, hipMemcpyDeviceToHost,
+                                   s));
+    if (states)
+        for (int i = 0; i < n_inst; i++)
+            if (states[i])
+                TE_HIP(ctx, hipMemcpyAsync(states[i], b->sampled + b->off[i], sizeof(double2) * b->dim[i],
+                                           hipMemcpyDeviceToHost, s));
+    TE_HIP(ctx, hipStreamSynchronize(s));
+    int first_fatal = -1, fatal_code = 0;
+    for (int i = 0; i < n_inst; i++) {
+        const StepState& h = hst[i];
+        te_result R{};
+        R.dim = (size_t)b->dim[i];
+        R.err = h.err;
+        R.evolved_time = h.t_now;
+        R.num_samples = (size_t)h.index_samples;
+        R.n_steps = (size_t)h.n_steps;
+        R.krylov_dim = (size_t)h.m;
+        if (h.fatal) {
+            R.status_code = -1;
+            if (first_fatal < 0) {
+                first_fatal = i;
+                fatal_code = h.fatal;
+            }
+        } else {
+            R.status_code = final_status(h, b->numErrEst[i], b->P.tol, std::sqrt(n2[i]));
+        }
+        results[i] = R;
+    }
+    if (first_fatal >= 0) {
+        const char* why = fatal_code == 1 ? "No small enough time step found to meet tolerance requirements."
+                          : fatal_code == 2 ? "eigen-solver did not converge"
+                                            : "time evolution makes no progress (t_step = 0)";
+        return fail(ctx, TE_ERR_FATAL, "instance %d: %s", first_fatal, why);
+    }
+    return TE_OK;
+}

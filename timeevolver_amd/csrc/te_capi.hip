@@ -132,6 +132,89 @@ static std::vector<double2> gauss_legendre(int n) {
     return pos;
 }
 
+int upload_quadrature(te_ctx* ctx, DevQuad& d) {
+    QuadTables q;
+    build_tanh_sinh(q, 15);
+    q.gl15 = gauss_legendre(15);
+    q.gl7 = gauss_legendre(7);
+    d.ts_rows = (int)q.ts_off.size() - 1;
+    d.gl15_n = (int)q.gl15.size();
+    d.gl7_n = (int)q.gl7.size();
+    TE_HIP(ctx, hipMalloc(&d.ts_nodes, sizeof(double2) * q.ts_nodes.size()));
+    TE_HIP(ctx, hipMalloc(&d.ts_off, sizeof(int) * q.ts_off.size()));
+    TE_HIP(ctx, hipMalloc(&d.gl15, sizeof(double2) * q.gl15.size()));
+    TE_HIP(ctx, hipMalloc(&d.gl7, sizeof(double2) * q.gl7.size()));
+    TE_HIP(ctx, hipMemcpy(d.ts_nodes, q.ts_nodes.data(), sizeof(double2) * q.ts_nodes.size(), hipMemcpyHostToDevice));
+    TE_HIP(ctx, hipMemcpy(d.ts_off, q.ts_off.data(), sizeof(int) * q.ts_off.size(), hipMemcpyHostToDevice));
+    TE_HIP(ctx, hipMemcpy(d.gl15, q.gl15.data(), sizeof(double2) * q.gl15.size(), hipMemcpyHostToDevice));
+    TE_HIP(ctx, hipMemcpy(d.gl7, q.gl7.data(), sizeof(double2) * q.gl7.size(), hipMemcpyHostToDevice));
+    return TE_OK;
+}
+
+void free_quadrature(DevQuad& d) {
+    if (d.ts_nodes) hipFree(d.ts_nodes);
+    if (d.ts_off) hipFree(d.ts_off);
+    if (d.gl15) hipFree(d.gl15);
+    if (d.gl7) hipFree(d.gl7);
+    d = DevQuad{};
+}
+
+StepConsts step_consts(const te_evolve_params& P, double tolRate, double numErrEst, long long n_samples,
+                       const DevQuad& q) {
+    StepConsts C{};
+    C.t = P.t;
+    C.samplingStep = P.sampling_step;
+    C.tolRate = tolRate;
+    C.tol = P.tol;
+    C.numErrEst = numErrEst;
+    C.n_samples = n_samples;
+    C.fastIntegration = P.fast_integration;
+    C.ts_rows = q.ts_rows;
+    C.ts_nodes = q.ts_nodes;
+    C.ts_off = q.ts_off;
+    C.gl15 = q.gl15;
+    C.gl7 = q.gl7;
+    C.gl15_n = q.gl15_n;
+    C.gl7_n = q.gl7_n;
+    return C;
+}
+
+int final_status(const StepState& h, double numErrEst, double tol, double state_norm) {
+    int statusCode = h.statusCode;
+    int nbErrors = 0;
+    if (std::fabs(state_norm - 1.0) > tol) {                                            // :446-452
+        log_msg(2, "CRITICAL WARNING: Norm of state vector is not inside specified tolerance. THE DESIRED ERROR "
+                   "BOUND WILL LIKELY BE VIOLATED.");
+        nbErrors++;
+        statusCode = 30;
+    }
+    const double total = numErrEst * (double)h.n_steps;                                // :459-504
+    if (h.nbErrRoundOff != 0) {
+        log_msg(2, "CRITICAL WARNING: The computed error bound was smaller than the estimate of the numerical "
+                   "error in %d Krylov spaces. THE DESIRED ERROR BOUND WILL LIKELY BE VIOLATED.",
+                h.nbErrRoundOff);
+        nbErrors++;
+        statusCode = 10;
+    }
+    if (h.nbErrInt != 0) {
+        log_msg(2, "CRITICAL WARNING: The error of numerical integration did not meet the required accuarcy in "
+                   "%d Krylov spaces. THE DESIRED ERROR BOUND MAY BE VIOLATED.",
+                h.nbErrInt);
+        nbErrors++;
+        statusCode = 20;
+    }
+    if (total > h.err && h.nbErrRoundOff == 0) {
+        if (total > tol) {
+            statusCode = 11;
+            nbErrors++;
+        } else if (statusCode != 1) {
+            statusCode = 2;
+        }
+    }
+    if (nbErrors > 1) statusCode = 100;
+    return statusCode;
+}
+
 }  // namespace te
 
 using namespace te;
@@ -418,9 +501,7 @@ struct te_evolver {
     std::vector<double2*> vec_dev;  // per observable (vector kind)
     std::vector<double> host_vals;  // n_obs x n_samples, host-callback observables
     std::vector<double2> host_state;
-    double2 *ts_nodes = nullptr, *gl15 = nullptr, *gl7 = nullptr;
-    int* ts_off = nullptr;
-    int ts_rows = 0, gl15_n = 0, gl7_n = 0;
+    DevQuad quad;
     ReduceScratch rs;
     int grid_stream = 8;
     std::vector<double2> e1;
@@ -446,8 +527,7 @@ static void evolver_free(te_evolver* ev) {
     hipSetDevice(ev->ctx->device);
     for (auto& v : ev->prof_ev)
         for (hipEvent_t e : v) hipEventDestroy(e);
-    double2* ptrs[] = {ev->psi0,    ev->U,   ev->sampled,  ev->tmp, ev->tmp2,    ev->wcoef,
-                       ev->Y,       ev->ts_nodes, ev->gl15, ev->gl7, ev->tmp_ref, ev->state_out};
+    double2* ptrs[] = {ev->psi0, ev->U, ev->sampled, ev->tmp, ev->tmp2, ev->wcoef, ev->Y, ev->tmp_ref, ev->state_out};
     for (double2* p : ptrs)
         if (p) hipFree(p);
     if (ev->ks) hipFree(ev->ks);
@@ -457,7 +537,7 @@ static void evolver_free(te_evolver* ev) {
     if (ev->obs_dev) hipFree(ev->obs_dev);
     if (ev->diag_all) hipFree(ev->diag_all);
     if (ev->diag_vals) hipFree(ev->diag_vals);
-    if (ev->ts_off) hipFree(ev->ts_off);
+    free_quadrature(ev->quad);
     for (double2* p : ev->vec_dev)
         if (p) hipFree(p);
     reduce_scratch_free(ev->rs);
@@ -640,30 +720,7 @@ static int evolver_create(te_ctx* ctx, te_matrix* H, const void* psi0, bool psi0
     }
 
     // quadrature tables
-    QuadTables q;
-    build_tanh_sinh(q, 15);
-    q.gl15 = gauss_legendre(15);
-    q.gl7 = gauss_legendre(7);
-    ev->ts_rows = (int)q.ts_off.size() - 1;
-    ev->gl15_n = (int)q.gl15.size();
-    ev->gl7_n = (int)q.gl7.size();
-    TE_TRY(dmalloc(ctx, &ev->ts_nodes, q.ts_nodes.size()));
-    TE_TRY(dmalloc(ctx, &ev->ts_off, q.ts_off.size()));
-    TE_TRY(dmalloc(ctx, &ev->gl15, q.gl15.size()));
-    TE_TRY(dmalloc(ctx, &ev->gl7, q.gl7.size()));
-    TE_TRY(hipMemcpy(ev->ts_nodes, q.ts_nodes.data(), sizeof(double2) * q.ts_nodes.size(), hipMemcpyHostToDevice) ==
-                   hipSuccess
-               ? TE_OK
-               : TE_ERR_HIP);
-    TE_TRY(hipMemcpy(ev->ts_off, q.ts_off.data(), sizeof(int) * q.ts_off.size(), hipMemcpyHostToDevice) == hipSuccess
-               ? TE_OK
-               : TE_ERR_HIP);
-    TE_TRY(hipMemcpy(ev->gl15, q.gl15.data(), sizeof(double2) * q.gl15.size(), hipMemcpyHostToDevice) == hipSuccess
-               ? TE_OK
-               : TE_ERR_HIP);
-    TE_TRY(hipMemcpy(ev->gl7, q.gl7.data(), sizeof(double2) * q.gl7.size(), hipMemcpyHostToDevice) == hipSuccess
-               ? TE_OK
-               : TE_ERR_HIP);
+    TE_TRY(upload_quadrature(ctx, ev->quad));
     ev->e1.assign(kMaxKrylov, make_double2(0.0, 0.0));
     ev->e1[0] = make_double2(1.0, 0.0);
     ev->host_state.resize(ev->generic_obs.empty() ? 0 : n);
@@ -871,21 +928,7 @@ extern "C" int te_evolver_run(te_evolver* ev, double* obs_values, double* evolve
     la.ks = ev->ks;
     la.partials = ev->rs.partials;
     la.ticket = ev->rs.ticket;
-    StepConsts C{};
-    C.t = ev->P.t;
-    C.samplingStep = ev->P.sampling_step;
-    C.tolRate = ev->tolRate;
-    C.tol = ev->P.tol;
-    C.numErrEst = ev->numErrEst;
-    C.n_samples = ev->n_samples;
-    C.fastIntegration = ev->P.fast_integration;
-    C.ts_rows = ev->ts_rows;
-    C.ts_nodes = ev->ts_nodes;
-    C.ts_off = ev->ts_off;
-    C.gl15 = ev->gl15;
-    C.gl7 = ev->gl7;
-    C.gl15_n = ev->gl15_n;
-    C.gl7_n = ev->gl7_n;
+    StepConsts C = step_consts(ev->P, ev->tolRate, ev->numErrEst, ev->n_samples, ev->quad);
     C.Y = ev->Y;
     C.wcoef = ev->wcoef;
     C.trace = ev->trace;
@@ -953,44 +996,12 @@ extern "C" int te_evolver_run(te_evolver* ev, double* obs_values, double* evolve
         R.num_samples = (size_t)h.index_samples;
         R.n_steps = (size_t)h.n_steps;
         R.krylov_dim = (size_t)h.m;
-        statusCode = h.statusCode;
-        // norm check (:446-452)
-        int nbErrors = 0;
+        // norm check (:446-452) and status aggregation (:459-504)
         launch_nrm2sq(ev->sampled, n, ev->rs, 0, s);
         double2 r2;
         TE_HIP(ctx, hipMemcpyAsync(&r2, ev->rs.result, sizeof(double2), hipMemcpyDeviceToHost, s));
         TE_HIP(ctx, hipStreamSynchronize(s));
-        if (std::fabs(std::sqrt(r2.x) - 1.0) > ev->P.tol) {
-            log_msg(2, "CRITICAL WARNING: Norm of state vector is not inside specified tolerance. THE DESIRED ERROR "
-                       "BOUND WILL LIKELY BE VIOLATED.");
-            nbErrors++;
-            statusCode = 30;
-        }
-        // status aggregation (:459-504)
-        const double total = ev->numErrEst * (double)h.n_steps;
-        if (h.nbErrRoundOff != 0) {
-            log_msg(2, "CRITICAL WARNING: The computed error bound was smaller than the estimate of the numerical "
-                       "error in %d Krylov spaces. THE DESIRED ERROR BOUND WILL LIKELY BE VIOLATED.",
-                    h.nbErrRoundOff);
-            nbErrors++;
-            statusCode = 10;
-        }
-        if (h.nbErrInt != 0) {
-            log_msg(2, "CRITICAL WARNING: The error of numerical integration did not meet the required accuarcy in "
-                       "%d Krylov spaces. THE DESIRED ERROR BOUND MAY BE VIOLATED.",
-                    h.nbErrInt);
-            nbErrors++;
-            statusCode = 20;
-        }
-        if (total > h.err && h.nbErrRoundOff == 0) {
-            if (total > ev->P.tol) {
-                statusCode = 11;
-                nbErrors++;
-            } else if (statusCode != 1) {
-                statusCode = 2;
-            }
-        }
-        if (nbErrors > 1) statusCode = 100;
+        statusCode = final_status(h, ev->numErrEst, ev->P.tol, std::sqrt(r2.x));
     }
     R.status_code = statusCode;
 

@@ -95,9 +95,39 @@ struct CombineArgs {
     unsigned int* ticket;
 };
 
+// Quadrature tables of the error integrals (tanh-sinh rows, Gauss-Legendre 15 and 7) in device
+// memory, computed on the host with libm.
+struct DevQuad {
+    double2* ts_nodes = nullptr;
+    int* ts_off = nullptr;
+    double2* gl15 = nullptr;
+    double2* gl7 = nullptr;
+    int ts_rows = 0, gl15_n = 0, gl7_n = 0;
+};
+int upload_quadrature(te_ctx* ctx, DevQuad& q);
+void free_quadrature(DevQuad& q);
+// The StepConsts fields shared by every instance of one parameter set (Y, wcoef and trace are
+// left null).
+StepConsts step_consts(const te_evolve_params& P, double tolRate, double numErrEst, long long n_samples,
+                       const DevQuad& q);
+// Final status of a run from its last StepState (core/krylovTimeEvolver.cpp:445-504): the norm
+// check of the returned state, then the error counters; logs the reference's warnings.
+int final_status(const StepState& h, double numErrEst, double tol, double state_norm);
+
+// Per-instance control flags of the batched driver, updated by the batched step controller.
+struct BatchCtl {
+    int live;       // the instance still has samples to take and no fatal error
+    int stepped;    // the controller completed a Krylov step of it in the current round
+    int cm;         // Krylov dimension of that step (the columns the combine pass reads)
+    int stall;      // consecutive steps with t_step = 0 and no sample
+};
+
 void launch_lanczos_apply(LanczosArgs a, const te_matrix* A, hipStream_t s);
 void launch_lanczos_project(const LanczosArgs& a, int grid, hipStream_t s);
 void launch_step_control(const StepConsts& c, StepState* st, KrylovScalars* ks, hipStream_t s);
+// One block per instance (c, st, ks, bc: n_inst entries each, device memory).
+void launch_batch_step_control(const StepConsts* c, StepState* st, KrylovScalars* ks, BatchCtl* bc, int n_inst,
+                               long long n_samples, hipStream_t s);
 // Picks the sample/observable chunking; S <= max_samples_per_launch(nobs).
 int combine_samples_per_launch(int nobs);
 int combine_obs_per_launch();

@@ -567,9 +567,10 @@ __device__ int find_step(CtlShared& S, const StepConsts& C, double t_stepSuggest
     return code;
 }
 
-// One Krylov step of timeEvolve after the Lanczos recurrence (:342-441).
-__global__ __launch_bounds__(kCtl) void step_control_kernel(StepConsts C, StepState* st, KrylovScalars* ks) {
-    __shared__ CtlShared S;
+// One Krylov step of timeEvolve after the Lanczos recurrence (:342-441), run by a block of
+// kCtl threads. Returns 0, or the fatal code it also stores in st->fatal (1: no small enough
+// step, :203-207; 2: the eigen-solver failed, :366-370); the value is the same in every thread.
+__device__ int step_control_body(CtlShared& S, const StepConsts& C, StepState* st, KrylovScalars* ks) {
     const int tid = threadIdx.x;
     const long long clk0 = wall_clock64();
     StepState L = *st;
@@ -601,7 +602,7 @@ __global__ __launch_bounds__(kCtl) void step_control_kernel(StepConsts C, StepSt
     __syncthreads();
     if (S.info != 0) {                               // :366-370
         if (tid == 0) st->fatal = 2;
-        return;
+        return 2;
     }
     if (tid < m) {
         S.lam[tid] = S.d[tid];
@@ -618,7 +619,7 @@ __global__ __launch_bounds__(kCtl) void step_control_kernel(StepConsts C, StepSt
         const int code = find_step(S, C, s0, C.t - L.t_now, 50, L.t_now == 0, mLong, mShort, &t_step, &err_step);
         if (code < 0) {
             if (tid == 0) st->fatal = 1;
-            return;
+            return 1;
         }
         L.errorCodeFindSubstep = code;
     }
@@ -656,6 +657,46 @@ __global__ __launch_bounds__(kCtl) void step_control_kernel(StepConsts C, StepSt
         *st = L;
         ks->broken = 0;
         ks->s[0] = 1.0;
+    }
+    return 0;
+}
+
+__global__ __launch_bounds__(kCtl) void step_control_kernel(StepConsts C, StepState* st, KrylovScalars* ks) {
+    __shared__ CtlShared S;
+    step_control_body(S, C, st, ks);
+}
+
+// Batched form: block i runs the Krylov step of instance i when it is live, then records what
+// the round's combine pass needs and whether the instance goes on. The stall rule is the one
+// the single-instance host loop applies: two consecutive steps with t_step = 0 and no sample.
+__global__ __launch_bounds__(kCtl) void batch_step_control_kernel(const StepConsts* Cs, StepState* st,
+                                                                  KrylovScalars* ks, BatchCtl* bc,
+                                                                  long long n_samples) {
+    __shared__ CtlShared S;
+    const int i = blockIdx.x;
+    if (!bc[i].live) {
+        if (threadIdx.x == 0) bc[i].stepped = 0;
+        return;
+    }
+    const int rc = step_control_body(S, Cs[i], st + i, ks + i);
+    if (threadIdx.x == 0) {
+        BatchCtl b = bc[i];
+        if (rc != 0) {
+            b.live = 0;
+            b.stepped = 0;
+        } else {
+            const StepState L = st[i];
+            b.stepped = 1;
+            b.cm = L.m;
+            b.stall = (L.t_step == 0.0 && L.S == 0) ? b.stall + 1 : 0;
+            if (b.stall >= 2) {
+                st[i].fatal = 3;
+                b.live = 0;
+            } else {
+                b.live = L.index_samples < n_samples ? 1 : 0;
+            }
+        }
+        bc[i] = b;
     }
 }
 
@@ -755,6 +796,11 @@ __global__ void store_obs_kernel(const double2* res, int slot, int mode, double*
 // ---------------------------------------------------------------------------------------
 void launch_step_control(const StepConsts& c, StepState* st, KrylovScalars* ks, hipStream_t s) {
     hipLaunchKernelGGL(step_control_kernel, dim3(1), dim3(kCtl), 0, s, c, st, ks);
+}
+
+void launch_batch_step_control(const StepConsts* c, StepState* st, KrylovScalars* ks, BatchCtl* bc, int n_inst,
+                               long long n_samples, hipStream_t s) {
+    hipLaunchKernelGGL(batch_step_control_kernel, dim3(n_inst), dim3(kCtl), 0, s, c, st, ks, bc, n_samples);
 }
 
 int combine_obs_per_launch() { return 32; }

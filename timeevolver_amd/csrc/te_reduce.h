@@ -6,6 +6,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "te_internal.h"
+
 namespace te {
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -114,6 +116,76 @@ __device__ __forceinline__ void xcd_sweep_waves(int n_items, int& first, int& en
     first = (int)((long long)n_items * x / 8) + (b >> 3) * wpb + w;
     end = (int)((long long)n_items * (x + 1) / 8);
     stride = (g >> 3) * wpb;
+}
+
+// ---------------------------------------------------------------------------------------
+// One row block of the row-block ("CSR-stream") SpMV: rows [r0, r1) whose entries fit
+// kStreamCap. Every thread issues all of its value, index and x loads (kStreamCap / kBlock of
+// each) before using any, the products go to LDS (prod: kStreamCap entries), and thread t then
+// sums rows r0 + t, ... in entry order, continuing acc_in[r] when given, and hands each sum to
+// epi.row(r, sum). Ends with a barrier, so prod can be reused at once.
+// ---------------------------------------------------------------------------------------
+typedef double d2v __attribute__((ext_vector_type(2)));
+
+// Streamed once per SpMV: loaded non-temporally so the gathered x keeps its place in L2/MALL.
+__device__ __forceinline__ double2 load_stream(const double2* p) {
+    const d2v t = __builtin_nontemporal_load(reinterpret_cast<const d2v*>(p));
+    return make_double2(t.x, t.y);
+}
+__device__ __forceinline__ void store_stream(double2* p, double2 v) {
+    d2v t;
+    t.x = v.x;
+    t.y = v.y;
+    __builtin_nontemporal_store(t, reinterpret_cast<d2v*>(p));
+}
+
+constexpr int kNI = kStreamCap / kBlock;
+static_assert(kStreamCap % kBlock == 0, "stream capacity must be a multiple of the block size");
+
+template <class Epi>
+__device__ __forceinline__ void stream_row_block(int r0, int r1, const int* __restrict__ ptr,
+                                                 const int* __restrict__ col, const double2* __restrict__ val,
+                                                 const double2* __restrict__ x, double2* prod, Epi& epi,
+                                                 const double2* __restrict__ acc_in) {
+    const int t = threadIdx.x;
+    const int k0 = ptr[r0];
+    const int nk = ptr[r1] - k0;
+    double2 v[kNI], xv[kNI];
+    int c[kNI];
+#pragma unroll
+    for (int i = 0; i < kNI; i++) {
+        const int k = t + i * kBlock;
+        if (k < nk) {
+            v[i] = load_stream(&val[k0 + k]);
+            c[i] = __builtin_nontemporal_load(&col[k0 + k]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kNI; i++) {
+        const int k = t + i * kBlock;
+        if (k < nk) xv[i] = x[c[i]];
+    }
+#pragma unroll
+    for (int i = 0; i < kNI; i++) {
+        const int k = t + i * kBlock;
+        if (k < nk) prod[k] = cmul(v[i], xv[i]);
+    }
+    __syncthreads();
+    for (int r = r0 + t; r < r1; r += kBlock) {
+        const int a = ptr[r] - k0, e = ptr[r + 1] - k0;
+        double sr = 0.0, si = 0.0;
+        if (acc_in) {               // continue the row sum of the preceding column slices
+            const double2 p = load_stream(&acc_in[r]);
+            sr = p.x;
+            si = p.y;
+        }
+        for (int k = a; k < e; k++) {
+            sr += prod[k].x;
+            si += prod[k].y;
+        }
+        epi.row(r, make_double2(sr, si));
+    }
+    __syncthreads();
 }
 
 }  // namespace te

@@ -189,22 +189,6 @@ __global__ __launch_bounds__(kBlock) void lanczos_project_kernel(LanczosArgs a) 
 // each) before using any, the products go to LDS, and thread t then sums rows r0 + t, ... in
 // entry order.
 // ---------------------------------------------------------------------------------------
-constexpr int kNI = kStreamCap / kBlock;
-typedef double d2v __attribute__((ext_vector_type(2)));
-
-// Streamed once per SpMV: loaded non-temporally so the gathered x keeps its place in L2/MALL.
-__device__ __forceinline__ double2 load_stream(const double2* p) {
-    const d2v t = __builtin_nontemporal_load(reinterpret_cast<const d2v*>(p));
-    return make_double2(t.x, t.y);
-}
-__device__ __forceinline__ void store_stream(double2* p, double2 v) {
-    d2v t;
-    t.x = v.x;
-    t.y = v.y;
-    __builtin_nontemporal_store(t, reinterpret_cast<d2v*>(p));
-}
-static_assert(kStreamCap % kBlock == 0, "stream capacity must be a multiple of the block size");
-
 template <class Epi>
 __device__ __forceinline__ void stream_rows(const int* __restrict__ rb, int nrb, const int* __restrict__ ptr,
                                             const int* __restrict__ col, const double2* __restrict__ val,
@@ -212,48 +196,7 @@ __device__ __forceinline__ void stream_rows(const int* __restrict__ rb, int nrb,
                                             const double2* __restrict__ acc_in) {
     int b0, b1, bstride;
     xcd_sweep(nrb, b0, b1, bstride);
-    const int t = threadIdx.x;
-    for (int b = b0; b < b1; b += bstride) {
-        const int r0 = rb[b], r1 = rb[b + 1];
-        const int k0 = ptr[r0];
-        const int nk = ptr[r1] - k0;
-        double2 v[kNI], xv[kNI];
-        int c[kNI];
-#pragma unroll
-        for (int i = 0; i < kNI; i++) {
-            const int k = t + i * kBlock;
-            if (k < nk) {
-                v[i] = load_stream(&val[k0 + k]);
-                c[i] = __builtin_nontemporal_load(&col[k0 + k]);
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < kNI; i++) {
-            const int k = t + i * kBlock;
-            if (k < nk) xv[i] = x[c[i]];
-        }
-#pragma unroll
-        for (int i = 0; i < kNI; i++) {
-            const int k = t + i * kBlock;
-            if (k < nk) prod[k] = cmul(v[i], xv[i]);
-        }
-        __syncthreads();
-        for (int r = r0 + t; r < r1; r += kBlock) {
-            const int a = ptr[r] - k0, e = ptr[r + 1] - k0;
-            double sr = 0.0, si = 0.0;
-            if (acc_in) {               // continue the row sum of the preceding column slices
-                const double2 p = load_stream(&acc_in[r]);
-                sr = p.x;
-                si = p.y;
-            }
-            for (int k = a; k < e; k++) {
-                sr += prod[k].x;
-                si += prod[k].y;
-            }
-            epi.row(r, make_double2(sr, si));
-        }
-        __syncthreads();
-    }
+    for (int b = b0; b < b1; b += bstride) stream_row_block(rb[b], rb[b + 1], ptr, col, val, x, prod, epi, acc_in);
 }
 
 struct SpmvEpi {

@@ -87,3 +87,17 @@ def fock_state(D, index=0):
     v = np.zeros(D, dtype=np.complex128)
     v[index] = 1.0
     return v
+
+
+def batch_instance(base_seed, i, L_choices=(6, 7, 8, 9, 10)):
+    """Instance i of a seeded batch of ring Bose-Hubbard models: a SplitMix64 stream seeded with
+    base_seed + i draws L uniformly from L_choices (N = L), then eps_k = u and J_k = 2u - 1 for
+    k < L, then U = 0.2 + 0.8 u. The initial state is random_state(D, that seed x 7919 + 1).
+    Returns (L, N, eps, J, U, state_seed)."""
+    rng = SplitMix64(base_seed + i)
+    nL = len(L_choices)
+    L = L_choices[min(int(rng.uniform() * nL), nL - 1)]
+    eps = [rng.uniform() for _ in range(L)]
+    J = [2.0 * rng.uniform() - 1.0 for _ in range(L)]
+    U = 0.2 + 0.8 * rng.uniform()
+    return L, L, eps, J, U, (base_seed + i) * 7919 + 1
