@@ -50,6 +50,9 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=1, help="Krylov steps the CPU baseline runs")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain measurement")
+    ap.add_argument("--batch-count", type=int, default=512)
+    ap.add_argument("--batched-only", action="store_true", help="only the batched measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -64,6 +67,9 @@ def main():
     import numpy as np
 
     from timeevolver_amd import capi
+    if args.batched_only:
+        print(json.dumps({"batched": bench_batched(np, capi, capi.Context(0), args)}), flush=True)
+        return
     from timeevolver_amd.hamiltonian import as_tuples, createNumberOperator
     from timeevolver_amd.workloads import (fock_state, ring_bose_hubbard_terms, ring_couplings, ring_dimension,
                                            ring_nnz)
@@ -158,7 +164,72 @@ def main():
 
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(np, capi, H, obs, D, args, r["n_steps"])
+    if not args.no_batched:
+        del ev, H, obs, basis
+        line["batched"] = bench_batched(np, capi, ctx, args)
     print(json.dumps(line), flush=True)
+
+
+def bench_batched(np, capi, ctx, args):
+    """Batched workload: `batch_count` ring Bose-Hubbard instances, instance i from a SplitMix64
+    stream seeded with seed + i: L uniform in {6..10} (D 462 to 92,378), N = L, its own eps_k,
+    J_k and U in [0.2, 1.0], a random complex initial state; T = 5, m = 30, tol = 1e-8, sampling
+    step 0.25, observable n_0. One batched run (te_batch_run) against the same instances run one
+    after the other through the plain GPU evolver (te_evolver_run, evolvers created beforehand).
+    Metric: aggregate Krylov steps per second."""
+    from timeevolver_amd.hamiltonian import as_tuples, createNumberOperator
+    from timeevolver_amd.workloads import batch_instance, random_state, ring_bose_hubbard_terms
+
+    T, m, tol, step = 5.0, 30, 1e-8, 0.25
+    t0 = time.perf_counter()
+    Hs, psis, obs = [], [], []
+    for i in range(args.batch_count):
+        L, N, eps, J, U, sseed = batch_instance(args.seed, i)
+        b = capi.Basis.basis(ctx, N, L)
+        Hs.append(capi.operator_build(ctx, b, as_tuples(ring_bose_hubbard_terms(L, eps, J, U))))
+        obs.append([capi.operator_build(ctx, b, as_tuples([createNumberOperator(0, 1)]))])
+        psis.append(random_state(b.D, sseed))
+    ctx.synchronize()
+    build_s = time.perf_counter() - t0
+    dims = [H.n_rows for H in Hs]
+    log("batched: %d instances built in %.2f s, sum D = %d, sum nnz = %d" %
+        (len(Hs), build_s, sum(dims), sum(H.nnz for H in Hs)))
+    be = capi.BatchEvolver(ctx, Hs, psis, T, step, obs, tol=tol, m=m)
+    rb = be.run(want_states=False)                       # warmup (captures the round graph)
+    times = []
+    for _ in range(max(args.steps, 1)):
+        ctx.synchronize()
+        t = time.perf_counter()
+        rb = be.run(want_states=False)
+        ctx.synchronize()
+        times.append(time.perf_counter() - t)
+    t_batch = float(np.median(times))
+    steps_batch = sum(r["n_steps"] for r in rb)
+    rounds = be.rounds()
+    log("batched: %.3f s, %d Krylov steps in %d rounds" % (t_batch, steps_batch, rounds))
+    be.close()
+
+    evs = [capi.Evolver(ctx, H, p, T, step, o, tol=tol, m=m) for H, p, o in zip(Hs, psis, obs)]
+    evs[0].run(want_state=False)                         # warmup
+    ctx.synchronize()
+    t = time.perf_counter()
+    rp = [ev.run(want_state=False) for ev in evs]
+    ctx.synchronize()
+    t_plain = time.perf_counter() - t
+    steps_plain = sum(r["n_steps"] for r in rp)
+    log("plain, one instance after another: %.3f s, %d Krylov steps" % (t_plain, steps_plain))
+    same = all(a["n_steps"] == b["n_steps"] and a["statusCode"] == b["statusCode"] for a, b in zip(rb, rp))
+    dobs = max(float(np.abs(a["observables"] - b["observables"]).max()) for a, b in zip(rb, rp))
+    for ev in evs:
+        ev.close()
+    return {"metric": "aggregate Krylov steps per second", "unit": "steps/s",
+            "workload": "%d ring Bose-Hubbard instances, L=N uniform in {6..10}, sum D=%d, T=%g, m=%d, tol=%g, "
+                        "sampling step %g, observable n_0" % (len(dims), sum(dims), T, m, tol, step),
+            "value": round(steps_batch / t_batch, 1), "plain_value": round(steps_plain / t_plain, 1),
+            "speedup_vs_plain": round((steps_batch / t_batch) / (steps_plain / t_plain), 2),
+            "batched_s": round(t_batch, 4), "plain_s": round(t_plain, 4), "krylov_steps": steps_batch,
+            "rounds": rounds, "same_steps_and_status_as_plain": same, "max_obs_diff_vs_plain": dobs,
+            "build_s": round(build_s, 2)}
 
 
 def measured_traffic(kernel, D):

@@ -202,6 +202,39 @@ int te_batch_info(const te_batch* b, size_t* packed_rows, size_t* n_samples, lon
  * The results are identical. */
 int te_batch_set_graph(te_batch* b, int enable);
 
+/* ---- adjoint gradient (no reference counterpart: the reference only evolves forward) ---- */
+typedef struct te_adjoint te_adjoint;
+typedef struct {
+    double forward_ms;       /* device time of the forward evolution */
+    double backward_ms;      /* device time of the backward pass */
+    int n_steps;             /* Krylov steps of the forward schedule (replayed backward) */
+    long long nodes;         /* quadrature nodes over all steps */
+    double max_phase;        /* largest integrand phase |omega| h of a step */
+    int capped;              /* steps whose rule was capped at the node limit (0 expected) */
+    double backward_err;     /* sum of the a-posteriori Krylov error integrals of lambda */
+    double psi_err;          /* the same for back-evolved psi (checkpoint interval >= 1) */
+} te_adjoint_info;
+/* Loss L = <psi(t)|O|psi(t)> for psi(t) = exp(-i e H t) psi0 (e = params->exp_factor, evolved by
+ * the plain method with params' t, tol, m and fast_integration; the sampling step is ignored)
+ * and its gradient with respect to theta_p for H = H_0 + sum_p theta_p G_p:
+ *   dL/dtheta_p = 2 e int_0^t Im <lambda(s)|G_p|psi(s)> ds,  lambda(s) = exp(i e H (t-s)) O psi(t).
+ * H: the Hamiltonian at the current theta; generators[p] = G_p (n_params <= 64, Hermitian, real
+ * diagonal; borrowed); observable O: diagonal. checkpoint_interval K: 0 keeps every forward
+ * Krylov basis (no recomputation; memory steps x (m+1) x dim complex128); K >= 1 keeps psi(t_k)
+ * every K steps and back-evolves psi with lambda, re-anchored at each checkpoint. */
+int te_adjoint_create(te_ctx* ctx, te_matrix* H, int n_params, te_matrix* const* generators, te_matrix* observable,
+                      const te_evolve_params* params, int checkpoint_interval, te_adjoint** out);
+void te_adjoint_destroy(te_adjoint* a);
+/* psi0: dim complex128 (host; |‖psi0‖ - 1| <= tol else TE_ERR_FATAL). loss: L; grad: n_params;
+ * result (may be NULL): the forward run's krylovReturn fields; info (may be NULL). */
+int te_adjoint_run(te_adjoint* a, const double* psi0, double* loss, double* grad, te_result* result,
+                   te_adjoint_info* info);
+/* Replaces H (same dimension and storage order: another te_operator_build of the same basis),
+ * keeping the generators; for gradient steps or finite differences at new coefficients. */
+int te_adjoint_set_hamiltonian(te_adjoint* a, te_matrix* H);
+/* psi(t) of the last run (dim complex128, host). */
+int te_adjoint_state(te_adjoint* a, double* psiT);
+
 /* Log level filter (krylovLogger::loggingLevel, core/krylovLogger.h:17: DEBUG, INFO, WARNING,
  * ERROR, FATAL = 0..4; default 0 prints everything, as the unfiltered Boost.Log core does).
  * Messages go to stderr. */

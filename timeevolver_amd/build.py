@@ -24,6 +24,7 @@ SOURCES = {
     "te_krylov.hip": ["-ffp-contract=off"],
     "te_builder.hip": ["-ffp-contract=off"],
     "te_batch.hip": [],
+    "te_adjoint.hip": [],
 }
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang",

@@ -48,6 +48,12 @@ class te_result(_c.Structure):
                 ("status_code", _c.c_int)]
 
 
+class te_adjoint_info(_c.Structure):
+    _fields_ = [("forward_ms", _c.c_double), ("backward_ms", _c.c_double), ("n_steps", _c.c_int),
+                ("nodes", _c.c_longlong), ("max_phase", _c.c_double), ("capped", _c.c_int),
+                ("backward_err", _c.c_double), ("psi_err", _c.c_double)]
+
+
 _SIGS = {
     "te_version": (_c.c_int, []),
     "te_set_log_level": (None, [_c.c_int]),
@@ -88,6 +94,11 @@ _SIGS = {
     "te_batch_run": (_c.c_int, [_p, _dp, _c.POINTER(_dp), _c.POINTER(te_result)]),
     "te_batch_info": (_c.c_int, [_p, _szp, _szp, _c.POINTER(_c.c_longlong)]),
     "te_batch_set_graph": (_c.c_int, [_p, _c.c_int]),
+    "te_adjoint_create": (_c.c_int, [_p, _p, _c.c_int, _pp, _p, _c.POINTER(te_evolve_params), _c.c_int, _pp]),
+    "te_adjoint_destroy": (None, [_p]),
+    "te_adjoint_run": (_c.c_int, [_p, _dp, _dp, _dp, _c.POINTER(te_result), _c.POINTER(te_adjoint_info)]),
+    "te_adjoint_set_hamiltonian": (_c.c_int, [_p, _p]),
+    "te_adjoint_state": (_c.c_int, [_p, _dp]),
 }
 
 _lib = None
@@ -454,6 +465,58 @@ class BatchEvolver:
     def close(self):
         if self.h:
             self.ctx.L.te_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Adjoint:
+    """te_adjoint_*: L = <psi(t)|O|psi(t)> and dL/dtheta_p for H = H_0 + sum_p theta_p G_p."""
+
+    def __init__(self, ctx, H, generators, observable, t, tol=1e-8, m=40, exp_factor=1.0, fast_integration=False,
+                 checkpoint_interval=0):
+        self.ctx = ctx
+        self._keep = [H, list(generators), observable]
+        self.P = len(generators)
+        Garr = (_p * max(self.P, 1))(*[g.h.value if isinstance(g.h, _p) else g.h for g in generators])
+        params = te_evolve_params(t, t, exp_factor, tol, m, int(fast_integration))
+        h = _p()
+        ctx.check(ctx.L.te_adjoint_create(ctx.h, H.h, self.P, Garr, observable.h, _c.byref(params),
+                                          int(checkpoint_interval), _c.byref(h)), "te_adjoint_create")
+        self.h = h
+        self.dim = H.n_rows
+
+    def set_hamiltonian(self, H):
+        self.ctx.check(self.ctx.L.te_adjoint_set_hamiltonian(self.h, H.h), "te_adjoint_set_hamiltonian")
+        self._keep[0] = H
+
+    def run(self, psi0):
+        """Returns (loss, grad ndarray, result dict, info dict)."""
+        psi = np.ascontiguousarray(psi0, dtype=np.complex128)
+        loss = _c.c_double()
+        grad = np.zeros(max(self.P, 1))
+        R = te_result()
+        info = te_adjoint_info()
+        self.ctx.check(self.ctx.L.te_adjoint_run(self.h, _ptr(psi.view(np.float64), _c.c_double), _c.byref(loss),
+                                                 _ptr(grad, _c.c_double), _c.byref(R), _c.byref(info)),
+                       "te_adjoint_run")
+        res = {"err": R.err, "evolvedTime": R.evolved_time, "n_steps": R.n_steps, "statusCode": R.status_code,
+               "krylovDim": R.krylov_dim}
+        inf = {k: getattr(info, k) for k, _ in te_adjoint_info._fields_}
+        return loss.value, grad[:self.P].copy(), res, inf
+
+    def state(self):
+        out = np.zeros(self.dim, dtype=np.complex128)
+        self.ctx.check(self.ctx.L.te_adjoint_state(self.h, _ptr(out.view(np.float64), _c.c_double)), "state")
+        return out
+
+    def close(self):
+        if self.h:
+            self.ctx.L.te_adjoint_destroy(self.h)
             self.h = None
 
     def __del__(self):

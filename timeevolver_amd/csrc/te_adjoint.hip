@@ -1,0 +1,914 @@
+// Adjoint gradient of L = <psi(T)|O|psi(T)> with respect to the coefficients theta_p of a
+// Hamiltonian H = H_0 + sum_p theta_p G_p (no counterpart in the reference, which only
+// evolves forward: core/krylovTimeEvolver.cpp:261-512). With psi(s) = exp(-i e H s) psi0
+// (e = expFactor) and lambda(s) = exp(i e H (T - s)) O psi(T),
+//   dL/dtheta_p = 2 e int_0^T Im <lambda(s)| G_p |psi(s)> ds.
+//
+// Forward: the plain method (same Lanczos kernels, step controller and error bound) with the
+// step schedule, every step's Lanczos scalars and either every Krylov basis (checkpoint
+// interval K = 0) or the boundary states psi(t_k) for k % K == 0 (K >= 1) kept in HBM.
+// Backward, step by step from T on the forward's own schedule:
+//  * lambda is back-evolved: a Lanczos space of +i e H from lambda(t_{k+1});
+//  * psi on the step comes from the kept forward basis (K = 0), or is back-evolved alongside
+//    lambda from psi(t_{k+1}) and replaced by the forward checkpoint every K steps, which
+//    bounds its drift (K >= 1);
+//  * both spaces are decomposed in one kernel, which picks a Gauss-Legendre rule for the
+//    step from their Ritz spectra (adjoint_coef_kernel);
+//  * the node states of both are formed in one pass over each basis, stored node-interleaved
+//    (a row holds 8 consecutive nodes, 128 bytes), and one labelled term-wise pass per group
+//    of 8 nodes evaluates every off-diagonal generator at once; diagonal generators reduce to
+//    one running vector Z(r) = sum_nodes w Im(conj(lambda(r)) psi(r)), dotted with each
+//    generator's diagonal at the end.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "te_internal.h"
+#include "te_krylov.h"
+#include "te_reduce.h"
+
+namespace te {
+
+constexpr int kNodeGroup = 8;       // nodes per node-interleaved block
+constexpr int kMatNodes = 64;       // nodes per materialisation pass
+constexpr int kAdjMaxLabels = 64;   // generators (LDS accumulators of the term-wise pass)
+constexpr int kAdjMaxSteps = 1 << 16;
+
+__global__ void diag_mul_kernel(const double* __restrict__ d, const double2* __restrict__ x, double2* __restrict__ y,
+                                int64_t n) {
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x)
+        y[r] = make_double2(d[r] * x[r].x, d[r] * x[r].y);
+}
+
+// After |v|^2 landed in res[0].x: the Lanczos start v_0 = v / |v| (s_0 = 1 / |v|), scale = |v|.
+__global__ void start_norm_kernel(const double2* res, KrylovScalars* ks, double* scale) {
+    const double nv = sqrt(res[0].x);
+    ks->s[0] = nv > 0.0 ? 1.0 / nv : 1.0;
+    ks->broken = 0;
+    *scale = nv;
+}
+
+// Node states of one basis: out[g][r][q'] = sum_j U[j][r] coef[node0 + 8 g + q'][j] for the
+// nodes of this pass (zero past the step's node count; out holds the pass's 8 blocks); end[r] =
+// sum_j U[j][r] cend[j] when end is given. Eight lanes share a row (lane g forms nodes
+// 8g..8g+7 of the pass), so the basis is read once per pass and every lane writes 128
+// contiguous bytes.
+struct MatArgs {
+    const double2* U;
+    int64_t ld, n;
+    int m;
+    const double2* coef;      // kAdjMaxNodes x kMaxKrylov
+    const double2* cend;      // kMaxKrylov (with end)
+    const AdjStep* step;
+    int node0;
+    double2* out;             // node blocks, n x 8 each
+    double2* end;
+};
+
+__global__ __launch_bounds__(256) void materialise_nodes_kernel(MatArgs a) {
+    __shared__ double2 cs[kMaxKrylov * kMatNodes];
+    __shared__ double2 ce[kMaxKrylov];
+    const int nn = a.step->n_nodes;
+    const int nloc = min(kMatNodes, nn - a.node0);
+    if (nloc <= 0 && !a.end) return;
+    for (int i = threadIdx.x; i < a.m * kMatNodes; i += blockDim.x) {
+        const int j = i / kMatNodes, q = i % kMatNodes;
+        cs[i] = q < nloc ? a.coef[(size_t)(a.node0 + q) * kMaxKrylov + j] : make_double2(0.0, 0.0);
+    }
+    if (a.end)
+        for (int j = threadIdx.x; j < a.m; j += blockDim.x) ce[j] = a.cend[j];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int g = lane & 7, rl = lane >> 3;
+    const int ng = (nloc + kNodeGroup - 1) / kNodeGroup;
+    const int64_t step = (int64_t)gridDim.x * 32;
+    for (int64_t r0 = ((int64_t)blockIdx.x * 4 + w) * 8; r0 < a.n; r0 += step) {
+        const int64_t r = r0 + rl;
+        if (r >= a.n) continue;
+        double2 acc[kNodeGroup];
+#pragma unroll
+        for (int q = 0; q < kNodeGroup; q++) acc[q] = make_double2(0.0, 0.0);
+        double2 ae = make_double2(0.0, 0.0);
+        for (int j = 0; j < a.m; j++) {
+            const double2 u = a.U[(size_t)j * a.ld + r];
+            const double2* c = cs + j * kMatNodes + g * kNodeGroup;
+#pragma unroll
+            for (int q = 0; q < kNodeGroup; q++) {
+                const double2 t = cmul(u, c[q]);
+                acc[q].x += t.x;
+                acc[q].y += t.y;
+            }
+            if (a.end && g == 0) {
+                const double2 t = cmul(u, ce[j]);
+                ae.x += t.x;
+                ae.y += t.y;
+            }
+        }
+        if (g < ng) {
+            double2* o = a.out + ((size_t)g * a.n + r) * kNodeGroup;
+#pragma unroll
+            for (int q = 0; q < kNodeGroup; q++) o[q] = acc[q];
+        }
+        if (a.end && g == 0) a.end[r] = ae;
+    }
+}
+
+// One group of 8 nodes: for every row r, L_q = w_q conj(lambda_q(r)),
+//   Z(r) += sum_q Im(L_q psi_q(r))                       (diagonal generators)
+//   acc[label] += Im(t sum_q L_q psi_q(c))  per entry (r, c, label, t)   (off-diagonal ones)
+// with per-thread accumulators in LDS; block sums per label go to partials, and the last
+// block adds their sum (in block order) to out[label].
+struct TermArgs {
+    int64_t n;
+    const int* ptr;             // labelled off-diagonal entries (null: none)
+    const int* col;
+    const unsigned char* lab;
+    const double2* val;
+    int n_labels;
+    const double2* psi;         // node block of the group, n x 8
+    const double2* lam;
+    const double* w;            // kAdjMaxNodes weights
+    const AdjStep* step;
+    int group;
+    double* zsum;
+    double* partials;           // gridDim.x x n_labels
+    unsigned int* ticket;
+    double* out;                // n_labels running sums
+};
+
+template <int LCAP>
+__global__ __launch_bounds__(256) void termwise_kernel(TermArgs a) {
+    __shared__ double acc[LCAP * 256];
+    __shared__ double wq[kNodeGroup];
+    const int tid = threadIdx.x;
+    for (int l = 0; l < a.n_labels; l++) acc[l * 256 + tid] = 0.0;
+    if (tid < kNodeGroup) {
+        const int q = a.group * kNodeGroup + tid;
+        wq[tid] = q < a.step->n_nodes ? a.w[q] : 0.0;
+    }
+    __syncthreads();
+    for (int64_t r = (int64_t)blockIdx.x * 256 + tid; r < a.n; r += (int64_t)gridDim.x * 256) {
+        double2 L[kNodeGroup];
+        const double2* lr = a.lam + (size_t)r * kNodeGroup;
+        const double2* pr = a.psi + (size_t)r * kNodeGroup;
+        double z = 0.0;
+#pragma unroll
+        for (int q = 0; q < kNodeGroup; q++) {
+            const double2 l = lr[q];
+            L[q] = make_double2(wq[q] * l.x, -wq[q] * l.y);
+            const double2 p = pr[q];
+            z += L[q].x * p.y + L[q].y * p.x;
+        }
+        a.zsum[r] += z;
+        if (a.ptr) {
+            for (int k = a.ptr[r]; k < a.ptr[r + 1]; k++) {
+                const int c = a.col[k];
+                const double2* pc = a.psi + (size_t)c * kNodeGroup;
+                double sr = 0.0, si = 0.0;
+#pragma unroll
+                for (int q = 0; q < kNodeGroup; q++) {
+                    const double2 p = pc[q];
+                    sr += L[q].x * p.x - L[q].y * p.y;
+                    si += L[q].x * p.y + L[q].y * p.x;
+                }
+                const double2 t = a.val[k];
+                acc[a.lab[k] * 256 + tid] += t.x * si + t.y * sr;
+            }
+        }
+    }
+    __shared__ double2 sh[4];
+    for (int l = 0; l < a.n_labels; l++) {
+        const double2 s = block_sum2<256>(make_double2(acc[l * 256 + tid], 0.0), sh);
+        if (tid == 0) a.partials[(size_t)blockIdx.x * a.n_labels + l] = s.x;
+    }
+    if (last_block_arrive(a.ticket)) {
+        for (int l = 0; l < a.n_labels; l++) {
+            double t = 0.0;
+            for (int b = tid; b < (int)gridDim.x; b += 256) t += a.partials[(size_t)b * a.n_labels + l];
+            const double2 s = block_sum2<256>(make_double2(t, 0.0), sh);
+            if (tid == 0) a.out[l] += s.x;
+        }
+        if (tid == 0) *a.ticket = 0;
+    }
+}
+
+// out[p] = sum_r diag[p][r] z[r] for every generator p (its diagonal part; zero rows where it
+// has none): block partials per generator, the last block sums them in block order.
+__global__ __launch_bounds__(256) void diag_grad_kernel(const double* __restrict__ diag, const double* __restrict__ z,
+                                                        int64_t n, int P, double* partials, unsigned int* ticket,
+                                                        double* out) {
+    __shared__ double2 sh[4];
+    const int tid = threadIdx.x;
+    for (int p = 0; p < P; p++) {
+        double acc = 0.0;
+        for (int64_t r = (int64_t)blockIdx.x * 256 + tid; r < n; r += (int64_t)gridDim.x * 256)
+            acc += diag[(size_t)p * n + r] * z[r];
+        const double2 s = block_sum2<256>(make_double2(acc, 0.0), sh);
+        if (tid == 0) partials[(size_t)blockIdx.x * P + p] = s.x;
+    }
+    if (last_block_arrive(ticket)) {
+        for (int p = 0; p < P; p++) {
+            double t = 0.0;
+            for (int b = tid; b < (int)gridDim.x; b += 256) t += partials[(size_t)b * P + p];
+            const double2 s = block_sum2<256>(make_double2(t, 0.0), sh);
+            if (tid == 0) out[p] = s.x;
+        }
+        if (tid == 0) *ticket = 0;
+    }
+}
+
+static void launch_termwise(const TermArgs& ta, int grid, hipStream_t s) {
+    if (ta.n_labels <= 8) hipLaunchKernelGGL(termwise_kernel<8>, dim3(grid), dim3(256), 0, s, ta);
+    else if (ta.n_labels <= 16) hipLaunchKernelGGL(termwise_kernel<16>, dim3(grid), dim3(256), 0, s, ta);
+    else if (ta.n_labels <= 32) hipLaunchKernelGGL(termwise_kernel<32>, dim3(grid), dim3(256), 0, s, ta);
+    else hipLaunchKernelGGL(termwise_kernel<64>, dim3(grid), dim3(256), 0, s, ta);
+}
+
+}  // namespace te
+
+using namespace te;
+
+struct te_adjoint {
+    te_ctx* ctx = nullptr;
+    te_matrix* H = nullptr;
+    te_matrix* O = nullptr;
+    int P = 0;
+    te_evolve_params prm{};
+    int K = 0;
+    int64_t n = 0;
+    int m0 = 0;
+    double tolRate = 0, numErrEst = 0;
+    const int* perm = nullptr;      // H's locality order (new -> old), null for the basis order
+    const int* iperm = nullptr;
+    int grid = 8;
+    // generators in the evolver's order
+    int* gptr = nullptr;
+    int* gcol = nullptr;
+    unsigned char* glab = nullptr;
+    double2* gval = nullptr;
+    int64_t g_nnz = 0;
+    double* gdiag = nullptr;        // P x n (Re of the diagonal part)
+    double* odiag = nullptr;        // n
+    // forward
+    std::vector<double2*> bases;    // K = 0: one (m0 + 1) x n basis per step
+    double2* U = nullptr;           // K >= 1
+    std::vector<double2*> ckpt;     // K >= 1: psi(t_k), k % K == 0
+    double2* psiT = nullptr;
+    double2* scratch = nullptr;     // currentVec of the last step
+    double2* psi0 = nullptr;
+    KrylovScalars* ks = nullptr;
+    KrylovScalars* ks_hist = nullptr;
+    StepState* st = nullptr;
+    StepState* st_host = nullptr;
+    double2 *wcoef = nullptr, *Y = nullptr;
+    double* trace = nullptr;
+    double* obs = nullptr;          // 2 samples of <O>
+    DevQuad quad;
+    ReduceScratch rs;
+    std::vector<double> t_steps;
+    std::vector<int> m_steps;
+    // backward
+    double2 *Vl = nullptr, *Vp = nullptr;
+    KrylovScalars *ksl = nullptr, *ksp = nullptr;
+    double* scale = nullptr;        // [0] psi start norm, [1] lambda start norm
+    double2* gl = nullptr;
+    int* gl_off = nullptr;
+    double2 *coef0 = nullptr, *coef1 = nullptr, *cend0 = nullptr, *cend1 = nullptr;
+    double* w = nullptr;
+    AdjStep* step = nullptr;
+    AdjStep* step_host = nullptr;
+    double2 *nodes_psi = nullptr, *nodes_lam = nullptr;
+    double2 *psi_next = nullptr, *lam_next = nullptr;
+    double* zsum = nullptr;
+    double* offacc = nullptr;       // P
+    double* dgrad = nullptr;        // P
+    double* partials = nullptr;
+    unsigned int* ticket = nullptr;
+    // last run
+    te_adjoint_info info{};
+};
+
+namespace te {
+
+static void adj_free_bases(te_adjoint* a) {
+    for (double2* p : a->bases)
+        if (p) hipFree(p);
+    a->bases.clear();
+    for (double2* p : a->ckpt)
+        if (p) hipFree(p);
+    a->ckpt.clear();
+}
+
+static void adj_free(te_adjoint* a) {
+    if (!a) return;
+    hipSetDevice(a->ctx->device);
+    adj_free_bases(a);
+    void* ptrs[] = {a->gptr, a->gcol,    a->glab, a->gval, a->gdiag,  a->odiag,  a->U,      a->psiT,
+                    a->scratch, a->psi0, a->ks,   a->ks_hist, a->st,  a->wcoef, a->Y,      a->trace,
+                    a->obs,  a->Vl,      a->Vp,   a->ksl,  a->ksp,    a->scale,  a->gl,     a->gl_off,
+                    a->coef0, a->coef1,  a->cend0, a->cend1, a->w,    a->step,   a->nodes_psi, a->nodes_lam,
+                    a->psi_next, a->lam_next, a->zsum, a->offacc, a->dgrad, a->partials, a->ticket};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    if (a->st_host) hipHostFree(a->st_host);
+    if (a->step_host) hipHostFree(a->step_host);
+    free_quadrature(a->quad);
+    reduce_scratch_free(a->rs);
+    delete a;
+}
+
+template <class T>
+static int aalloc(te_ctx* ctx, T** p, size_t count) {
+    TE_HIP(ctx, hipMalloc((void**)p, sizeof(T) * std::max<size_t>(count, 1)));
+    return TE_OK;
+}
+
+// Gauss-Legendre orders 1..kGlMax on [-1, 1]: nodes ascending with their weights.
+static void gl_table(std::vector<double2>& nodes, std::vector<int>& off) {
+    off.assign(kGlMax + 2, 0);
+    for (int n = 1; n <= kGlMax; n++) {
+        off[n] = (int)nodes.size();
+        std::vector<double2> row;
+        for (int i = 1; i <= n; i++) {
+            double z = std::cos(M_PI * (i - 0.25) / (n + 0.5));
+            double dp = 1.0;
+            for (int it = 0; it < 100; it++) {
+                double p0 = 1.0, p1 = z;
+                for (int k = 2; k <= n; k++) {
+                    const double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+                    p0 = p1;
+                    p1 = p2;
+                }
+                dp = n * (z * p1 - p0) / (z * z - 1.0);
+                const double dz = p1 / dp;
+                z -= dz;
+                if (std::fabs(dz) < 1e-16) break;
+            }
+            double p0 = 1.0, p1 = z;
+            for (int k = 2; k <= n; k++) {
+                const double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            dp = n * (z * p1 - p0) / (z * z - 1.0);
+            row.push_back(make_double2(z, 2.0 / ((1.0 - z * z) * dp * dp)));
+        }
+        std::sort(row.begin(), row.end(), [](const double2& x, const double2& y) { return x.x < y.x; });
+        nodes.insert(nodes.end(), row.begin(), row.end());
+    }
+    off[kGlMax + 1] = (int)nodes.size();
+}
+
+}  // namespace te
+
+#define TE_TRY(expr)           \
+    do {                       \
+        int rc_ = (expr);      \
+        if (rc_ != TE_OK) {    \
+            te::adj_free(a);   \
+            return rc_;        \
+        }                      \
+    } while (0)
+#define TE_TRYH(call) TE_TRY((call) == hipSuccess ? TE_OK : te::fail(ctx, TE_ERR_HIP, "%s failed", #call))
+
+extern "C" int te_adjoint_create(te_ctx* ctx, te_matrix* H, int n_params, te_matrix* const* generators,
+                                 te_matrix* observable, const te_evolve_params* params, int checkpoint_interval,
+                                 te_adjoint** out) {
+    using namespace te;
+    if (!ctx || !H || n_params < 0 || (n_params > 0 && !generators) || !observable || !params || !out ||
+        checkpoint_interval < 0)
+        return TE_ERR_ARG;
+    *out = nullptr;
+    if (n_params > kAdjMaxLabels)
+        return fail(ctx, TE_ERR_UNSUPPORTED, "at most %d generators", kAdjMaxLabels);
+    if (H->n_rows != H->n_cols || H->n_rows == 0) return fail(ctx, TE_ERR_FATAL, "Invalid Hilbertspace dimension");
+    if (!observable->diagonal || observable->n_rows != H->n_rows)
+        return fail(ctx, TE_ERR_UNSUPPORTED, "the loss observable must be a diagonal matrix of H's dimension");
+    if (!(params->t > 0)) return fail(ctx, TE_ERR_ARG, "t must be > 0");
+    if (params->m < 1) return fail(ctx, TE_ERR_ARG, "Krylov dimension must be >= 1");
+    for (int p = 0; p < n_params; p++)
+        if (!generators[p] || generators[p]->n_rows != H->n_rows || generators[p]->n_cols != H->n_cols)
+            return fail(ctx, TE_ERR_ARG, "generator %d: null or wrong dimension", p);
+    hipSetDevice(ctx->device);
+    te_adjoint* a = new te_adjoint();
+    a->ctx = ctx;
+    a->H = H;
+    a->O = observable;
+    a->P = n_params;
+    a->prm = *params;
+    a->prm.sampling_step = params->t;          // samples at 0 and t only
+    a->K = checkpoint_interval;
+    const int64_t n = H->n_rows;
+    a->n = n;
+    a->m0 = (int)std::min<int64_t>(params->m, n);
+    if (a->m0 > kMaxKrylov) {
+        adj_free(a);
+        return fail(ctx, TE_ERR_UNSUPPORTED, "Krylov dimension %d exceeds %d", a->m0, kMaxKrylov);
+    }
+    double nrm1 = 0;
+    TE_TRY(te_matrix_norm1(H, &nrm1));
+    a->tolRate = params->tol / params->t;
+    a->numErrEst = (double)n * nrm1 * DBL_EPSILON;
+    a->grid = stream_grid(ctx, n);
+    if (H->use_vd && H->vd.perm) {
+        a->perm = H->vd.perm;
+        a->iperm = H->vd.iperm;
+    }
+    hipStream_t s = ctx->stream;
+
+    // generators: off-diagonal entries labelled by generator, diagonal parts as dense rows,
+    // all in the evolver's order (row r of the copy is row perm[r] of the basis order)
+    std::vector<int> hperm(n), hiperm(n);
+    if (a->perm) {
+        TE_TRYH(hipMemcpy(hperm.data(), a->perm, sizeof(int) * n, hipMemcpyDeviceToHost));
+        TE_TRYH(hipMemcpy(hiperm.data(), a->iperm, sizeof(int) * n, hipMemcpyDeviceToHost));
+    } else {
+        for (int64_t i = 0; i < n; i++) hperm[i] = hiperm[i] = (int)i;
+    }
+    std::vector<std::vector<int64_t>> gp(n_params);
+    std::vector<std::vector<int>> gc(n_params);
+    std::vector<std::vector<double2>> gv(n_params);
+    std::vector<double> hdiag((size_t)n_params * n, 0.0);
+    int64_t off_nnz = 0;
+    for (int p = 0; p < n_params; p++) {
+        te_matrix* G = generators[p];
+        gp[p].resize(n + 1);
+        gc[p].resize(G->nnz);
+        gv[p].resize(G->nnz);
+        TE_TRY(te_matrix_to_csr(G, (double*)gv[p].data(), gp[p].data(), gc[p].data()));
+        for (int64_t r = 0; r < n; r++)
+            for (int64_t k = gp[p][r]; k < gp[p][r + 1]; k++) {
+                if (gc[p][k] == r) {
+                    hdiag[(size_t)p * n + hiperm[r]] += gv[p][k].x;
+                } else {
+                    off_nnz++;
+                }
+            }
+    }
+    if (off_nnz >= ((int64_t)1 << 31)) {
+        adj_free(a);
+        return fail(ctx, TE_ERR_UNSUPPORTED, "too many generator entries");
+    }
+    a->g_nnz = off_nnz;
+    if (off_nnz > 0) {
+        std::vector<int> hptr(n + 1, 0), hcol(off_nnz);
+        std::vector<unsigned char> hlab(off_nnz);
+        std::vector<double2> hval(off_nnz);
+        int64_t k = 0;
+        for (int64_t r = 0; r < n; r++) {              // row r of the evolver order
+            hptr[r] = (int)k;
+            const int ro = hperm[r];
+            for (int p = 0; p < n_params; p++)
+                for (int64_t e = gp[p][ro]; e < gp[p][ro + 1]; e++)
+                    if (gc[p][e] != ro) {
+                        hcol[k] = hiperm[gc[p][e]];
+                        hlab[k] = (unsigned char)p;
+                        hval[k] = gv[p][e];
+                        k++;
+                    }
+        }
+        hptr[n] = (int)k;
+        TE_TRY(aalloc(ctx, &a->gptr, n + 1));
+        TE_TRY(aalloc(ctx, &a->gcol, off_nnz));
+        TE_TRY(aalloc(ctx, &a->glab, off_nnz));
+        TE_TRY(aalloc(ctx, &a->gval, off_nnz));
+        TE_TRYH(hipMemcpy(a->gptr, hptr.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+        TE_TRYH(hipMemcpy(a->gcol, hcol.data(), sizeof(int) * off_nnz, hipMemcpyHostToDevice));
+        TE_TRYH(hipMemcpy(a->glab, hlab.data(), off_nnz, hipMemcpyHostToDevice));
+        TE_TRYH(hipMemcpy(a->gval, hval.data(), sizeof(double2) * off_nnz, hipMemcpyHostToDevice));
+    }
+    TE_TRY(aalloc(ctx, &a->gdiag, (size_t)std::max(n_params, 1) * n));
+    if (n_params > 0)
+        TE_TRYH(hipMemcpy(a->gdiag, hdiag.data(), sizeof(double) * n_params * n, hipMemcpyHostToDevice));
+    TE_TRY(aalloc(ctx, &a->odiag, n));
+    if (a->perm) {
+        launch_gather_re(observable->diag_re, a->perm, a->odiag, n, s);
+    } else {
+        TE_TRYH(hipMemcpyAsync(a->odiag, observable->diag_re, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+    }
+
+    // forward and backward state
+    TE_TRY(aalloc(ctx, &a->psi0, n));
+    TE_TRY(aalloc(ctx, &a->psiT, n));
+    TE_TRY(aalloc(ctx, &a->scratch, n));
+    if (a->K >= 1) TE_TRY(aalloc(ctx, &a->U, (size_t)n * (a->m0 + 1)));
+    TE_TRY(aalloc(ctx, &a->ks, 1));
+    TE_TRY(aalloc(ctx, &a->ks_hist, kAdjMaxSteps));
+    TE_TRY(aalloc(ctx, &a->st, 1));
+    TE_TRYH(hipHostMalloc((void**)&a->st_host, sizeof(StepState)));
+    TE_TRY(aalloc(ctx, &a->wcoef, kMaxKrylov));
+    TE_TRY(aalloc(ctx, &a->Y, 2 * kMaxKrylov));
+    TE_TRY(aalloc(ctx, &a->trace, kAdjMaxSteps));
+    TE_TRY(aalloc(ctx, &a->obs, 2));
+    TE_TRY(upload_quadrature(ctx, a->quad));
+    TE_TRY(reduce_scratch_alloc(ctx, a->rs, std::max({a->grid, H->shape.grid, H->vd.grid, H->vd.sell_grid})));
+    TE_TRY(aalloc(ctx, &a->Vl, (size_t)n * (a->m0 + 1)));
+    if (a->K >= 1) TE_TRY(aalloc(ctx, &a->Vp, (size_t)n * (a->m0 + 1)));
+    TE_TRY(aalloc(ctx, &a->ksl, 1));
+    TE_TRY(aalloc(ctx, &a->ksp, 1));
+    TE_TRY(aalloc(ctx, &a->scale, 2));
+    std::vector<double2> gln;
+    std::vector<int> glo;
+    gl_table(gln, glo);
+    TE_TRY(aalloc(ctx, &a->gl, gln.size()));
+    TE_TRY(aalloc(ctx, &a->gl_off, glo.size()));
+    TE_TRYH(hipMemcpy(a->gl, gln.data(), sizeof(double2) * gln.size(), hipMemcpyHostToDevice));
+    TE_TRYH(hipMemcpy(a->gl_off, glo.data(), sizeof(int) * glo.size(), hipMemcpyHostToDevice));
+    TE_TRY(aalloc(ctx, &a->coef0, (size_t)kAdjMaxNodes * kMaxKrylov));
+    TE_TRY(aalloc(ctx, &a->coef1, (size_t)kAdjMaxNodes * kMaxKrylov));
+    TE_TRY(aalloc(ctx, &a->cend0, kMaxKrylov));
+    TE_TRY(aalloc(ctx, &a->cend1, kMaxKrylov));
+    TE_TRY(aalloc(ctx, &a->w, kAdjMaxNodes));
+    TE_TRY(aalloc(ctx, &a->step, 1));
+    TE_TRYH(hipHostMalloc((void**)&a->step_host, sizeof(AdjStep)));
+    TE_TRY(aalloc(ctx, &a->nodes_psi, (size_t)kMatNodes * n));
+    TE_TRY(aalloc(ctx, &a->nodes_lam, (size_t)kMatNodes * n));
+    TE_TRY(aalloc(ctx, &a->psi_next, n));
+    TE_TRY(aalloc(ctx, &a->lam_next, n));
+    TE_TRY(aalloc(ctx, &a->zsum, n));
+    TE_TRY(aalloc(ctx, &a->offacc, kAdjMaxLabels));
+    TE_TRY(aalloc(ctx, &a->dgrad, kAdjMaxLabels));
+    TE_TRY(aalloc(ctx, &a->partials, (size_t)a->grid * kAdjMaxLabels));
+    TE_TRY(aalloc(ctx, &a->ticket, 1));
+    TE_TRYH(hipMemsetAsync(a->ticket, 0, sizeof(unsigned int), s));
+    TE_TRYH(hipStreamSynchronize(s));
+    *out = a;
+    return TE_OK;
+}
+
+extern "C" void te_adjoint_destroy(te_adjoint* a) { te::adj_free(a); }
+
+#undef TE_TRY
+#undef TE_TRYH
+
+namespace te {
+
+// One Lanczos recurrence of m iterations on U from U[:,0] (normalised by s_0 in ks).
+static void lanczos(te_adjoint* a, double2* U, KrylovScalars* ks, int m, double2 alpha) {
+    LanczosArgs la{};
+    la.n = a->n;
+    la.row_block = a->H->row_block;
+    la.n_row_blocks = a->H->n_row_blocks;
+    la.ptr = a->H->row_ptr;
+    la.col = a->H->col;
+    la.val = a->H->val;
+    la.U = U;
+    la.ld = a->n;
+    la.m = m;
+    la.alpha = alpha;
+    la.tolRate = a->tolRate;
+    la.ks = ks;
+    la.partials = a->rs.partials;
+    la.ticket = a->rs.ticket;
+    for (int j = 0; j < m; j++) {
+        la.j = j;
+        launch_lanczos_apply(la, a->H, a->ctx->stream);
+        launch_lanczos_project(la, a->grid, a->ctx->stream);
+    }
+}
+
+// Normalises the Lanczos start U[:,0]: ks->s[0] = 1/|U[:,0]|, *scale = |U[:,0]|.
+static void start_norm(te_adjoint* a, const double2* v, KrylovScalars* ks, double* scale) {
+    launch_nrm2sq(v, a->n, a->rs, 0, a->ctx->stream);
+    hipLaunchKernelGGL(start_norm_kernel, dim3(1), dim3(1), 0, a->ctx->stream, a->rs.result, ks, scale);
+}
+
+static int forward(te_adjoint* a, te_result* R) {
+    te_ctx* ctx = a->ctx;
+    hipStream_t s = ctx->stream;
+    const int64_t n = a->n;
+    const double2 alpha = make_double2(0.0, -1.0 * a->prm.exp_factor);
+    StepState init{};
+    init.t_step = 1.0 / 10.0;
+    init.m = a->m0;
+    init.index_samples = 1;
+    *a->st_host = init;
+    KrylovScalars ks0{};
+    ks0.s[0] = 1.0;
+    std::vector<double2> e1(kMaxKrylov, make_double2(0.0, 0.0));
+    e1[0] = make_double2(1.0, 0.0);
+    TE_HIP(ctx, hipMemcpyAsync(a->ks, &ks0, sizeof(ks0), hipMemcpyHostToDevice, s));
+    TE_HIP(ctx, hipMemcpyAsync(a->st, a->st_host, sizeof(StepState), hipMemcpyHostToDevice, s));
+    TE_HIP(ctx, hipMemcpyAsync(a->Y, e1.data(), sizeof(double2) * kMaxKrylov, hipMemcpyHostToDevice, s));
+    const long long n_samples = 2;
+    StepConsts C = step_consts(a->prm, a->tolRate, a->numErrEst, n_samples, a->quad);
+    C.Y = a->Y;
+    C.wcoef = a->wcoef;
+    C.trace = a->trace;
+    C.trace_cap = kAdjMaxSteps;
+    a->t_steps.clear();
+    a->m_steps.clear();
+    for (double2* p : a->ckpt) hipFree(p);
+    a->ckpt.clear();
+    auto basis = [&](size_t k) -> double2* {
+        if (a->K >= 1) return a->U;
+        while (a->bases.size() <= k) {
+            double2* p = nullptr;
+            if (hipMalloc(&p, sizeof(double2) * (size_t)n * (a->m0 + 1)) != hipSuccess) return nullptr;
+            a->bases.push_back(p);
+        }
+        return a->bases[k];
+    };
+    double2* U = basis(0);
+    if (!U) return fail(ctx, TE_ERR_ALLOC, "no memory for the forward Krylov bases");
+    TE_HIP(ctx, hipMemcpyAsync(U, a->psi0, sizeof(double2) * n, hipMemcpyDeviceToDevice, s));
+    int m = a->m0;
+    StepState& h = *a->st_host;
+    int stalled = 0;
+    CombineArgs cb{};
+    cb.n = n;
+    cb.ld = n;
+    cb.wcoef = a->wcoef;
+    cb.Y = a->Y;
+    cb.diag = a->odiag;
+    cb.nobs = 1;
+    cb.obs_base = 0;
+    cb.obs_values = a->obs;
+    cb.n_samples = n_samples;
+    cb.partials = a->rs.partials;
+    cb.ticket = a->rs.ticket;
+    // sample 0: <O> of psi0
+    {
+        CombineArgs c0 = cb;
+        c0.U = U;
+        c0.m = 1;
+        c0.U0 = U;
+        c0.write_w = 0;
+        c0.y_base = 0;
+        c0.S = 1;
+        c0.write_last = 0;
+        launch_combine(c0, a->grid, s);
+    }
+    size_t k = 0;
+    while (h.index_samples < n_samples) {
+        if (k >= (size_t)kAdjMaxSteps) return fail(ctx, TE_ERR_UNSUPPORTED, "more than %d Krylov steps", kAdjMaxSteps);
+        if (a->K >= 1 && k % a->K == 0) {                 // checkpoint psi(t_k)
+            double2* c = nullptr;
+            TE_HIP(ctx, hipMalloc(&c, sizeof(double2) * n));
+            a->ckpt.push_back(c);
+            TE_HIP(ctx, hipMemcpyAsync(c, U, sizeof(double2) * n, hipMemcpyDeviceToDevice, s));
+        }
+        lanczos(a, U, a->ks, m, alpha);
+        launch_step_control(C, a->st, a->ks, s);
+        TE_HIP(ctx, hipGetLastError());
+        TE_HIP(ctx, hipMemcpyAsync(a->st_host, a->st, sizeof(StepState), hipMemcpyDeviceToHost, s));
+        TE_HIP(ctx, hipMemcpyAsync(a->ks_hist + k, a->ks, sizeof(KrylovScalars), hipMemcpyDeviceToDevice, s));
+        TE_HIP(ctx, hipStreamSynchronize(s));
+        if (h.fatal == 1)
+            return fail(ctx, TE_ERR_FATAL, "Error: No small enough time step found to meet tolerance requirements.");
+        if (h.fatal == 2) return fail(ctx, TE_ERR_FATAL, "Internal error: eigen-solver did not converge");
+        stalled = (h.t_step == 0.0 && h.S == 0) ? stalled + 1 : 0;
+        if (stalled >= 2) return fail(ctx, TE_ERR_FATAL, "time evolution makes no progress (t_step = 0)");
+        m = h.m;
+        a->t_steps.push_back(h.t_step);
+        a->m_steps.push_back(h.m);
+        const bool last = h.index_samples >= n_samples;
+        double2* Unext = last ? a->scratch : basis(k + 1);
+        if (!Unext) return fail(ctx, TE_ERR_ALLOC, "no memory for the forward Krylov bases");
+        CombineArgs c1 = cb;
+        c1.U = U;
+        c1.m = m;
+        c1.U0 = Unext;                 // currentVec -> column 0 of the next step's basis
+        c1.write_w = h.write_w;
+        c1.y_base = h.sample_base;
+        c1.S = h.S;
+        c1.write_last = h.S > 0 ? 1 : 0;
+        c1.last_out = a->psiT;
+        launch_combine(c1, a->grid, s);
+        if (!h.write_w) {             // breakdown step: currentVec is the sampled state
+            TE_HIP(ctx, hipMemcpyAsync(Unext, a->psiT, sizeof(double2) * n, hipMemcpyDeviceToDevice, s));
+        }
+        TE_HIP(ctx, hipGetLastError());
+        U = Unext;
+        k++;
+    }
+    // result and status (:445-504)
+    launch_nrm2sq(a->psiT, n, a->rs, 0, s);
+    double2 r2;
+    TE_HIP(ctx, hipMemcpyAsync(&r2, a->rs.result, sizeof(double2), hipMemcpyDeviceToHost, s));
+    TE_HIP(ctx, hipStreamSynchronize(s));
+    R->err = h.err;
+    R->evolved_time = h.t_now;
+    R->num_samples = (size_t)h.index_samples;
+    R->n_steps = (size_t)h.n_steps;
+    R->dim = (size_t)n;
+    R->krylov_dim = (size_t)h.m;
+    R->status_code = final_status(h, a->numErrEst, a->prm.tol, std::sqrt(r2.x));
+    return TE_OK;
+}
+
+static int backward(te_adjoint* a) {
+    te_ctx* ctx = a->ctx;
+    hipStream_t s = ctx->stream;
+    const int64_t n = a->n;
+    const double2 alpha_b = make_double2(0.0, 1.0 * a->prm.exp_factor);   // +i e H: evolution back in time
+    const int nsteps = (int)a->t_steps.size();
+    const int eg = std::max(8, std::min<int>((int)((n + 255) / 256), ctx->num_cus * 4)) / 8 * 8;
+    hipLaunchKernelGGL(diag_mul_kernel, dim3(eg), dim3(256), 0, s, a->odiag, a->psiT, a->Vl, n);   // lambda(T)
+    if (a->K >= 1)
+        TE_HIP(ctx, hipMemcpyAsync(a->Vp, a->psiT, sizeof(double2) * n, hipMemcpyDeviceToDevice, s));
+    TE_HIP(ctx, hipMemsetAsync(a->zsum, 0, sizeof(double) * n, s));
+    TE_HIP(ctx, hipMemsetAsync(a->offacc, 0, sizeof(double) * kAdjMaxLabels, s));
+    a->info.backward_err = 0;
+    a->info.psi_err = 0;
+    a->info.nodes = 0;
+    a->info.max_phase = 0;
+    a->info.capped = 0;
+    const int mg = std::max(8, std::min<int>((int)((n + 31) / 32), ctx->num_cus * 4)) / 8 * 8;
+    const int tg = a->grid;
+    for (int k = nsteps - 1; k >= 0; k--) {
+        const double hstep = a->t_steps[k];
+        // lambda: Lanczos of +i e H from lambda(t_{k+1})
+        start_norm(a, a->Vl, a->ksl, a->scale + 1);
+        lanczos(a, a->Vl, a->ksl, a->m0, alpha_b);
+        AdjCoefArgs ca{};
+        ca.ks[1] = a->ksl;
+        ca.m[1] = a->m0;          // replaced by m_hbd below when the recurrence broke down
+        ca.from_end[1] = 1;
+        ca.scale[1] = a->scale + 1;
+        const double2* Upsi;
+        if (a->K == 0) {
+            Upsi = a->bases[k];
+            ca.ks[0] = a->ks_hist + k;
+            ca.m[0] = a->m_steps[k];
+            ca.from_end[0] = 0;
+            ca.scale[0] = nullptr;
+        } else {
+            // psi(t_{k+1}): the forward checkpoint when one is kept there, else the state
+            // back-evolved by the previous backward step
+            if (k + 1 < nsteps && (k + 1) % a->K == 0)
+                TE_HIP(ctx, hipMemcpyAsync(a->Vp, a->ckpt[(k + 1) / a->K], sizeof(double2) * n,
+                                           hipMemcpyDeviceToDevice, s));
+            start_norm(a, a->Vp, a->ksp, a->scale);
+            lanczos(a, a->Vp, a->ksp, a->m0, alpha_b);
+            Upsi = a->Vp;
+            ca.ks[0] = a->ksp;
+            ca.m[0] = a->m0;
+            ca.from_end[0] = 1;
+            ca.scale[0] = a->scale;
+        }
+        // Krylov dimensions after a breakdown (read back with the step record)
+        KrylovScalars kl{};
+        TE_HIP(ctx, hipMemcpyAsync(&kl, a->ksl, sizeof(KrylovScalars), hipMemcpyDeviceToHost, s));
+        KrylovScalars kp{};
+        if (a->K >= 1) TE_HIP(ctx, hipMemcpyAsync(&kp, a->ksp, sizeof(KrylovScalars), hipMemcpyDeviceToHost, s));
+        TE_HIP(ctx, hipStreamSynchronize(s));
+        if (kl.broken) ca.m[1] = kl.m_hbd;
+        if (a->K >= 1 && kp.broken) ca.m[0] = kp.m_hbd;
+        ca.h = hstep;
+        ca.gl = a->gl;
+        ca.gl_off = a->gl_off;
+        ca.coef[0] = a->coef0;
+        ca.coef[1] = a->coef1;
+        ca.coef_end[0] = a->cend0;
+        ca.coef_end[1] = a->cend1;
+        ca.w = a->w;
+        ca.out = a->step;
+        launch_adjoint_coef(ca, s);
+        TE_HIP(ctx, hipMemcpyAsync(a->step_host, a->step, sizeof(AdjStep), hipMemcpyDeviceToHost, s));
+        TE_HIP(ctx, hipStreamSynchronize(s));
+        const AdjStep st = *a->step_host;
+        a->info.nodes += st.n_nodes;
+        a->info.max_phase = std::max(a->info.max_phase, st.phase);
+        a->info.backward_err += st.err[1];
+        if (a->K >= 1) a->info.psi_err += st.err[0];
+        if (st.capped) a->info.capped++;
+        if (st.capped == 2) return fail(ctx, TE_ERR_FATAL, "Internal error: eigen-solver did not converge");
+        // node states, 64 per pass, and the states at t_k
+        for (int node0 = 0; node0 < st.n_nodes; node0 += kMatNodes) {
+            MatArgs mp{Upsi, n, n, ca.m[0], a->coef0, a->cend0, a->step, node0, a->nodes_psi,
+                       (node0 == 0 && a->K >= 1) ? a->psi_next : nullptr};
+            MatArgs ml{a->Vl, n, n, ca.m[1], a->coef1, a->cend1, a->step, node0, a->nodes_lam,
+                       node0 == 0 ? a->lam_next : nullptr};
+            hipLaunchKernelGGL(materialise_nodes_kernel, dim3(mg), dim3(256), 0, s, mp);
+            hipLaunchKernelGGL(materialise_nodes_kernel, dim3(mg), dim3(256), 0, s, ml);
+            const int ng = (std::min(kMatNodes, st.n_nodes - node0) + kNodeGroup - 1) / kNodeGroup;
+            for (int g = 0; g < ng; g++) {
+                TermArgs ta{};
+                ta.n = n;
+                ta.ptr = a->g_nnz > 0 ? a->gptr : nullptr;
+                ta.col = a->gcol;
+                ta.lab = a->glab;
+                ta.val = a->gval;
+                ta.n_labels = a->P;
+                ta.psi = a->nodes_psi + (size_t)g * n * kNodeGroup;
+                ta.lam = a->nodes_lam + (size_t)g * n * kNodeGroup;
+                ta.w = a->w;
+                ta.step = a->step;
+                ta.group = node0 / kNodeGroup + g;
+                ta.zsum = a->zsum;
+                ta.partials = a->partials;
+                ta.ticket = a->ticket;
+                ta.out = a->offacc;
+                launch_termwise(ta, tg, s);
+            }
+        }
+        TE_HIP(ctx, hipGetLastError());
+        // the next (earlier) step starts from the states at t_k
+        TE_HIP(ctx, hipMemcpyAsync(a->Vl, a->lam_next, sizeof(double2) * n, hipMemcpyDeviceToDevice, s));
+        if (a->K >= 1)
+            TE_HIP(ctx, hipMemcpyAsync(a->Vp, a->psi_next, sizeof(double2) * n, hipMemcpyDeviceToDevice, s));
+    }
+    if (a->P > 0)
+        hipLaunchKernelGGL(diag_grad_kernel, dim3(tg), dim3(256), 0, s, a->gdiag, a->zsum, n, a->P, a->partials,
+                           a->ticket, a->dgrad);
+    TE_HIP(ctx, hipGetLastError());
+    return TE_OK;
+}
+
+}  // namespace te
+
+extern "C" int te_adjoint_run(te_adjoint* a, const double* psi0, double* loss, double* grad, te_result* result,
+                              te_adjoint_info* info) {
+    using namespace te;
+    if (!a || !psi0 || !loss || (a->P > 0 && !grad)) return TE_ERR_ARG;
+    te_ctx* ctx = a->ctx;
+    hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    const int64_t n = a->n;
+    TE_HIP(ctx, hipMemcpyAsync(a->scratch, psi0, sizeof(double2) * n, hipMemcpyHostToDevice, s));
+    if (a->perm) launch_gather(a->scratch, a->perm, a->psi0, n, s);
+    else TE_HIP(ctx, hipMemcpyAsync(a->psi0, a->scratch, sizeof(double2) * n, hipMemcpyDeviceToDevice, s));
+    launch_nrm2sq(a->psi0, n, a->rs, 0, s);
+    double2 r;
+    TE_HIP(ctx, hipMemcpyAsync(&r, a->rs.result, sizeof(double2), hipMemcpyDeviceToHost, s));
+    TE_HIP(ctx, hipStreamSynchronize(s));
+    if (std::fabs(std::sqrt(r.x) - 1.0) > a->prm.tol) return fail(ctx, TE_ERR_FATAL, "Initial vector is not normalized");
+    te_result R{};
+    hipEvent_t e0, e1, e2;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipEventCreate(&e2);
+    hipEventRecord(e0, s);
+    int rc = forward(a, &R);
+    hipEventRecord(e1, s);
+    if (rc == TE_OK) rc = backward(a);
+    hipEventRecord(e2, s);
+    double hl[2] = {0, 0};
+    std::vector<double> off(kAdjMaxLabels), dg(kAdjMaxLabels);
+    if (rc == TE_OK) {
+        TE_HIP(ctx, hipMemcpyAsync(hl, a->obs, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
+        TE_HIP(ctx, hipMemcpyAsync(off.data(), a->offacc, sizeof(double) * kAdjMaxLabels, hipMemcpyDeviceToHost, s));
+        TE_HIP(ctx, hipMemcpyAsync(dg.data(), a->dgrad, sizeof(double) * kAdjMaxLabels, hipMemcpyDeviceToHost, s));
+    }
+    TE_HIP(ctx, hipStreamSynchronize(s));
+    float f_ms = 0, b_ms = 0;
+    hipEventElapsedTime(&f_ms, e0, e1);
+    hipEventElapsedTime(&b_ms, e1, e2);
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipEventDestroy(e2);
+    if (rc != TE_OK) return rc;
+    *loss = hl[1];
+    for (int p = 0; p < a->P; p++) grad[p] = 2.0 * a->prm.exp_factor * (off[p] + (a->P > 0 ? dg[p] : 0.0));
+    a->info.forward_ms = f_ms;
+    a->info.backward_ms = b_ms;
+    a->info.n_steps = (int)a->t_steps.size();
+    if (result) *result = R;
+    if (info) *info = a->info;
+    return TE_OK;
+}
+
+extern "C" int te_adjoint_set_hamiltonian(te_adjoint* a, te_matrix* H) {
+    using namespace te;
+    if (!a || !H) return TE_ERR_ARG;
+    te_ctx* ctx = a->ctx;
+    if (H->n_rows != a->n || H->n_cols != a->n) return fail(ctx, TE_ERR_ARG, "Hamiltonian of another dimension");
+    const int* perm = (H->use_vd && H->vd.perm) ? H->vd.perm : nullptr;
+    if ((perm == nullptr) != (a->perm == nullptr)) return fail(ctx, TE_ERR_ARG, "Hamiltonian held in another order");
+    if (perm) {
+        std::vector<int> p0(a->n), p1(a->n);
+        TE_HIP(ctx, hipMemcpy(p0.data(), a->perm, sizeof(int) * a->n, hipMemcpyDeviceToHost));
+        TE_HIP(ctx, hipMemcpy(p1.data(), perm, sizeof(int) * a->n, hipMemcpyDeviceToHost));
+        if (p0 != p1) return fail(ctx, TE_ERR_ARG, "Hamiltonian held in another order");
+    }
+    double nrm1 = 0;
+    const int rc = te_matrix_norm1(H, &nrm1);
+    if (rc != TE_OK) return rc;
+    if (std::max({H->shape.grid, H->vd.grid, H->vd.sell_grid}) > a->rs.grid)
+        return fail(ctx, TE_ERR_ARG, "Hamiltonian needs a larger reduction grid");
+    a->H = H;
+    a->perm = perm;
+    a->iperm = perm ? H->vd.iperm : nullptr;
+    a->numErrEst = (double)a->n * nrm1 * DBL_EPSILON;
+    return TE_OK;
+}
+
+extern "C" int te_adjoint_state(te_adjoint* a, double* psiT) {
+    using namespace te;
+    if (!a || !psiT) return TE_ERR_ARG;
+    te_ctx* ctx = a->ctx;
+    hipSetDevice(ctx->device);
+    hipStream_t s = ctx->stream;
+    const double2* src = a->psiT;
+    if (a->perm) {
+        launch_gather(a->psiT, a->iperm, a->scratch, a->n, s);
+        src = a->scratch;
+    }
+    TE_HIP(ctx, hipMemcpyAsync(psiT, src, sizeof(double2) * a->n, hipMemcpyDeviceToHost, s));
+    TE_HIP(ctx, hipStreamSynchronize(s));
+    return TE_OK;
+}

@@ -41,7 +41,7 @@ int fail(te_ctx* ctx, int code, const char* fmt, ...) {
     return code;
 }
 
-static int grid_for(const te_ctx* ctx, int64_t work_items) {
+int stream_grid(const te_ctx* ctx, int64_t work_items) {
     static const int per_cu = [] {
         const char* v = std::getenv("TE_STREAM_BLOCKS_PER_CU");      // launch-shape sweeps
         return v ? std::max(1, std::atoi(v)) : 4;
@@ -58,7 +58,7 @@ SpmvShape choose_shape(const te_ctx* ctx, int64_t n_rows, int64_t nnz) {
     int lpr = 1;
     while (lpr < 32 && lpr < mean) lpr <<= 1;
     s.lanes_per_row = lpr;
-    s.grid = grid_for(ctx, n_rows * lpr);
+    s.grid = stream_grid(ctx, n_rows * lpr);
     return s;
 }
 
@@ -609,7 +609,7 @@ static int evolver_create(te_ctx* ctx, te_matrix* H, const void* psi0, bool psi0
     ev->n_samples = (long long)std::floor(params->t / params->sampling_step) + 1;          // :57
     ev->tolRate = params->tol / params->t;                                                // :287
     ev->numErrEst = (double)n * nrm1 * DBL_EPSILON;                                       // :289
-    ev->grid_stream = grid_for(ctx, n);
+    ev->grid_stream = stream_grid(ctx, n);
 
     TE_TRY(dmalloc(ctx, &ev->psi0, n));
     TE_TRY(dmalloc(ctx, &ev->U, (size_t)n * (m + 1)));

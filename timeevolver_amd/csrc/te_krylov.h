@@ -114,6 +114,35 @@ StepConsts step_consts(const te_evolve_params& P, double tolRate, double numErrE
 // check of the returned state, then the error counters; logs the reference's warnings.
 int final_status(const StepState& h, double numErrEst, double tol, double state_norm);
 
+// ---- adjoint: Krylov coefficients of one backward step ----
+constexpr int kGlMax = 48;          // largest Gauss-Legendre order in the node table
+constexpr int kAdjMaxNodes = 256;   // quadrature nodes per Krylov step
+
+// What the host reads back after each backward step.
+struct AdjStep {
+    int n_nodes;         // quadrature nodes of the step (order x sub-intervals)
+    int order, n_sub;
+    int capped;          // the phase needed more than kAdjMaxNodes nodes (accuracy not assured)
+    double phase;        // |omega|_max h: largest phase of the integrand over the step
+    double err[2];       // a-posteriori Krylov error integral of each basis over the step
+};
+
+struct AdjCoefArgs {
+    const KrylovScalars* ks[2];     // basis 0: psi, basis 1: lambda
+    int m[2];                       // Krylov dimension (m_hbd after a breakdown)
+    int from_end[2];                // 0: the basis starts at the step start t_k (coefficients
+                                    //    at x = s - t_k); 1: it starts at t_{k+1} (at h - x)
+    const double* scale[2];         // norm of the basis' start vector (null: 1)
+    double h;                       // step length
+    const double2* gl;              // Gauss-Legendre (node, weight) on [-1, 1], order n at gl_off[n]
+    const int* gl_off;
+    double2* coef[2];               // kAdjMaxNodes x kMaxKrylov coefficients per basis
+    double2* coef_end[2];           // kMaxKrylov: the basis vector combination at t_k
+    double* w;                      // kAdjMaxNodes quadrature weights (on [t_k, t_{k+1}])
+    AdjStep* out;
+};
+void launch_adjoint_coef(const AdjCoefArgs& a, hipStream_t s);
+
 // Per-instance control flags of the batched driver, updated by the batched step controller.
 struct BatchCtl {
     int live;       // the instance still has samples to take and no fatal error

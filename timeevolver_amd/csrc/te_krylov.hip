@@ -701,6 +701,184 @@ __global__ __launch_bounds__(kCtl) void batch_step_control_kernel(const StepCons
 }
 
 // ---------------------------------------------------------------------------------------
+// Adjoint: both Krylov spaces of a backward step decomposed at once (wavefront 0: basis 0,
+// wavefront 1: basis 1), then the quadrature rule and every node's coefficient vectors.
+//
+// With sigma = -1 for a basis that starts at t_k and +1 for one that starts at t_{k+1}, basis
+// b's vector at s = t_k + x varies as exp(i x sigma_b theta) over its Ritz values theta, so the
+// integrand <lambda|G|psi> holds the frequencies sigma_0 theta_0 - sigma_1 theta_1. The
+// Gauss-Legendre rule is chosen for the largest of them: the error bound of an n-point rule on
+// exp(i omega x) over a length-H interval, relative to H, is
+//   (omega H)^{2n} (n!)^4 / ((2n+1) ((2n)!)^3),
+// and the rule is the fewest nodes (order n <= kGlMax times equal sub-intervals) bringing it
+// below 1e-13.
+// ---------------------------------------------------------------------------------------
+struct AdjShared {
+    double Q[2][kMaxKrylov * kMaxKrylov];
+    double lam[2][kMaxKrylov], d[2][kMaxKrylov], e[2][kMaxKrylov], g[2][kMaxKrylov], s[2][kMaxKrylov];
+    double p[2][kMaxKrylov];
+    double lo[2], hi[2];
+    int info[2];
+    int order, n_sub;
+    double phase;
+};
+
+// out[j] = scale s_j [exp(x H_m) e1]_j from the eigen-decomposition of basis b, computed by the
+// 64 lanes of the calling wavefront (lane k forms the phase term of Ritz value k, shuffles carry
+// it to the lanes that sum row j).
+__device__ void adj_coef_wave(const AdjShared& S, int b, int m, double x, double scale, double2* out) {
+    const int lane = threadIdx.x & 63;
+    double phr = 0.0, phi = 0.0;
+    if (lane < m) {
+        double sn, cs;
+        sincos(x * S.lam[b][lane], &sn, &cs);
+        phr = S.g[b][lane] * cs;
+        phi = -S.g[b][lane] * sn;
+    }
+    double re = 0.0, im = 0.0;
+    for (int k = 0; k < m; k++) {
+        const double pr = __shfl(phr, k, 64), pi = __shfl(phi, k, 64);
+        if (lane < m) {
+            const double q = S.Q[b][k * 64 + lane];
+            re += q * pr;
+            im += q * pi;
+        }
+    }
+    if (lane < m) {
+        double2 v;
+        switch (lane & 3) {                    // multiply by i^j exactly
+            case 0: v = make_double2(re, im); break;
+            case 1: v = make_double2(-im, re); break;
+            case 2: v = make_double2(-re, -im); break;
+            default: v = make_double2(im, -re); break;
+        }
+        const double f = S.s[b][lane] * scale;
+        out[lane] = make_double2(v.x * f, v.y * f);
+    }
+}
+
+__device__ double gl_log_bound(int n, double phase) {
+    if (phase <= 0.0) return -1e300;
+    return 2.0 * n * log(phase) + 4.0 * lgamma(n + 1.0) - log(2.0 * n + 1.0) - 3.0 * lgamma(2.0 * n + 1.0);
+}
+
+__global__ __launch_bounds__(256) void adjoint_coef_kernel(AdjCoefArgs a) {
+    __shared__ AdjShared S;
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    if (wv < 2 && lane < a.m[wv]) {
+        const int b = wv, m = a.m[b];
+        const KrylovScalars* ks = a.ks[b];
+        S.d[b][lane] = -ks->a[lane].y;
+        S.e[b][lane] = (lane + 1 < m) ? ks->beta[lane] : 0.0;
+        S.s[b][lane] = ks->s[lane];
+    }
+    __syncthreads();
+    if (wv < 2) {                     // each wavefront decomposes one basis (no barrier inside)
+        const int info = wave_tridiag_ql(a.m[wv], S.d[wv], S.e[wv], S.Q[wv]);
+        if (lane == 0) S.info[wv] = info;
+    }
+    __syncthreads();
+    if (wv < 2 && lane < a.m[wv]) {
+        const int b = wv, m = a.m[b];
+        S.lam[b][lane] = S.d[b][lane];
+        S.g[b][lane] = S.Q[b][lane * 64];
+        S.p[b][lane] = S.Q[b][lane * 64 + m - 1] * S.Q[b][lane * 64];
+    }
+    __syncthreads();
+    if (tid < 2) {
+        const int b = tid;
+        double lo = 1e300, hi = -1e300;
+        for (int k = 0; k < a.m[b]; k++) {
+            const double v = (a.from_end[b] ? 1.0 : -1.0) * S.lam[b][k];
+            lo = fmin(lo, v);
+            hi = fmax(hi, v);
+        }
+        S.lo[b] = lo;
+        S.hi[b] = hi;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const double om = fmax(fabs(S.hi[0] - S.lo[1]), fabs(S.lo[0] - S.hi[1]));
+        const double phase = om * a.h;
+        const double lim = log(1e-13);
+        int best_n = kGlMax, best_sub = kAdjMaxNodes / kGlMax, capped = 1;
+        for (int sub = 1; sub * 2 <= kAdjMaxNodes; sub++) {
+            int n = 2;
+            while (n <= kGlMax && gl_log_bound(n, phase / sub) > lim) n++;
+            if (n <= kGlMax && n * sub <= kAdjMaxNodes) {
+                if (capped || n * sub < best_n * best_sub) {
+                    best_n = n;
+                    best_sub = sub;
+                    capped = 0;
+                }
+                if (n <= kGlMax / 2) break;       // more sub-intervals only add nodes
+            }
+        }
+        S.order = best_n;
+        S.n_sub = best_sub;
+        S.phase = phase;
+        AdjStep o{};
+        o.n_nodes = best_n * best_sub;
+        o.order = best_n;
+        o.n_sub = best_sub;
+        o.capped = capped | (S.info[0] != 0) | (S.info[1] != 0) ? (capped ? 1 : 2) : 0;
+        o.phase = phase;
+        *a.out = o;
+    }
+    __syncthreads();
+    const int order = S.order, nsub = S.n_sub, nn = order * nsub;
+    const double sl = a.h / nsub;
+    // quadrature nodes and weights on [0, h]; node q = sub-interval q / order, point q % order
+    for (int q = tid; q < nn; q += blockDim.x) {
+        const double2 gw = a.gl[a.gl_off[order] + q % order];
+        a.w[q] = 0.5 * sl * gw.y;
+    }
+    // coefficients: wavefront wv handles (node, basis) pairs wv, wv + 4, ...; the last pair of
+    // each basis is its combination at t_k
+    for (int it = wv; it < 2 * (nn + 1); it += 4) {
+        const int b = it / (nn + 1), q = it % (nn + 1);
+        double x;
+        double2* out;
+        if (q < nn) {
+            const double2 gw = a.gl[a.gl_off[order] + q % order];
+            x = sl * ((q / order) + 0.5 * (1.0 + gw.x));
+            out = a.coef[b] + (size_t)q * kMaxKrylov;
+        } else {
+            x = 0.0;
+            out = a.coef_end[b];
+        }
+        const double arg = a.from_end[b] ? a.h - x : x;
+        adj_coef_wave(S, b, a.m[b], arg, a.scale[b] ? *a.scale[b] : 1.0, out);
+    }
+    // error integrals h_b int |[exp(y H_m) e1]_{m-1}| over the step, with the same rule
+    __syncthreads();
+    if (wv < 2) {
+        const int b = wv, m = a.m[b];
+        double acc = 0.0;
+        for (int q = 0; q < nn; q++) {
+            const double2 gw = a.gl[a.gl_off[order] + q % order];
+            const double x = sl * ((q / order) + 0.5 * (1.0 + gw.x));
+            const double arg = a.from_end[b] ? a.h - x : x;
+            double re = 0.0, im = 0.0;
+            if (lane < m) {
+                double sn, cs;
+                sincos(arg * S.lam[b][lane], &sn, &cs);
+                re = S.p[b][lane] * cs;
+                im = -S.p[b][lane] * sn;
+            }
+            re = wave_sum(re);
+            im = wave_sum(im);
+            acc += 0.5 * sl * gw.y * hypot(re, im);
+        }
+        if (lane == 0) a.out->err[b] = a.ks[b]->h * acc * (a.scale[b] ? *a.scale[b] : 1.0);
+    }
+}
+
+void launch_adjoint_coef(const AdjCoefArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(adjoint_coef_kernel, dim3(1), dim3(256), 0, s, a);
+}
+
+// ---------------------------------------------------------------------------------------
 // V w, V y_s and diagonal observables in one pass over the Krylov basis.
 // ---------------------------------------------------------------------------------------
 __device__ __forceinline__ void block_rows_k(int64_t n, int64_t& lo, int64_t& hi) {
