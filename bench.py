@@ -53,6 +53,9 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain measurement")
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--batched-only", action="store_true", help="only the batched measurement")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient measurement")
+    ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint-gradient measurement")
+    ap.add_argument("--adjoint-size", type=int, default=12)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -69,6 +72,9 @@ def main():
     from timeevolver_amd import capi
     if args.batched_only:
         print(json.dumps({"batched": bench_batched(np, capi, capi.Context(0), args)}), flush=True)
+        return
+    if args.adjoint_only:
+        print(json.dumps({"adjoint": bench_adjoint(np, capi, capi.Context(0), args)}), flush=True)
         return
     from timeevolver_amd.hamiltonian import as_tuples, createNumberOperator
     from timeevolver_amd.workloads import (fock_state, ring_bose_hubbard_terms, ring_couplings, ring_dimension,
@@ -164,9 +170,11 @@ def main():
 
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(np, capi, H, obs, D, args, r["n_steps"])
+    del ev, H, obs, basis
     if not args.no_batched:
-        del ev, H, obs, basis
         line["batched"] = bench_batched(np, capi, ctx, args)
+    if not args.no_adjoint:
+        line["adjoint"] = bench_adjoint(np, capi, ctx, args)
     print(json.dumps(line), flush=True)
 
 
@@ -230,6 +238,92 @@ def bench_batched(np, capi, ctx, args):
             "batched_s": round(t_batch, 4), "plain_s": round(t_plain, 4), "krylov_steps": steps_batch,
             "rounds": rounds, "same_steps_and_status_as_plain": same, "max_obs_diff_vs_plain": dobs,
             "build_s": round(build_s, 2)}
+
+
+def bench_adjoint(np, capi, ctx, args):
+    """Adjoint workload: ring Bose-Hubbard L = N = adjoint_size (12: D = 1,352,078), seeded eps_k and
+    J_k, U = 0.5, Fock |N,0,...,0>, T = 5, m = 40, tol = 1e-8, loss <psi(T)|n_0|psi(T)>, gradient with
+    respect to the L eps_k and L J_k. Forward cost = the plain evolver's time-to-T for the same
+    run (samples at 0 and T); the adjoint is timed (forward + backward) for checkpoint intervals
+    K in {0, 1, 4, 16}; the K = 0 gradient is checked against central differences (h = 1e-6) of
+    the plain evolver's loss at tol 1e-12."""
+    from timeevolver_amd.adjoint import ring_problem
+    from timeevolver_amd.hamiltonian import as_tuples, createNumberOperator
+    from timeevolver_amd.workloads import fock_state, ring_bose_hubbard_terms, ring_couplings
+
+    L = N = args.adjoint_size
+    T, m, tol, U = 5.0, 40, 1e-8, 0.5
+    eps, J = ring_couplings(L, args.seed)
+    theta = np.array(eps + J)
+    t0 = time.perf_counter()
+    prob = ring_problem(ctx, L, N, U, T, tol=tol, m=m, checkpoint_interval=0, theta0=theta)
+    D = prob.basis.D
+    psi0 = fock_state(D)
+    n0 = capi.operator_build(ctx, prob.basis, as_tuples([createNumberOperator(0, 1.0)]))
+    log("adjoint: L=N=%d D=%d set up in %.2f s" % (L, D, time.perf_counter() - t0))
+
+    def plain_loss(th, tl):
+        H = capi.operator_build(ctx, prob.basis, as_tuples(ring_bose_hubbard_terms(L, list(th[:L]), list(th[L:]), U)))
+        ev = capi.Evolver(ctx, H, psi0, T, T, [n0], tol=tl, m=m)
+        r = ev.run(want_state=False)
+        ev.close()
+        return r["observables"][0, 1], r
+
+    ev = capi.Evolver(ctx, prob.H, psi0, T, T, [n0], tol=tol, m=m)
+    ev.run(want_state=False)
+    fwd = []
+    for _ in range(3):
+        ctx.synchronize()
+        t = time.perf_counter()
+        r = ev.run(want_state=False)
+        ctx.synchronize()
+        fwd.append(time.perf_counter() - t)
+    t_fwd = float(np.median(fwd))
+    loss_plain = r["observables"][0, 1]
+    ev.close()
+    out = {"workload": "ring Bose-Hubbard L=N=%d, D=%d, T=%g, m=%d, tol=%g, loss <n_0>(T), gradient wrt %d eps_k + %d "
+                       "J_k" % (L, D, T, m, tol, L, L),
+           "forward_s": round(t_fwd, 4), "forward_steps": r["n_steps"], "K": {}}
+    grad0 = None
+    for K in (0, 1, 4, 16):
+        pk = prob if K == 0 else ring_problem(ctx, L, N, U, T, tol=tol, m=m, checkpoint_interval=K, theta0=theta)
+        pk.adj.run(psi0)
+        ts = []
+        for _ in range(3):
+            ctx.synchronize()
+            t = time.perf_counter()
+            loss, grad, res, info = pk.adj.run(psi0)
+            ctx.synchronize()
+            ts.append(time.perf_counter() - t)
+        tk = float(np.median(ts))
+        if K == 0:
+            grad0 = grad
+            out["loss"] = loss
+            out["loss_minus_plain"] = loss - loss_plain
+        out["K"][str(K)] = {"total_s": round(tk, 4), "ratio_vs_forward": round(tk / t_fwd, 3),
+                            "device_forward_ms": round(info["forward_ms"], 2),
+                            "device_backward_ms": round(info["backward_ms"], 2), "nodes": info["nodes"],
+                            "max_phase": round(info["max_phase"], 1), "lambda_err": info["backward_err"],
+                            "psi_err": info["psi_err"],
+                            "grad_rel_diff_vs_K0": float(np.linalg.norm(grad - grad0) / np.linalg.norm(grad0))}
+        log("adjoint K=%d: %.3f s (forward %.3f s) ratio %.2f nodes %d" % (K, tk, t_fwd, tk / t_fwd, info["nodes"]))
+        if K != 0:
+            pk.adj.close()
+    # central differences of the plain evolver's loss, tight tolerance
+    t = time.perf_counter()
+    h = 1e-6
+    fd = np.zeros_like(theta)
+    for p in range(len(theta)):
+        tp, tm = theta.copy(), theta.copy()
+        tp[p] += h
+        tm[p] -= h
+        fd[p] = (plain_loss(tp, 1e-12)[0] - plain_loss(tm, 1e-12)[0]) / (2 * h)
+    out["fd_rel_err"] = float(np.linalg.norm(grad0 - fd) / np.linalg.norm(fd))
+    out["fd_max_abs_err"] = float(np.abs(grad0 - fd).max())
+    out["grad_norm"] = float(np.linalg.norm(grad0))
+    log("adjoint: FD check %.2e relative (%.1f s)" % (out["fd_rel_err"], time.perf_counter() - t))
+    out["ratio_vs_forward"] = out["K"]["0"]["ratio_vs_forward"]
+    return out
 
 
 def measured_traffic(kernel, D):

@@ -22,8 +22,8 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
-#include <map>
 #include <vector>
 
 #include "te_internal.h"
@@ -33,7 +33,7 @@
 namespace te {
 
 constexpr int kNodeGroup = 8;       // nodes per node-interleaved block
-constexpr int kMatNodes = 64;       // nodes per materialisation pass
+constexpr int kMatNodes = 32;       // nodes per materialisation pass (4 groups)
 constexpr int kAdjMaxLabels = 64;   // generators (LDS accumulators of the term-wise pass)
 constexpr int kAdjMaxSteps = 1 << 16;
 
@@ -52,10 +52,10 @@ __global__ void start_norm_kernel(const double2* res, KrylovScalars* ks, double*
 }
 
 // Node states of one basis: out[g][r][q'] = sum_j U[j][r] coef[node0 + 8 g + q'][j] for the
-// nodes of this pass (zero past the step's node count; out holds the pass's 8 blocks); end[r] =
-// sum_j U[j][r] cend[j] when end is given. Eight lanes share a row (lane g forms nodes
-// 8g..8g+7 of the pass), so the basis is read once per pass and every lane writes 128
-// contiguous bytes.
+// NG groups of 8 nodes of this pass (zero past the step's node count; out holds the pass's
+// blocks); end[r] = sum_j U[j][r] cend[j] when end is given. One lane per row: the basis is
+// read once per pass (64 rows, 1 KB per wavefront load), the coefficients are uniform and come
+// in through scalar loads, and a lane writes 128 contiguous bytes per group.
 struct MatArgs {
     const double2* U;
     int64_t ld, n;
@@ -68,51 +68,56 @@ struct MatArgs {
     double2* end;
 };
 
+template <int NG>
 __global__ __launch_bounds__(256) void materialise_nodes_kernel(MatArgs a) {
-    __shared__ double2 cs[kMaxKrylov * kMatNodes];
-    __shared__ double2 ce[kMaxKrylov];
+    constexpr int NQ = NG * kNodeGroup;
+    __shared__ double2 cs[kMaxKrylov * (NQ + 1)];        // cs[j * (NQ + 1) + q], q = NQ: cend
     const int nn = a.step->n_nodes;
-    const int nloc = min(kMatNodes, nn - a.node0);
-    if (nloc <= 0 && !a.end) return;
-    for (int i = threadIdx.x; i < a.m * kMatNodes; i += blockDim.x) {
-        const int j = i / kMatNodes, q = i % kMatNodes;
-        cs[i] = q < nloc ? a.coef[(size_t)(a.node0 + q) * kMaxKrylov + j] : make_double2(0.0, 0.0);
+    const int nloc = min(NQ, nn - a.node0);
+    for (int i = threadIdx.x; i < a.m * (NQ + 1); i += blockDim.x) {
+        const int j = i / (NQ + 1), q = i % (NQ + 1);
+        double2 c = make_double2(0.0, 0.0);
+        if (q < nloc) c = a.coef[(size_t)(a.node0 + q) * kMaxKrylov + j];
+        else if (q == NQ && a.end) c = a.cend[j];
+        cs[i] = c;
     }
-    if (a.end)
-        for (int j = threadIdx.x; j < a.m; j += blockDim.x) ce[j] = a.cend[j];
     __syncthreads();
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int g = lane & 7, rl = lane >> 3;
-    const int ng = (nloc + kNodeGroup - 1) / kNodeGroup;
-    const int64_t step = (int64_t)gridDim.x * 32;
-    for (int64_t r0 = ((int64_t)blockIdx.x * 4 + w) * 8; r0 < a.n; r0 += step) {
-        const int64_t r = r0 + rl;
-        if (r >= a.n) continue;
-        double2 acc[kNodeGroup];
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.n; r += (int64_t)gridDim.x * blockDim.x) {
+        double2 acc[NQ];
 #pragma unroll
-        for (int q = 0; q < kNodeGroup; q++) acc[q] = make_double2(0.0, 0.0);
+        for (int q = 0; q < NQ; q++) acc[q] = make_double2(0.0, 0.0);
         double2 ae = make_double2(0.0, 0.0);
         for (int j = 0; j < a.m; j++) {
             const double2 u = a.U[(size_t)j * a.ld + r];
-            const double2* c = cs + j * kMatNodes + g * kNodeGroup;
+            const double2* cj = cs + j * (NQ + 1);
 #pragma unroll
-            for (int q = 0; q < kNodeGroup; q++) {
-                const double2 t = cmul(u, c[q]);
-                acc[q].x += t.x;
-                acc[q].y += t.y;
+            for (int q = 0; q < NQ; q++) {
+                const double2 c = cj[q];
+                acc[q].x += u.x * c.x - u.y * c.y;
+                acc[q].y += u.x * c.y + u.y * c.x;
             }
-            if (a.end && g == 0) {
-                const double2 t = cmul(u, ce[j]);
-                ae.x += t.x;
-                ae.y += t.y;
+            if (a.end) {
+                const double2 c = cj[NQ];
+                ae.x += u.x * c.x - u.y * c.y;
+                ae.y += u.x * c.y + u.y * c.x;
             }
         }
-        if (g < ng) {
+#pragma unroll
+        for (int g = 0; g < NG; g++) {
             double2* o = a.out + ((size_t)g * a.n + r) * kNodeGroup;
 #pragma unroll
-            for (int q = 0; q < kNodeGroup; q++) o[q] = acc[q];
+            for (int q = 0; q < kNodeGroup; q++) o[q] = acc[g * kNodeGroup + q];
         }
-        if (a.end && g == 0) a.end[r] = ae;
+        if (a.end) a.end[r] = ae;
+    }
+}
+
+static void launch_materialise(const MatArgs& m, int ng, int grid, hipStream_t s) {
+    switch (ng) {
+        case 1: hipLaunchKernelGGL(materialise_nodes_kernel<1>, dim3(grid), dim3(256), 0, s, m); break;
+        case 2: hipLaunchKernelGGL(materialise_nodes_kernel<2>, dim3(grid), dim3(256), 0, s, m); break;
+        case 3: hipLaunchKernelGGL(materialise_nodes_kernel<3>, dim3(grid), dim3(256), 0, s, m); break;
+        default: hipLaunchKernelGGL(materialise_nodes_kernel<4>, dim3(grid), dim3(256), 0, s, m); break;
     }
 }
 
@@ -717,8 +722,14 @@ static int backward(te_adjoint* a) {
     a->info.nodes = 0;
     a->info.max_phase = 0;
     a->info.capped = 0;
-    const int mg = std::max(8, std::min<int>((int)((n + 31) / 32), ctx->num_cus * 4)) / 8 * 8;
+    const int mg = std::max(8, std::min<int>((int)((n + 255) / 256), ctx->num_cus * 8)) / 8 * 8;
     const int tg = a->grid;
+    // node groups per materialisation pass (launch-shape sweeps: TE_ADJ_PASS_GROUPS = 1..4)
+    static const int pass_groups = [] {
+        const char* v = std::getenv("TE_ADJ_PASS_GROUPS");
+        return v ? std::max(1, std::min(4, std::atoi(v))) : 2;
+    }();
+    const int pass_nodes = pass_groups * kNodeGroup;
     for (int k = nsteps - 1; k >= 0; k--) {
         const double hstep = a->t_steps[k];
         // lambda: Lanczos of +i e H from lambda(t_{k+1})
@@ -778,14 +789,14 @@ static int backward(te_adjoint* a) {
         if (st.capped) a->info.capped++;
         if (st.capped == 2) return fail(ctx, TE_ERR_FATAL, "Internal error: eigen-solver did not converge");
         // node states, 64 per pass, and the states at t_k
-        for (int node0 = 0; node0 < st.n_nodes; node0 += kMatNodes) {
+        for (int node0 = 0; node0 < st.n_nodes; node0 += pass_nodes) {
             MatArgs mp{Upsi, n, n, ca.m[0], a->coef0, a->cend0, a->step, node0, a->nodes_psi,
                        (node0 == 0 && a->K >= 1) ? a->psi_next : nullptr};
             MatArgs ml{a->Vl, n, n, ca.m[1], a->coef1, a->cend1, a->step, node0, a->nodes_lam,
                        node0 == 0 ? a->lam_next : nullptr};
-            hipLaunchKernelGGL(materialise_nodes_kernel, dim3(mg), dim3(256), 0, s, mp);
-            hipLaunchKernelGGL(materialise_nodes_kernel, dim3(mg), dim3(256), 0, s, ml);
-            const int ng = (std::min(kMatNodes, st.n_nodes - node0) + kNodeGroup - 1) / kNodeGroup;
+            const int ng = (std::min(pass_nodes, st.n_nodes - node0) + kNodeGroup - 1) / kNodeGroup;
+            launch_materialise(mp, ng, mg, s);
+            launch_materialise(ml, ng, mg, s);
             for (int g = 0; g < ng; g++) {
                 TermArgs ta{};
                 ta.n = n;

@@ -57,47 +57,65 @@ __device__ __forceinline__ double fast_hypot(double a, double b) {
 
 // ---------------------------------------------------------------------------------------
 // Implicit QL with Wilkinson shifts on a symmetric tridiagonal matrix (Golub & Van Loan,
-// section 8.3), run by one wavefront: every lane computes the rotation scalars (uniform
-// values kept in LDS) and lane r applies each rotation to row r of Q.
+// section 8.3), run by one wavefront. Lane k keeps d[k] and e[k] in registers; the rotation
+// scalars are a uniform chain that reads d[i], d[i+1], e[i] with lane reads (no LDS round trip),
+// the convergence test of all k at once is one ballot, and lane r applies each rotation to row r
+// of Q (LDS). The operations and their order are those of the scalar algorithm, so the results
+// do not depend on this organisation. d, e: LDS, n entries in, eigenvalues out in d.
 // ---------------------------------------------------------------------------------------
+__device__ __forceinline__ double lane_read(double v, int k) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), k);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 __device__ int wave_tridiag_ql(int n, double* d, double* e, double* Q) {
     const int lane = threadIdx.x & 63;
     for (int k = 0; k < n; k++) Q[k * 64 + lane] = (lane == k) ? 1.0 : 0.0;
     if (n == 1) return 0;
-    e[n - 1] = 0.0;
-    for (int l = 0; l < n; l++) {
+    double dr = lane < n ? d[lane] : 0.0;
+    double er = lane < n - 1 ? e[lane] : 0.0;
+    int info = 0;
+    for (int l = 0; l < n && !info; l++) {
         int iter = 0;
         while (true) {
-            int mm;
-            for (mm = l; mm < n - 1; mm++) {
-                const double dd = fabs(d[mm]) + fabs(d[mm + 1]);
-                if (fabs(e[mm]) <= DBL_EPSILON * dd) break;
-            }
+            // first k >= l with |e_k| <= eps (|d_k| + |d_{k+1}|), or n - 1
+            const double dn = __shfl_down(dr, 1, 64);
+            const bool conv = (lane >= n - 1) || (fabs(er) <= DBL_EPSILON * (fabs(dr) + fabs(dn)));
+            const unsigned long long mask = __ballot(conv) >> l;
+            const int mm = l + __builtin_ctzll(mask);
             if (mm == l) break;
-            if (++iter > 60) return 1;
-            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+            if (++iter > 60) {
+                info = 1;
+                break;
+            }
+            const double dl = lane_read(dr, l), el = lane_read(er, l);
+            double g = (lane_read(dr, l + 1) - dl) / (2.0 * el);
             double r = fast_hypot(g, 1.0);
-            g = d[mm] - d[l] + e[l] / (g + copysign(r, g));
+            g = lane_read(dr, mm) - dl + el / (g + copysign(r, g));
             double s = 1.0, c = 1.0, p = 0.0;
             bool early = false;
             for (int i = mm - 1; i >= l; i--) {
-                const double f = s * e[i];
-                const double bb = c * e[i];
+                const double ei = lane_read(er, i);
+                const double f = s * ei;
+                const double bb = c * ei;
                 r = fast_hypot(f, g);
-                e[i + 1] = r;
+                if (lane == i + 1) er = r;
                 if (r == 0.0) {
-                    d[i + 1] -= p;
-                    e[mm] = 0.0;
+                    const double di1 = lane_read(dr, i + 1) - p;
+                    if (lane == i + 1) dr = di1;
+                    if (lane == mm) er = 0.0;
                     early = true;
                     break;
                 }
                 const double ir = 1.0 / r;
                 s = f * ir;
                 c = g * ir;
-                g = d[i + 1] - p;
-                r = (d[i] - g) * s + 2.0 * c * bb;
+                g = lane_read(dr, i + 1) - p;
+                r = (lane_read(dr, i) - g) * s + 2.0 * c * bb;
                 p = s * r;
-                d[i + 1] = g + p;
+                if (lane == i + 1) dr = g + p;
                 g = c * r - bb;
                 if (lane < n) {
                     const double qf = Q[(i + 1) * 64 + lane];
@@ -107,12 +125,16 @@ __device__ int wave_tridiag_ql(int n, double* d, double* e, double* Q) {
                 }
             }
             if (early) continue;
-            d[l] -= p;
-            e[l] = g;
-            e[mm] = 0.0;
+            if (lane == l) {
+                dr -= p;
+                er = g;
+            }
+            if (lane == mm) er = 0.0;
         }
     }
-    return 0;
+    if (lane < n) d[lane] = dr;
+    if (lane < n) e[lane] = er;
+    return info;
 }
 
 // f(x) = h |sum_k p_k e^{-i x lambda_k}| at S.xs[0..npts) -> S.fx; kGrp lanes per point.
@@ -821,7 +843,8 @@ __global__ __launch_bounds__(256) void adjoint_coef_kernel(AdjCoefArgs a) {
         o.n_nodes = best_n * best_sub;
         o.order = best_n;
         o.n_sub = best_sub;
-        o.capped = capped | (S.info[0] != 0) | (S.info[1] != 0) ? (capped ? 1 : 2) : 0;
+        // 2: an eigen-decomposition failed; 1: the rule was capped at kAdjMaxNodes nodes
+        o.capped = (S.info[0] != 0 || S.info[1] != 0) ? 2 : capped;
         o.phase = phase;
         *a.out = o;
     }
