@@ -249,6 +249,7 @@ struct te_adjoint {
     const int* perm = nullptr;      // H's locality order (new -> old), null for the basis order
     const int* iperm = nullptr;
     int grid = 8;
+    int pgrid = 8;
     // generators in the evolver's order
     int* gptr = nullptr;
     int* gcol = nullptr;
@@ -418,6 +419,7 @@ extern "C" int te_adjoint_create(te_ctx* ctx, te_matrix* H, int n_params, te_mat
     a->tolRate = params->tol / params->t;
     a->numErrEst = (double)n * nrm1 * DBL_EPSILON;
     a->grid = stream_grid(ctx, n);
+    a->pgrid = project_grid(ctx, n);
     if (H->use_vd && H->vd.perm) {
         a->perm = H->vd.perm;
         a->iperm = H->vd.iperm;
@@ -509,7 +511,8 @@ extern "C" int te_adjoint_create(te_ctx* ctx, te_matrix* H, int n_params, te_mat
     TE_TRY(aalloc(ctx, &a->trace, kAdjMaxSteps));
     TE_TRY(aalloc(ctx, &a->obs, 2));
     TE_TRY(upload_quadrature(ctx, a->quad));
-    TE_TRY(reduce_scratch_alloc(ctx, a->rs, std::max({a->grid, H->shape.grid, H->vd.grid, H->vd.sell_grid})));
+    TE_TRY(reduce_scratch_alloc(ctx, a->rs,
+                                std::max({a->grid, a->pgrid, H->shape.grid, H->vd.grid, H->vd.sell_grid})));
     TE_TRY(aalloc(ctx, &a->Vl, (size_t)n * (a->m0 + 1)));
     if (a->K >= 1) TE_TRY(aalloc(ctx, &a->Vp, (size_t)n * (a->m0 + 1)));
     TE_TRY(aalloc(ctx, &a->ksl, 1));
@@ -571,7 +574,7 @@ static void lanczos(te_adjoint* a, double2* U, KrylovScalars* ks, int m, double2
     for (int j = 0; j < m; j++) {
         la.j = j;
         launch_lanczos_apply(la, a->H, a->ctx->stream);
-        launch_lanczos_project(la, a->grid, a->ctx->stream);
+        launch_lanczos_project(la, a->pgrid, a->ctx->stream);
     }
 }
 

@@ -52,6 +52,19 @@ int stream_grid(const te_ctx* ctx, int64_t work_items) {
     return (int)((g + 7) / 8 * 8);
 }
 
+int project_grid(const te_ctx* ctx, int64_t rows) {
+    // Two blocks per CU, each streaming one long contiguous range: on the L = N = 14 ring the
+    // projection ran 187 us at 2 against 230 us at 4 and 289 us at 8 blocks per CU.
+    static const int per_cu = [] {
+        const char* v = std::getenv("TE_PROJECT_BLOCKS_PER_CU");    // launch-shape sweeps
+        return v ? std::max(1, std::atoi(v)) : 2;
+    }();
+    const int cap = std::max(8, ctx->num_cus * per_cu);
+    int64_t g = (rows + kBlock - 1) / kBlock;
+    g = std::max<int64_t>(8, std::min<int64_t>(g, cap));
+    return (int)((g + 7) / 8 * 8);
+}
+
 SpmvShape choose_shape(const te_ctx* ctx, int64_t n_rows, int64_t nnz) {
     SpmvShape s;
     const double mean = n_rows > 0 ? (double)nnz / (double)n_rows : 1.0;
@@ -504,6 +517,7 @@ struct te_evolver {
     DevQuad quad;
     ReduceScratch rs;
     int grid_stream = 8;
+    int grid_project = 8;
     std::vector<double2> e1;
     // Locality order of H's value-dictionary copy (null: the basis order). psi0, U, the
     // sampled states, tmp, diag_all and vec_dev are then in that order; tmp_ref and state_out
@@ -610,6 +624,7 @@ static int evolver_create(te_ctx* ctx, te_matrix* H, const void* psi0, bool psi0
     ev->tolRate = params->tol / params->t;                                                // :287
     ev->numErrEst = (double)n * nrm1 * DBL_EPSILON;                                       // :289
     ev->grid_stream = stream_grid(ctx, n);
+    ev->grid_project = project_grid(ctx, n);
 
     TE_TRY(dmalloc(ctx, &ev->psi0, n));
     TE_TRY(dmalloc(ctx, &ev->U, (size_t)n * (m + 1)));
@@ -626,7 +641,8 @@ static int evolver_create(te_ctx* ctx, te_matrix* H, const void* psi0, bool psi0
     TE_TRY(dmalloc(ctx, &ev->trace, ev->trace_cap));
     TE_TRY(dmalloc(ctx, &ev->obs_dev, (size_t)std::max(n_obs, 1) * ev->n_samples));
     TE_TRY(reduce_scratch_alloc(ctx, ev->rs,
-                                std::max({ev->grid_stream, H->shape.grid, H->vd.grid, H->vd.sell_grid})));
+                                std::max({ev->grid_stream, ev->grid_project, H->shape.grid, H->vd.grid,
+                                          H->vd.sell_grid})));
 
     if (psi0_on_device)
         TE_TRY(hipMemcpy(ev->psi0, psi0, sizeof(double2) * n, hipMemcpyDeviceToDevice) == hipSuccess ? TE_OK
@@ -947,7 +963,7 @@ extern "C" int te_evolver_run(te_evolver* ev, double* obs_values, double* evolve
             launch_lanczos_apply(la, ev->H, s);
             prof_end(ev, 0);
             prof_begin(ev, 1);
-            launch_lanczos_project(la, ev->grid_stream, s);
+            launch_lanczos_project(la, ev->grid_project, s);
             prof_end(ev, 1);
         }
         prof_begin(ev, 2);

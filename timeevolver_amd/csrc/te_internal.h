@@ -96,6 +96,7 @@ struct VdCsr {
     int n_slices = 0;
     int64_t sell_entries = 0;
     int sell_grid = 0;
+    int sell_chunk = 0;                 // SellView::chunk
 };
 
 constexpr int kSlice = 64;
@@ -112,6 +113,8 @@ struct SellView {
     const double2* diag;
     int n_slices;
     int64_t n;
+    int chunk;      // 0: XCD x sweeps the x-th eighth of the slices; > 0: chunks of `chunk` slices
+                    // dealt round-robin to the XCDs, so all XCDs sweep one front together
 };
 
 // What the kernels read of a VdCsr.
@@ -162,6 +165,8 @@ namespace te {
 // Grid of the row-range streaming kernels (projection, combine) for work_items rows: a
 // multiple of 8 (one share per XCD), at most TE_STREAM_BLOCKS_PER_CU (default 4) blocks per CU.
 int stream_grid(const te_ctx* ctx, int64_t work_items);
+// Grid of the Lanczos projection (TE_PROJECT_BLOCKS_PER_CU, default 2, blocks per CU).
+int project_grid(const te_ctx* ctx, int64_t rows);
 // SpMV launch shape for a matrix (lanes per row from the mean row length, grid from the CUs).
 SpmvShape choose_shape(const te_ctx* ctx, int64_t n_rows, int64_t nnz);
 // Derives the row blocks of the stream kernel on the device and picks the kernel (call after
