@@ -131,3 +131,21 @@ def test_batch_errors(gpu):
         capi.BatchEvolver(gpu, [x[1] for x in inst], [x[4] for x in inst], 1.0, 0.25, [[x[1]] for x in inst],
                           tol=1e-8)
     assert e.value.code == capi.TE_ERR_UNSUPPORTED
+
+
+def test_batch_sell_and_csr_paths(gpu, monkeypatch):
+    """The sliced-ELL batch (instances in their locality order) and the CSR row-block batch
+    (TE_BATCH_SELL=0) agree with each other and with the plain evolver."""
+    specs = _seeded_specs(77, 5, (5, 6, 7))
+    inst = _instances(gpu, specs, (0, 1))
+    T, step, tol, m = 1.5, 0.25, 1e-8, 30
+    args = ([x[1] for x in inst], [x[4] for x in inst], T, step, [x[2] for x in inst])
+    r_sell = capi.BatchEvolver(gpu, *args, tol=tol, m=m).run()
+    monkeypatch.setenv("TE_BATCH_SELL", "0")
+    r_csr = capi.BatchEvolver(gpu, *args, tol=tol, m=m).run()
+    for (b, H, obs, terms, psi), a, c in zip(inst, r_sell, r_csr):
+        rp = capi.Evolver(gpu, H, psi, T, step, obs, tol=tol, m=m).run()
+        for r in (a, c):
+            assert r["n_steps"] == rp["n_steps"] and r["statusCode"] == rp["statusCode"]
+            assert np.linalg.norm(r["state"] - rp["state"]) <= r["err"] + rp["err"] + 1e-10
+            assert np.abs(r["observables"] - rp["observables"]).max() <= 2 * 7 * (r["err"] + rp["err"]) + 1e-10

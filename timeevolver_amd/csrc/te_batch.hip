@@ -8,10 +8,12 @@
 // how the work reaches the device:
 //  * the Krylov bases of all instances are packed into one array of (m_max + 1) columns, an
 //    instance owning rows [off_i, off_i + D_i) of every column;
-//  * one launch per kernel kind covers every instance: the Lanczos apply walks SpMV units (up
-//    to kBatchRB row blocks of one instance), the project/combine passes walk vector units (up
-//    to vrows rows of one instance); each unit first reads its instance's flags and skips the
-//    instance when it is finished, broken down, or past its Krylov dimension;
+//  * one launch per kernel kind covers every instance: the Lanczos apply walks SpMV units (a run
+//    of sliced-ELL slices of one instance when every instance has its value-dictionary copy,
+//    vectors then in each instance's own locality order; else up to kBatchRB CSR row blocks),
+//    the project/combine passes walk vector units (up to vrows rows of one instance); each unit
+//    first reads its instance's flags and skips the instance when it is finished, broken down,
+//    or past its Krylov dimension;
 //  * reductions are per instance and deterministic: a unit writes its block partial, the last
 //    unit of the instance to arrive (per-instance ticket) sums the instance's partials in unit
 //    order and resets the ticket;
@@ -22,17 +24,27 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
 #include "te_internal.h"
 #include "te_krylov.h"
 #include "te_reduce.h"
+#include "te_sell.h"
 
 namespace te {
 
-constexpr int kBatchRB = 4;          // row blocks (kStreamChunk entries + rows each) per SpMV unit
-constexpr int kBatchVecRows = 4096;  // rows per vector unit (grown when the observable partials would be large)
+constexpr int kBatchRB = 4;          // row blocks (kStreamChunk entries + rows each) per CSR SpMV unit
+// Slices (kSlice rows each) per sliced-ELL SpMV unit: the batch's slices per block of the grid,
+// clamped to [16, 128]. Each unit ends with a block reduction and a ticket and may restage its
+// instance's value table, so units should be large, but a small batch still needs units for
+// every block: on 512 ring instances with L in {6..10} (181k slices) the batched run took
+// 0.381 s with 16-slice units, 0.281 s with 32, 0.247 s with 64, 0.244 s with 128.
+constexpr int kBatchSlicesMin = 16, kBatchSlicesMax = 128;
+// Rows per vector unit (projection, combine), grown when the observable partials would be large:
+// 4096 / 8192 / 16384 / 32768 rows gave 0.269 / 0.252 / 0.247 / 0.260 s on the same batch.
+constexpr int kBatchVecRows = 16384;
 constexpr int kBatchSC = 8;          // samples per pass of the combine kernel
 constexpr int kBatchMaxObs = 4;      // diagonal observables per instance
 constexpr int kBatchTraceCap = 4096; // recorded step sizes per instance
@@ -46,11 +58,12 @@ struct BatchInst {
     const int* col;
     const double2* val;
     const int* rb;          // its row blocks (kStreamChunk merge-path chunks)
+    SellView sell;          // its sliced-ELL value-dictionary copy (when the batch uses it)
 };
 
 struct BatchDev {
     const BatchInst* inst;
-    const int4* sunit;      // (instance, first row block, end row block, 0)
+    const int4* sunit;      // (instance, first row block, end row block, 0), or slices for sliced ELL
     int n_sunits;
     const int4* vunit;      // (instance, first row, end row, 0), rows local to the instance
     int n_vunits;
@@ -142,6 +155,52 @@ __global__ __launch_bounds__(kBlock) void batch_apply_kernel(BatchDev a, int j) 
         epi.dot = make_double2(0.0, 0.0);
         for (int b = su.y; b < su.z; b++)
             stream_row_block(I.rb[b], I.rb[b + 1], I.ptr, I.col, I.val, epi.uj, prod, epi, nullptr);
+        const double2 bs = block_sum2<kBlock>(epi.dot, sh);
+        if (threadIdx.x == 0) a.spart[u] = bs;
+        if (last_unit_arrive(a.sticket + i, (unsigned)I.nsu)) {
+            const double2 tot = reduce_partials<kBlock>(a.spart + I.su0, I.nsu, 1, sh);
+            if (threadIdx.x == 0) {
+                ks->a[j] = make_double2(sj * tot.x, sj * tot.y);
+                a.sticket[i] = 0;
+            }
+        }
+    }
+}
+
+// The same for instances held as sliced-ELL value-dictionary copies (vectors in each instance's
+// locality order): a unit is a run of slices of one instance, its wavefronts take the
+// slices in turn, and the instance's value table is staged in LDS when the unit's instance
+// differs from the block's previous one.
+__global__ __launch_bounds__(kBlock) void batch_apply_sell_kernel(BatchDev a, int j) {
+    __shared__ double2 vt[kVdMaxValues];
+    __shared__ double2 sh[kBlock / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int staged = -1;
+    int u0, u1, us;
+    xcd_sweep(a.n_sunits, u0, u1, us);
+    for (int u = u0; u < u1; u += us) {
+        const int4 su = a.sunit[u];
+        const int i = su.x;
+        if (!lanczos_active(a, i, j)) continue;
+        const BatchInst& I = a.inst[i];
+        const SellView m = I.sell;
+        if (i != staged) {
+            __syncthreads();                             // every wavefront is done with the old table
+            for (int q = threadIdx.x; q < m.n_values; q += kBlock) vt[q] = m.vtab[q];
+            __syncthreads();
+            staged = i;
+        }
+        KrylovScalars* ks = a.ks + i;
+        const double sj = ks->s[j];
+        ApplyEpiVd epi;
+        epi.c1 = make_double2(a.alpha.x * sj, a.alpha.y * sj);
+        epi.c2 = j > 0 ? ks->beta[j - 1] * ks->s[j - 1] : 0.0;
+        epi.up = j > 0 ? a.U + (size_t)(j - 1) * a.ld + I.off : nullptr;
+        epi.w = a.U + (size_t)(j + 1) * a.ld + I.off;
+        epi.dot = make_double2(0.0, 0.0);
+        const double2* x = a.U + (size_t)j * a.ld + I.off;
+        for (int sl = su.y + wv; sl < su.z; sl += kBlock / 64)
+            sell_slice(m, x, vt, __builtin_amdgcn_readfirstlane(sl), lane, epi);
         const double2 bs = block_sum2<kBlock>(epi.dot, sh);
         if (threadIdx.x == 0) a.spart[u] = bs;
         if (last_unit_arrive(a.sticket + i, (unsigned)I.nsu)) {
@@ -371,6 +430,9 @@ struct te_batch {
     std::vector<double> numErrEst;
     BatchDev dev{};
     int grid_s = 8, grid_v = 8;
+    bool sell = false;                 // instances held as sliced-ELL copies (locality order)
+    std::vector<const int*> perm, iperm;
+    double2* tmp = nullptr;            // packed staging vector (reordering)
     // device buffers
     BatchInst* inst = nullptr;
     int4 *sunit = nullptr, *vunit = nullptr;
@@ -400,7 +462,7 @@ static void batch_free(te_batch* b) {
     hipSetDevice(b->ctx->device);
     if (b->exec) hipGraphExecDestroy(b->exec);
     if (b->graph) hipGraphDestroy(b->graph);
-    void* ptrs[] = {b->inst, b->sunit,   b->vunit,  b->U,  b->psi0, b->sampled, b->wcoef, b->Y,
+    void* ptrs[] = {b->inst, b->sunit,   b->vunit,  b->U,  b->psi0, b->sampled, b->wcoef, b->Y, b->tmp,
                     b->spart, b->vpart,  b->sticket, b->vticket, b->ks, b->st, b->bc, b->consts,
                     b->diag, b->opart,   b->obs,    b->nrm, b->trace, b->n_live};
     for (void* p : ptrs)
@@ -420,7 +482,8 @@ static int balloc(te_ctx* ctx, T** p, size_t count) {
 static void batch_round(te_batch* b, hipStream_t s) {
     const BatchDev& d = b->dev;
     for (int j = 0; j < b->m_max; j++) {
-        hipLaunchKernelGGL(batch_apply_kernel, dim3(b->grid_s), dim3(kBlock), 0, s, d, j);
+        if (b->sell) hipLaunchKernelGGL(batch_apply_sell_kernel, dim3(b->grid_s), dim3(kBlock), 0, s, d, j);
+        else hipLaunchKernelGGL(batch_apply_kernel, dim3(b->grid_s), dim3(kBlock), 0, s, d, j);
         hipLaunchKernelGGL(batch_project_kernel, dim3(b->grid_v), dim3(kBlock), 0, s, d, j);
     }
     launch_batch_step_control(b->consts, b->st, b->ks, b->bc, b->n_inst, b->n_samples, s);
@@ -474,6 +537,28 @@ extern "C" int te_batch_create(te_ctx* ctx, int n_inst, te_matrix* const* H, con
     b->m0.resize(n_inst);
     b->numErrEst.resize(n_inst);
 
+    // sliced-ELL path when every instance has the copy (TE_BATCH_SELL=0: CSR row blocks)
+    {
+        const char* e = std::getenv("TE_BATCH_SELL");
+        bool all = !(e && e[0] == '0');
+        for (int i = 0; i < n_inst && all; i++)
+            all = H[i] && H[i]->use_vd && H[i]->vd.sell && H[i]->vd.n_values <= kVdMaxValues;
+        b->sell = all;
+    }
+    // launch shape: blocks per CU of the streaming kernels (8: 0.247 s against 0.278 s at 4)
+    const char* gs_env = std::getenv("TE_BATCH_BLOCKS_PER_CU");     // launch-shape sweeps
+    const int per_cu = gs_env ? std::max(1, std::atoi(gs_env)) : 8;
+    const int grid_cap = std::max(8, ctx->num_cus * per_cu);
+    int unit_slices = kBatchSlicesMin;
+    if (b->sell) {
+        int64_t total_slices = 0;
+        for (int i = 0; i < n_inst; i++) total_slices += H[i]->vd.n_slices;
+        unit_slices = (int)std::max<int64_t>(kBatchSlicesMin, std::min<int64_t>(kBatchSlicesMax, total_slices / grid_cap));
+    }
+    const char* us_env = std::getenv("TE_BATCH_UNIT_SLICES");       // launch-shape sweeps
+    if (us_env) unit_slices = std::max(1, std::atoi(us_env));
+    b->perm.assign(n_inst, nullptr);
+    b->iperm.assign(n_inst, nullptr);
     // instances: dimensions, Krylov dimensions, unit tables
     std::vector<BatchInst> hinst(n_inst);
     std::vector<int4> hsu, hvu;
@@ -535,13 +620,23 @@ extern "C" int te_batch_create(te_ctx* ctx, int n_inst, te_matrix* const* H, con
         I.val = A->val;
         I.rb = A->row_block;
         I.su0 = (int)hsu.size();
-        for (int r = 0; r < A->n_row_blocks; r += kBatchRB)
-            hsu.push_back(make_int4(i, r, std::min(r + kBatchRB, A->n_row_blocks), 0));
+        if (b->sell) {
+            I.sell = sell_view_of(A);
+            b->perm[i] = A->vd.perm;
+            b->iperm[i] = A->vd.iperm;
+            for (int sl = 0; sl < I.sell.n_slices; sl += unit_slices)
+                hsu.push_back(make_int4(i, sl, std::min(sl + unit_slices, I.sell.n_slices), 0));
+        } else {
+            I.sell = SellView{};
+            for (int r = 0; r < A->n_row_blocks; r += kBatchRB)
+                hsu.push_back(make_int4(i, r, std::min(r + kBatchRB, A->n_row_blocks), 0));
+        }
         I.nsu = (int)hsu.size() - I.su0;
     }
     b->ld = total;
     // vector units: kBatchVecRows rows, grown so the observable partials stay under 256 MiB
-    int vrows = kBatchVecRows;
+    const char* vr_env = std::getenv("TE_BATCH_VEC_ROWS");          // launch-shape sweeps
+    int vrows = vr_env ? std::max(256, std::atoi(vr_env)) : kBatchVecRows;
     while (true) {
         int64_t nvu = 0;
         for (int i = 0; i < n_inst; i++) nvu += (b->dim[i] + vrows - 1) / vrows;
@@ -561,6 +656,7 @@ extern "C" int te_batch_create(te_ctx* ctx, int n_inst, te_matrix* const* H, con
     TE_TRY(balloc(ctx, &b->U, (size_t)total * (b->m_max + 1)));
     TE_TRY(balloc(ctx, &b->psi0, (size_t)total));
     TE_TRY(balloc(ctx, &b->sampled, (size_t)total));
+    TE_TRY(balloc(ctx, &b->tmp, (size_t)total));
     TE_TRY(balloc(ctx, &b->wcoef, (size_t)n_inst * kMaxKrylov));
     TE_TRY(balloc(ctx, &b->Y, (size_t)n_inst * ns * kMaxKrylov));
     TE_TRY(balloc(ctx, &b->spart, hsu.size()));
@@ -590,10 +686,20 @@ extern "C" int te_batch_create(te_ctx* ctx, int n_inst, te_matrix* const* H, con
     TE_TRYH(hipMemsetAsync(b->diag, 0, sizeof(double) * std::max(n_obs, 1) * total, s));
     std::vector<StepConsts> hc(n_inst);
     for (int i = 0; i < n_inst; i++) {
-        TE_TRYH(hipMemcpyAsync(b->psi0 + b->off[i], psi0[i], sizeof(double2) * b->dim[i], hipMemcpyHostToDevice, s));
-        for (int o = 0; o < n_obs; o++)
-            TE_TRYH(hipMemcpyAsync(b->diag + (size_t)o * total + b->off[i], obs[(size_t)i * n_obs + o].matrix->diag_re,
-                                   sizeof(double) * b->dim[i], hipMemcpyDeviceToDevice, s));
+        // states and observables enter in the instance's own row order
+        if (b->perm[i]) {
+            TE_TRYH(hipMemcpyAsync(b->tmp + b->off[i], psi0[i], sizeof(double2) * b->dim[i], hipMemcpyHostToDevice, s));
+            launch_gather(b->tmp + b->off[i], b->perm[i], b->psi0 + b->off[i], b->dim[i], s);
+        } else {
+            TE_TRYH(hipMemcpyAsync(b->psi0 + b->off[i], psi0[i], sizeof(double2) * b->dim[i], hipMemcpyHostToDevice,
+                                   s));
+        }
+        for (int o = 0; o < n_obs; o++) {
+            const double* dre = obs[(size_t)i * n_obs + o].matrix->diag_re;
+            double* dst = b->diag + (size_t)o * total + b->off[i];
+            if (b->perm[i]) launch_gather_re(dre, b->perm[i], dst, b->dim[i], s);
+            else TE_TRYH(hipMemcpyAsync(dst, dre, sizeof(double) * b->dim[i], hipMemcpyDeviceToDevice, s));
+        }
         StepConsts C = step_consts(b->P, b->tolRate, b->numErrEst[i], b->n_samples, b->quad);
         C.Y = b->Y + (size_t)i * ns * kMaxKrylov;
         C.wcoef = b->wcoef + (size_t)i * kMaxKrylov;
@@ -645,13 +751,12 @@ extern "C" int te_batch_create(te_ctx* ctx, int n_inst, te_matrix* const* H, con
     d.nrm = b->nrm;
     d.n_inst = n_inst;
     d.n_live = b->n_live;
-    // LDS of the apply kernel (36 KB) allows four blocks per CU
-    auto grid = [&](int items, int per_cu) {
-        int g = std::min(items, ctx->num_cus * per_cu);
+    auto grid = [&](int items, int blocks_per_cu) {
+        int g = std::min(items, ctx->num_cus * blocks_per_cu);
         return std::max(8, (g + 7) / 8 * 8);
     };
-    b->grid_s = grid(d.n_sunits, 4);
-    b->grid_v = grid(d.n_vunits, 4);
+    b->grid_s = grid(d.n_sunits, per_cu);
+    b->grid_v = grid(d.n_vunits, per_cu);
     *out = b;
     return TE_OK;
 }
@@ -759,9 +864,14 @@ extern "C" int te_batch_run(te_batch* b, double* obs_values, double* const* stat
                                    s));
     if (states)
         for (int i = 0; i < n_inst; i++)
-            if (states[i])
-                TE_HIP(ctx, hipMemcpyAsync(states[i], b->sampled + b->off[i], sizeof(double2) * b->dim[i],
-                                           hipMemcpyDeviceToHost, s));
+            if (states[i]) {
+                const double2* src = b->sampled + b->off[i];
+                if (b->iperm[i]) {                   // back to the basis order
+                    launch_gather(src, b->iperm[i], b->tmp + b->off[i], b->dim[i], s);
+                    src = b->tmp + b->off[i];
+                }
+                TE_HIP(ctx, hipMemcpyAsync(states[i], src, sizeof(double2) * b->dim[i], hipMemcpyDeviceToHost, s));
+            }
     TE_HIP(ctx, hipStreamSynchronize(s));
     int first_fatal = -1, fatal_code = 0;
     for (int i = 0; i < n_inst; i++) {

@@ -173,6 +173,8 @@ SpmvShape choose_shape(const te_ctx* ctx, int64_t n_rows, int64_t nnz);
 // row_ptr is on the device). perm (device, n_rows entries, new -> old), when given, is the row
 // and column order of the value-dictionary copy the Lanczos kernels read; it is copied.
 int prepare_spmv(te_ctx* ctx, te_matrix* A, const int* perm = nullptr);
+// The sliced-ELL view of A's value-dictionary copy (valid when A->use_vd && A->vd.sell).
+SellView sell_view_of(const te_matrix* A);
 // Vector gathers between the reference order and a matrix's locality order.
 // dst[i] = src[idx[i]] for i < n.
 void launch_gather(const double2* src, const int* idx, double2* dst, int64_t n, hipStream_t s);

@@ -1,0 +1,106 @@
+// Device code shared by the single-instance and batched Lanczos kernels for the sliced-ELL
+// value-dictionary form of a matrix (slices of kSlice rows stored slot-major, 16-bit indices into
+// a table of distinct off-diagonal values, dense diagonal).
+#pragma once
+
+#include "te_internal.h"
+#include "te_reduce.h"
+
+namespace te {
+
+__device__ __forceinline__ void sell_load_group(const int* cp, const unsigned short* vp, int* c, unsigned* v) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        c[q] = __builtin_nontemporal_load(cp + q * kSlice);
+        v[q] = __builtin_nontemporal_load(vp + q * kSlice);
+    }
+}
+
+// Slice s of a sliced-ELL value-dictionary matrix by the calling wavefront (lane = row of the
+// slice): y_r = diag_r x_r + sum_j vt[vidx_j] x[col_j] (diagonal first, then column order: the
+// same sums as the row-block kernel), handed to epi.row. Slots past a row's length are loaded
+// (col 0, value 0) and not added. Four slots are loaded before any is used, and the next group's
+// slots load while the current group's gathers are in flight. vt: the value table in LDS.
+template <class Epi>
+__device__ __forceinline__ void sell_slice(const SellView& m, const double2* __restrict__ x, const double2* vt, int s,
+                                           int lane, Epi& epi) {
+    const int o = m.off[s];
+    const int ls = (m.off[s + 1] - o) / kSlice;                   // slots of this slice
+    const int ng = (ls + 3) >> 2;                                  // groups of 4 slots
+    const int64_t r = (int64_t)s * kSlice + lane;
+    const unsigned lr = m.lr[r];
+    const int nl = (int)(lr & 255u), rfirst = ls - (int)(lr >> 8);  // slots [0,nl) and [rfirst,ls)
+    double2 xr = make_double2(0.0, 0.0), d = xr;
+    typename Epi::Pre pre{};
+    if (r < m.n) {
+        xr = x[r];
+        d = load_stream(&m.diag[r]);
+        pre = epi.load((int)r);
+    }
+    const int* cp = m.col + o + lane;
+    const unsigned short* vp = m.vidx + o + lane;
+    int c[4];
+    unsigned v[4];
+    sell_load_group(cp, vp, c, v);
+    double sr = d.x * xr.x - d.y * xr.y, si = d.x * xr.y + d.y * xr.x;
+    // Group g's gathers and group g+1's column loads are in flight together (the last group
+    // is peeled so that the loop body is straight-line code for the wait counters). Slots
+    // past the slice end (last group) read the next slice's entries or the zero padding: in
+    // bounds, valid columns, never added.
+    auto accumulate = [&](const unsigned* vv, const double2* xv, int g) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const double2 p = cmul(vt[vv[q]], xv[q]);
+            const int j = 4 * g + q;
+            const bool in = (j < nl) | ((j >= rfirst) & (j < ls));   // selects, not branches
+            sr = in ? sr + p.x : sr;
+            si = in ? si + p.y : si;
+        }
+    };
+    for (int g = 0; g + 1 < ng; g++) {
+        double2 xv[4];
+        unsigned vv[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            xv[q] = x[c[q]];
+            vv[q] = v[q];
+        }
+        cp += 4 * kSlice;
+        vp += 4 * kSlice;
+        sell_load_group(cp, vp, c, v);
+        accumulate(vv, xv, g);
+    }
+    if (ng > 0) {
+        double2 xv[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) xv[q] = x[c[q]];
+        accumulate(v, xv, ng - 1);
+    }
+    if (r < m.n) epi.row((int)r, make_double2(sr, si), xr, pre);
+}
+
+// Lanczos apply epilogue on one row: w = c1 (A u_j)_r + c2 u_{j-1}[r] (non-temporal store),
+// dot += conj(u_j[r]) w_r; `load` fetches u_{j-1}[r] ahead of the row's sum.
+struct ApplyEpiVd {
+    using Pre = double2;
+    double2 c1;
+    double c2;
+    const double2* up;
+    double2* w;
+    double2 dot;
+    __device__ __forceinline__ Pre load(int r) const {
+        return up ? load_stream(&up[r]) : make_double2(0.0, 0.0);
+    }
+    __device__ __forceinline__ void row(int r, double2 s, double2 u, Pre p) {
+        double2 wr = cmul(c1, s);
+        if (up) {
+            wr.x += c2 * p.x;
+            wr.y += c2 * p.y;
+        }
+        store_stream(&w[r], wr);
+        dot.x += u.x * wr.x + u.y * wr.y;
+        dot.y += u.x * wr.y - u.y * wr.x;
+    }
+};
+
+}  // namespace te

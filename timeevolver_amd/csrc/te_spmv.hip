@@ -18,6 +18,7 @@
 #include "te_internal.h"
 #include "te_krylov.h"
 #include "te_reduce.h"
+#include "te_sell.h"
 
 #include <hipcub/hipcub.hpp>
 
@@ -420,28 +421,6 @@ __global__ __launch_bounds__(kBlock) void spmv_vd_kernel(VdView m, const double2
 }
 
 // As ApplyEpi, with u_j[r] (= x[r]) and u_{j-1}[r] loaded ahead by the row loop.
-struct ApplyEpiVd {
-    using Pre = double2;
-    double2 c1;
-    double c2;
-    const double2* up;
-    double2* w;
-    double2 dot;
-    __device__ __forceinline__ Pre load(int r) const {
-        return up ? load_stream(&up[r]) : make_double2(0.0, 0.0);
-    }
-    __device__ __forceinline__ void row(int r, double2 s, double2 u, Pre p) {
-        double2 wr = cmul(c1, s);
-        if (up) {
-            wr.x += c2 * p.x;
-            wr.y += c2 * p.y;
-        }
-        store_stream(&w[r], wr);
-        dot.x += u.x * wr.x + u.y * wr.y;
-        dot.y += u.x * wr.y - u.y * wr.x;
-    }
-};
-
 __global__ __launch_bounds__(kBlock) void lanczos_apply_vd_kernel(LanczosArgs a) {
     KrylovScalars* ks = a.ks;
     if (ks->broken) return;
@@ -473,14 +452,6 @@ __global__ __launch_bounds__(kBlock) void lanczos_apply_vd_kernel(LanczosArgs a)
 // not added. Four slots are loaded before any is used, and the wavefronts of a CU work on
 // different slices, so gathers from x overlap without LDS staging or barriers.
 // ---------------------------------------------------------------------------------------
-__device__ __forceinline__ void sell_load_group(const int* cp, const unsigned short* vp, int* c, unsigned* v) {
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-        c[q] = __builtin_nontemporal_load(cp + q * kSlice);
-        v[q] = __builtin_nontemporal_load(vp + q * kSlice);
-    }
-}
-
 template <class Epi>
 __device__ __forceinline__ void sell_rows(const SellView& m, const double2* __restrict__ x, double2* vt, Epi& epi) {
     for (int i = threadIdx.x; i < m.n_values; i += blockDim.x) vt[i] = m.vtab[i];
@@ -510,59 +481,7 @@ __device__ __forceinline__ void sell_rows(const SellView& m, const double2* __re
             if (s >= end) break;
         }
         s = __builtin_amdgcn_readfirstlane(s);
-        const int o = m.off[s];
-        const int ls = (m.off[s + 1] - o) / kSlice;                   // slots of this slice
-        const int ng = (ls + 3) >> 2;                                  // groups of 4 slots
-        const int64_t r = (int64_t)s * kSlice + lane;
-        const unsigned lr = m.lr[r];
-        const int nl = (int)(lr & 255u), rfirst = ls - (int)(lr >> 8);  // slots [0,nl) and [rfirst,ls)
-        double2 xr = make_double2(0.0, 0.0), d = xr;
-        typename Epi::Pre pre{};
-        if (r < m.n) {
-            xr = x[r];
-            d = load_stream(&m.diag[r]);
-            pre = epi.load((int)r);
-        }
-        const int* cp = m.col + o + lane;
-        const unsigned short* vp = m.vidx + o + lane;
-        int c[4];
-        unsigned v[4];
-        sell_load_group(cp, vp, c, v);
-        double sr = d.x * xr.x - d.y * xr.y, si = d.x * xr.y + d.y * xr.x;
-        // Group g's gathers and group g+1's column loads are in flight together (the last group
-        // is peeled so that the loop body is straight-line code for the wait counters). Slots
-        // past the slice end (last group) read the next slice's entries or the zero padding: in
-        // bounds, valid columns, never added.
-        auto accumulate = [&](const unsigned* vv, const double2* xv, int g) {
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const double2 p = cmul(vt[vv[q]], xv[q]);
-                const int j = 4 * g + q;
-                const bool in = (j < nl) | ((j >= rfirst) & (j < ls));   // selects, not branches
-                sr = in ? sr + p.x : sr;
-                si = in ? si + p.y : si;
-            }
-        };
-        for (int g = 0; g + 1 < ng; g++) {
-            double2 xv[4];
-            unsigned vv[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                xv[q] = x[c[q]];
-                vv[q] = v[q];
-            }
-            cp += 4 * kSlice;
-            vp += 4 * kSlice;
-            sell_load_group(cp, vp, c, v);
-            accumulate(vv, xv, g);
-        }
-        if (ng > 0) {
-            double2 xv[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) xv[q] = x[c[q]];
-            accumulate(v, xv, ng - 1);
-        }
-        if (r < m.n) epi.row((int)r, make_double2(sr, si), xr, pre);
+        sell_slice(m, x, vt, s, lane, epi);
     }
 }
 
@@ -677,6 +596,8 @@ static SellView sell_view(const te_matrix* A) {
     return SellView{v.sell_off, v.sell_col, v.sell_vidx, v.sell_lr, v.vtab,       v.n_values,
                     v.diag,     v.n_slices, A->n_rows,  v.sell_chunk};
 }
+
+SellView sell_view_of(const te_matrix* A) { return sell_view(A); }
 
 // y = alpha A x in the reference order (a reordered value-dictionary copy is for the evolver).
 void launch_spmv(const te_matrix* A, double2 alpha, const double2* x, double2* y, hipStream_t s) {
