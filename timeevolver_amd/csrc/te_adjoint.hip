@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <vector>
 
 #include "te_internal.h"
@@ -33,7 +34,7 @@
 namespace te {
 
 constexpr int kNodeGroup = 8;       // nodes per node-interleaved block
-constexpr int kMatNodes = 32;       // nodes per materialisation pass (4 groups)
+constexpr int kMatNodes = 64;       // nodes per materialisation pass (8 groups)
 constexpr int kAdjMaxLabels = 64;   // generators (LDS accumulators of the term-wise pass)
 constexpr int kAdjMaxSteps = 1 << 16;
 
@@ -52,10 +53,11 @@ __global__ void start_norm_kernel(const double2* res, KrylovScalars* ks, double*
 }
 
 // Node states of one basis: out[g][r][q'] = sum_j U[j][r] coef[node0 + 8 g + q'][j] for the
-// NG groups of 8 nodes of this pass (zero past the step's node count; out holds the pass's
-// blocks); end[r] = sum_j U[j][r] cend[j] when end is given. One lane per row: the basis is
-// read once per pass (64 rows, 1 KB per wavefront load), the coefficients are uniform and come
-// in through scalar loads, and a lane writes 128 contiguous bytes per group.
+// nodes of this pass (zero past the step's node count; out holds the pass's blocks); end[r] =
+// sum_j U[j][r] cend[j] when end is given. LPR lanes share a row, lane h of it forming nodes
+// [16 h, 16 h + 16) of the pass: the basis is read once per pass of up to 16 LPR nodes (the
+// lanes of a row load the same address), the coefficients are LDS broadcasts, and a lane writes
+// two 128-byte blocks.
 struct MatArgs {
     const double2* U;
     int64_t ld, n;
@@ -68,9 +70,11 @@ struct MatArgs {
     double2* end;
 };
 
-template <int NG>
+constexpr int kLaneNodes = 16;     // nodes formed by one lane
+
+template <int LPR>
 __global__ __launch_bounds__(256) void materialise_nodes_kernel(MatArgs a) {
-    constexpr int NQ = NG * kNodeGroup;
+    constexpr int NQ = LPR * kLaneNodes;
     __shared__ double2 cs[kMaxKrylov * (NQ + 1)];        // cs[j * (NQ + 1) + q], q = NQ: cend
     const int nn = a.step->n_nodes;
     const int nloc = min(NQ, nn - a.node0);
@@ -82,57 +86,72 @@ __global__ __launch_bounds__(256) void materialise_nodes_kernel(MatArgs a) {
         cs[i] = c;
     }
     __syncthreads();
-    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.n; r += (int64_t)gridDim.x * blockDim.x) {
-        double2 acc[NQ];
+    const int h = threadIdx.x % LPR;
+    const int ng = (nloc + kNodeGroup - 1) / kNodeGroup;        // blocks written by this pass
+    const int64_t rows_per_grid = (int64_t)gridDim.x * (blockDim.x / LPR);
+    for (int64_t r = (int64_t)blockIdx.x * (blockDim.x / LPR) + threadIdx.x / LPR; r < a.n; r += rows_per_grid) {
+        double2 acc[kLaneNodes];
 #pragma unroll
-        for (int q = 0; q < NQ; q++) acc[q] = make_double2(0.0, 0.0);
+        for (int q = 0; q < kLaneNodes; q++) acc[q] = make_double2(0.0, 0.0);
         double2 ae = make_double2(0.0, 0.0);
         for (int j = 0; j < a.m; j++) {
             const double2 u = a.U[(size_t)j * a.ld + r];
-            const double2* cj = cs + j * (NQ + 1);
+            const double2* cj = cs + j * (NQ + 1) + h * kLaneNodes;
 #pragma unroll
-            for (int q = 0; q < NQ; q++) {
+            for (int q = 0; q < kLaneNodes; q++) {
                 const double2 c = cj[q];
                 acc[q].x += u.x * c.x - u.y * c.y;
                 acc[q].y += u.x * c.y + u.y * c.x;
             }
-            if (a.end) {
-                const double2 c = cj[NQ];
+            if (a.end && h == 0) {
+                const double2 c = cs[j * (NQ + 1) + NQ];
                 ae.x += u.x * c.x - u.y * c.y;
                 ae.y += u.x * c.y + u.y * c.x;
             }
         }
 #pragma unroll
-        for (int g = 0; g < NG; g++) {
-            double2* o = a.out + ((size_t)g * a.n + r) * kNodeGroup;
+        for (int gl = 0; gl < kLaneNodes / kNodeGroup; gl++) {
+            const int g = h * (kLaneNodes / kNodeGroup) + gl;
+            if (g < ng) {
+                double2* o = a.out + ((size_t)g * a.n + r) * kNodeGroup;
 #pragma unroll
-            for (int q = 0; q < kNodeGroup; q++) o[q] = acc[g * kNodeGroup + q];
+                for (int q = 0; q < kNodeGroup; q++) o[q] = acc[gl * kNodeGroup + q];
+            }
         }
-        if (a.end) a.end[r] = ae;
+        if (a.end && h == 0) a.end[r] = ae;
     }
 }
 
-static void launch_materialise(const MatArgs& m, int ng, int grid, hipStream_t s) {
-    switch (ng) {
-        case 1: hipLaunchKernelGGL(materialise_nodes_kernel<1>, dim3(grid), dim3(256), 0, s, m); break;
-        case 2: hipLaunchKernelGGL(materialise_nodes_kernel<2>, dim3(grid), dim3(256), 0, s, m); break;
-        case 3: hipLaunchKernelGGL(materialise_nodes_kernel<3>, dim3(grid), dim3(256), 0, s, m); break;
-        default: hipLaunchKernelGGL(materialise_nodes_kernel<4>, dim3(grid), dim3(256), 0, s, m); break;
-    }
+// Lanes per row for a pass of `nodes` nodes (at most kMatNodes).
+static void launch_materialise(const MatArgs& m, int nodes, int grid, hipStream_t s) {
+    if (nodes <= kLaneNodes) hipLaunchKernelGGL(materialise_nodes_kernel<1>, dim3(grid), dim3(256), 0, s, m);
+    else if (nodes <= 2 * kLaneNodes) hipLaunchKernelGGL(materialise_nodes_kernel<2>, dim3(grid), dim3(256), 0, s, m);
+    else hipLaunchKernelGGL(materialise_nodes_kernel<4>, dim3(grid), dim3(256), 0, s, m);
 }
 
 // One group of 8 nodes: for every row r, L_q = w_q conj(lambda_q(r)),
-//   Z(r) += sum_q Im(L_q psi_q(r))                       (diagonal generators)
+//   Z(r) += sum_q Im(L_q psi_q(r))                              (diagonal generators)
 //   acc[label] += Im(t sum_q L_q psi_q(c))  per entry (r, c, label, t)   (off-diagonal ones)
-// with per-thread accumulators in LDS; block sums per label go to partials, and the last
+// The labelled entries are held as sliced ELL (64-row slices, slot-major, padded to the slice's
+// longest row, row lengths predicate the padding) with values through a table of the distinct
+// values in LDS (or stored when there are too many); a wavefront takes a slice, lane = row,
+// loading the next slot's column, label and value index before the current slot's gather.
+// Per-thread label accumulators live in LDS; block sums per label go to partials and the last
 // block adds their sum (in block order) to out[label].
+constexpr int kTermMaxValues = 1024;
+
 struct TermArgs {
     int64_t n;
-    const int* ptr;             // labelled off-diagonal entries (null: none)
-    const int* col;
-    const unsigned char* lab;
-    const double2* val;
-    int n_labels;
+    int n_slices;               // 0: no off-diagonal entries
+    const int* soff;            // n_slices + 1 slot offsets (multiples of 64)
+    const int* scol;
+    const unsigned char* slab;  // compact label (generators with off-diagonal entries)
+    const unsigned short* svid; // value index (table) ...
+    const double2* sval;        // ... or the value
+    const unsigned short* slen; // entries per row
+    const double2* vtab;
+    int n_values;
+    int n_labels;               // compact labels
     const double2* psi;         // node block of the group, n x 8
     const double2* lam;
     const double* w;            // kAdjMaxNodes weights
@@ -144,34 +163,66 @@ struct TermArgs {
     double* out;                // n_labels running sums
 };
 
-template <int LCAP>
+template <int LCAP, bool TABLE>
 __global__ __launch_bounds__(256) void termwise_kernel(TermArgs a) {
     __shared__ double acc[LCAP * 256];
+    __shared__ double2 vt[TABLE ? kTermMaxValues : 1];
     __shared__ double wq[kNodeGroup];
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     for (int l = 0; l < a.n_labels; l++) acc[l * 256 + tid] = 0.0;
+    if (TABLE)
+        for (int i = tid; i < a.n_values; i += 256) vt[i] = a.vtab[i];
     if (tid < kNodeGroup) {
         const int q = a.group * kNodeGroup + tid;
         wq[tid] = q < a.step->n_nodes ? a.w[q] : 0.0;
     }
     __syncthreads();
-    for (int64_t r = (int64_t)blockIdx.x * 256 + tid; r < a.n; r += (int64_t)gridDim.x * 256) {
+    const int n_sl = (int)((a.n + 63) / 64);
+    for (int sl = blockIdx.x * 4 + wv; sl < n_sl; sl += gridDim.x * 4) {
+        const int64_t r = (int64_t)sl * 64 + lane;
         double2 L[kNodeGroup];
-        const double2* lr = a.lam + (size_t)r * kNodeGroup;
-        const double2* pr = a.psi + (size_t)r * kNodeGroup;
-        double z = 0.0;
+        int len = 0;
+        if (r < a.n) {
+            const double2* lr = a.lam + (size_t)r * kNodeGroup;
+            const double2* pr = a.psi + (size_t)r * kNodeGroup;
+            double z = 0.0;
 #pragma unroll
-        for (int q = 0; q < kNodeGroup; q++) {
-            const double2 l = lr[q];
-            L[q] = make_double2(wq[q] * l.x, -wq[q] * l.y);
-            const double2 p = pr[q];
-            z += L[q].x * p.y + L[q].y * p.x;
+            for (int q = 0; q < kNodeGroup; q++) {
+                const double2 l = lr[q];
+                L[q] = make_double2(wq[q] * l.x, -wq[q] * l.y);
+                const double2 p = pr[q];
+                z += L[q].x * p.y + L[q].y * p.x;
+            }
+            a.zsum[r] += z;
+            if (a.n_slices) len = a.slen[r];
+        } else {
+#pragma unroll
+            for (int q = 0; q < kNodeGroup; q++) L[q] = make_double2(0.0, 0.0);
         }
-        a.zsum[r] += z;
-        if (a.ptr) {
-            for (int k = a.ptr[r]; k < a.ptr[r + 1]; k++) {
-                const int c = a.col[k];
-                const double2* pc = a.psi + (size_t)c * kNodeGroup;
+        if (!a.n_slices) continue;
+        const int o = a.soff[sl];
+        const int ls = (a.soff[sl + 1] - o) >> 6;
+        int c = 0, lb = 0, vi = 0;
+        double2 tv = make_double2(0.0, 0.0);
+        if (ls > 0) {
+            const size_t k = (size_t)o + lane;
+            c = a.scol[k];
+            lb = a.slab[k];
+            if (TABLE) vi = a.svid[k];
+            else tv = a.sval[k];
+        }
+        for (int j = 0; j < ls; j++) {
+            const int cc = c, lbl = lb;
+            const double2 t = TABLE ? vt[vi] : tv;
+            if (j + 1 < ls) {                          // next slot's indices in flight
+                const size_t k = (size_t)o + (size_t)(j + 1) * 64 + lane;
+                c = a.scol[k];
+                lb = a.slab[k];
+                if (TABLE) vi = a.svid[k];
+                else tv = a.sval[k];
+            }
+            if (j < len) {
+                const double2* pc = a.psi + (size_t)cc * kNodeGroup;
                 double sr = 0.0, si = 0.0;
 #pragma unroll
                 for (int q = 0; q < kNodeGroup; q++) {
@@ -179,8 +230,7 @@ __global__ __launch_bounds__(256) void termwise_kernel(TermArgs a) {
                     sr += L[q].x * p.x - L[q].y * p.y;
                     si += L[q].x * p.y + L[q].y * p.x;
                 }
-                const double2 t = a.val[k];
-                acc[a.lab[k] * 256 + tid] += t.x * si + t.y * sr;
+                acc[lbl * 256 + tid] += t.x * si + t.y * sr;
             }
         }
     }
@@ -225,11 +275,17 @@ __global__ __launch_bounds__(256) void diag_grad_kernel(const double* __restrict
     }
 }
 
+template <bool TABLE>
+static void launch_termwise_t(const TermArgs& ta, int grid, hipStream_t s) {
+    if (ta.n_labels <= 8) hipLaunchKernelGGL((termwise_kernel<8, TABLE>), dim3(grid), dim3(256), 0, s, ta);
+    else if (ta.n_labels <= 16) hipLaunchKernelGGL((termwise_kernel<16, TABLE>), dim3(grid), dim3(256), 0, s, ta);
+    else if (ta.n_labels <= 32) hipLaunchKernelGGL((termwise_kernel<32, TABLE>), dim3(grid), dim3(256), 0, s, ta);
+    else hipLaunchKernelGGL((termwise_kernel<64, TABLE>), dim3(grid), dim3(256), 0, s, ta);
+}
+
 static void launch_termwise(const TermArgs& ta, int grid, hipStream_t s) {
-    if (ta.n_labels <= 8) hipLaunchKernelGGL(termwise_kernel<8>, dim3(grid), dim3(256), 0, s, ta);
-    else if (ta.n_labels <= 16) hipLaunchKernelGGL(termwise_kernel<16>, dim3(grid), dim3(256), 0, s, ta);
-    else if (ta.n_labels <= 32) hipLaunchKernelGGL(termwise_kernel<32>, dim3(grid), dim3(256), 0, s, ta);
-    else hipLaunchKernelGGL(termwise_kernel<64>, dim3(grid), dim3(256), 0, s, ta);
+    if (ta.vtab) launch_termwise_t<true>(ta, grid, s);
+    else launch_termwise_t<false>(ta, grid, s);
 }
 
 }  // namespace te
@@ -251,10 +307,16 @@ struct te_adjoint {
     int grid = 8;
     int pgrid = 8;
     // generators in the evolver's order
-    int* gptr = nullptr;
-    int* gcol = nullptr;
-    unsigned char* glab = nullptr;
-    double2* gval = nullptr;
+    int* soff = nullptr;            // labelled off-diagonal entries, sliced ELL
+    int* scol = nullptr;
+    unsigned char* slab = nullptr;
+    unsigned short* svid = nullptr;
+    double2* sval = nullptr;
+    unsigned short* slen = nullptr;
+    double2* vtab = nullptr;
+    int n_values = 0, n_slices = 0;
+    int n_off = 0;                  // generators with off-diagonal entries (compact labels)
+    std::vector<int> off_label;     // generator -> compact label, or -1
     int64_t g_nnz = 0;
     double* gdiag = nullptr;        // P x n (Re of the diagonal part)
     double* odiag = nullptr;        // n
@@ -312,7 +374,7 @@ static void adj_free(te_adjoint* a) {
     if (!a) return;
     hipSetDevice(a->ctx->device);
     adj_free_bases(a);
-    void* ptrs[] = {a->gptr, a->gcol,    a->glab, a->gval, a->gdiag,  a->odiag,  a->U,      a->psiT,
+    void* ptrs[] = {a->soff, a->scol, a->slab, a->svid, a->sval, a->slen, a->vtab, a->gdiag,  a->odiag,  a->U,      a->psiT,
                     a->scratch, a->psi0, a->ks,   a->ks_hist, a->st,  a->wcoef, a->Y,      a->trace,
                     a->obs,  a->Vl,      a->Vp,   a->ksl,  a->ksp,    a->scale,  a->gl,     a->gl_off,
                     a->coef0, a->coef1,  a->cend0, a->cend1, a->w,    a->step,   a->nodes_psi, a->nodes_lam,
@@ -460,32 +522,92 @@ extern "C" int te_adjoint_create(te_ctx* ctx, te_matrix* H, int n_params, te_mat
         return fail(ctx, TE_ERR_UNSUPPORTED, "too many generator entries");
     }
     a->g_nnz = off_nnz;
+    a->off_label.assign(n_params, -1);
+    for (int p = 0; p < n_params; p++) {
+        bool has = false;
+        for (int64_t r = 0; r < n && !has; r++)
+            for (int64_t k = gp[p][r]; k < gp[p][r + 1]; k++)
+                if (gc[p][k] != r) {
+                    has = true;
+                    break;
+                }
+        if (has) a->off_label[p] = a->n_off++;
+    }
     if (off_nnz > 0) {
-        std::vector<int> hptr(n + 1, 0), hcol(off_nnz);
-        std::vector<unsigned char> hlab(off_nnz);
-        std::vector<double2> hval(off_nnz);
-        int64_t k = 0;
-        for (int64_t r = 0; r < n; r++) {              // row r of the evolver order
-            hptr[r] = (int)k;
+        // rows of the evolver order in slices of 64, slot-major, padded to the slice's longest row
+        std::vector<int> len(n, 0);
+        for (int64_t r = 0; r < n; r++) {
             const int ro = hperm[r];
             for (int p = 0; p < n_params; p++)
                 for (int64_t e = gp[p][ro]; e < gp[p][ro + 1]; e++)
-                    if (gc[p][e] != ro) {
-                        hcol[k] = hiperm[gc[p][e]];
-                        hlab[k] = (unsigned char)p;
-                        hval[k] = gv[p][e];
-                        k++;
-                    }
+                    if (gc[p][e] != ro) len[r]++;
         }
-        hptr[n] = (int)k;
-        TE_TRY(aalloc(ctx, &a->gptr, n + 1));
-        TE_TRY(aalloc(ctx, &a->gcol, off_nnz));
-        TE_TRY(aalloc(ctx, &a->glab, off_nnz));
-        TE_TRY(aalloc(ctx, &a->gval, off_nnz));
-        TE_TRYH(hipMemcpy(a->gptr, hptr.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
-        TE_TRYH(hipMemcpy(a->gcol, hcol.data(), sizeof(int) * off_nnz, hipMemcpyHostToDevice));
-        TE_TRYH(hipMemcpy(a->glab, hlab.data(), off_nnz, hipMemcpyHostToDevice));
-        TE_TRYH(hipMemcpy(a->gval, hval.data(), sizeof(double2) * off_nnz, hipMemcpyHostToDevice));
+        const int ns = (int)((n + 63) / 64);
+        std::vector<int> hoff(ns + 1, 0);
+        for (int sl = 0; sl < ns; sl++) {
+            int mx = 0;
+            for (int l = 0; l < 64 && (int64_t)sl * 64 + l < n; l++) mx = std::max(mx, len[(size_t)sl * 64 + l]);
+            if (mx > 65535) {
+                adj_free(a);
+                return fail(ctx, TE_ERR_UNSUPPORTED, "generator rows longer than 65535 entries");
+            }
+            if ((int64_t)hoff[sl] + (int64_t)mx * 64 >= ((int64_t)1 << 31)) {
+                adj_free(a);
+                return fail(ctx, TE_ERR_UNSUPPORTED, "too many generator entries");
+            }
+            hoff[sl + 1] = hoff[sl] + mx * 64;
+        }
+        const size_t slots = (size_t)hoff[ns];
+        std::vector<int> hcol(slots, 0);
+        std::vector<unsigned char> hlab(slots, 0);
+        std::vector<double2> hval(slots, make_double2(0.0, 0.0));
+        std::vector<unsigned short> hlen(n);
+        std::map<std::pair<unsigned long long, unsigned long long>, int> table;
+        std::vector<unsigned short> hvid(slots, 0);
+        std::vector<double2> htab;
+        for (int64_t r = 0; r < n; r++) {
+            const int ro = hperm[r];
+            const int sl = (int)(r / 64), l = (int)(r % 64);
+            int j = 0;
+            for (int p = 0; p < n_params; p++)
+                for (int64_t e = gp[p][ro]; e < gp[p][ro + 1]; e++)
+                    if (gc[p][e] != ro) {
+                        const size_t k = (size_t)hoff[sl] + (size_t)j * 64 + l;
+                        hcol[k] = hiperm[gc[p][e]];
+                        hlab[k] = (unsigned char)a->off_label[p];
+                        hval[k] = gv[p][e];
+                        unsigned long long kx, ky;
+                        std::memcpy(&kx, &gv[p][e].x, 8);
+                        std::memcpy(&ky, &gv[p][e].y, 8);
+                        auto it = table.find({kx, ky});
+                        if (it == table.end()) {
+                            it = table.emplace(std::make_pair(kx, ky), (int)htab.size()).first;
+                            htab.push_back(gv[p][e]);
+                        }
+                        hvid[k] = (unsigned short)std::min(it->second, 65535);
+                        j++;
+                    }
+            hlen[r] = (unsigned short)j;
+        }
+        a->n_slices = ns;
+        TE_TRY(aalloc(ctx, &a->soff, ns + 1));
+        TE_TRY(aalloc(ctx, &a->scol, slots));
+        TE_TRY(aalloc(ctx, &a->slab, slots));
+        TE_TRY(aalloc(ctx, &a->slen, n));
+        TE_TRYH(hipMemcpy(a->soff, hoff.data(), sizeof(int) * (ns + 1), hipMemcpyHostToDevice));
+        TE_TRYH(hipMemcpy(a->scol, hcol.data(), sizeof(int) * slots, hipMemcpyHostToDevice));
+        TE_TRYH(hipMemcpy(a->slab, hlab.data(), slots, hipMemcpyHostToDevice));
+        TE_TRYH(hipMemcpy(a->slen, hlen.data(), sizeof(unsigned short) * n, hipMemcpyHostToDevice));
+        if ((int)htab.size() <= kTermMaxValues) {        // values through the table
+            a->n_values = (int)htab.size();
+            TE_TRY(aalloc(ctx, &a->svid, slots));
+            TE_TRY(aalloc(ctx, &a->vtab, htab.size()));
+            TE_TRYH(hipMemcpy(a->svid, hvid.data(), sizeof(unsigned short) * slots, hipMemcpyHostToDevice));
+            TE_TRYH(hipMemcpy(a->vtab, htab.data(), sizeof(double2) * htab.size(), hipMemcpyHostToDevice));
+        } else {
+            TE_TRY(aalloc(ctx, &a->sval, slots));
+            TE_TRYH(hipMemcpy(a->sval, hval.data(), sizeof(double2) * slots, hipMemcpyHostToDevice));
+        }
     }
     TE_TRY(aalloc(ctx, &a->gdiag, (size_t)std::max(n_params, 1) * n));
     if (n_params > 0)
@@ -727,12 +849,12 @@ static int backward(te_adjoint* a) {
     a->info.capped = 0;
     const int mg = std::max(8, std::min<int>((int)((n + 255) / 256), ctx->num_cus * 8)) / 8 * 8;
     const int tg = a->grid;
-    // node groups per materialisation pass (launch-shape sweeps: TE_ADJ_PASS_GROUPS = 1..4)
-    static const int pass_groups = [] {
-        const char* v = std::getenv("TE_ADJ_PASS_GROUPS");
-        return v ? std::max(1, std::min(4, std::atoi(v))) : 2;
+    // nodes per materialisation pass (launch-shape sweeps: TE_ADJ_PASS_NODES = 16, 32 or 64)
+    static const int pass_nodes = [] {
+        const char* v = std::getenv("TE_ADJ_PASS_NODES");
+        const int p = v ? std::atoi(v) : kMatNodes;
+        return p <= 16 ? 16 : (p <= 32 ? 32 : 64);
     }();
-    const int pass_nodes = pass_groups * kNodeGroup;
     for (int k = nsteps - 1; k >= 0; k--) {
         const double hstep = a->t_steps[k];
         // lambda: Lanczos of +i e H from lambda(t_{k+1})
@@ -797,17 +919,23 @@ static int backward(te_adjoint* a) {
                        (node0 == 0 && a->K >= 1) ? a->psi_next : nullptr};
             MatArgs ml{a->Vl, n, n, ca.m[1], a->coef1, a->cend1, a->step, node0, a->nodes_lam,
                        node0 == 0 ? a->lam_next : nullptr};
-            const int ng = (std::min(pass_nodes, st.n_nodes - node0) + kNodeGroup - 1) / kNodeGroup;
-            launch_materialise(mp, ng, mg, s);
-            launch_materialise(ml, ng, mg, s);
+            const int pn = std::min(pass_nodes, st.n_nodes - node0);
+            const int ng = (pn + kNodeGroup - 1) / kNodeGroup;
+            launch_materialise(mp, pn, mg, s);
+            launch_materialise(ml, pn, mg, s);
             for (int g = 0; g < ng; g++) {
                 TermArgs ta{};
                 ta.n = n;
-                ta.ptr = a->g_nnz > 0 ? a->gptr : nullptr;
-                ta.col = a->gcol;
-                ta.lab = a->glab;
-                ta.val = a->gval;
-                ta.n_labels = a->P;
+                ta.n_slices = a->n_slices;
+                ta.soff = a->soff;
+                ta.scol = a->scol;
+                ta.slab = a->slab;
+                ta.svid = a->svid;
+                ta.sval = a->sval;
+                ta.slen = a->slen;
+                ta.vtab = a->vtab;
+                ta.n_values = a->n_values;
+                ta.n_labels = a->n_off;
                 ta.psi = a->nodes_psi + (size_t)g * n * kNodeGroup;
                 ta.lam = a->nodes_lam + (size_t)g * n * kNodeGroup;
                 ta.w = a->w;
@@ -877,7 +1005,10 @@ extern "C" int te_adjoint_run(te_adjoint* a, const double* psi0, double* loss, d
     hipEventDestroy(e2);
     if (rc != TE_OK) return rc;
     *loss = hl[1];
-    for (int p = 0; p < a->P; p++) grad[p] = 2.0 * a->prm.exp_factor * (off[p] + (a->P > 0 ? dg[p] : 0.0));
+    for (int p = 0; p < a->P; p++) {
+        const double o = a->off_label[p] >= 0 ? off[a->off_label[p]] : 0.0;
+        grad[p] = 2.0 * a->prm.exp_factor * (o + dg[p]);
+    }
     a->info.forward_ms = f_ms;
     a->info.backward_ms = b_ms;
     a->info.n_steps = (int)a->t_steps.size();
