@@ -60,8 +60,9 @@ __device__ __forceinline__ double fast_hypot(double a, double b) {
 // section 8.3), run by one wavefront. Lane k keeps d[k] and e[k] in registers; the rotation
 // scalars are a uniform chain that reads d[i], d[i+1], e[i] with lane reads (no LDS round trip),
 // the convergence test of all k at once is one ballot, and lane r applies each rotation to row r
-// of Q (LDS). The operations and their order are those of the scalar algorithm, so the results
-// do not depend on this organisation. d, e: LDS, n entries in, eigenvalues out in d.
+// of Q (LDS). The rotation's norm and its reciprocal come from the reciprocal square root with
+// one Newton step (within an ulp or two of the rounded values). d, e: LDS, n entries in,
+// eigenvalues out in d.
 // ---------------------------------------------------------------------------------------
 __device__ __forceinline__ double lane_read(double v, int k) {
     const long long b = __double_as_longlong(v);
@@ -100,7 +101,20 @@ __device__ int wave_tridiag_ql(int n, double* d, double* e, double* Q) {
                 const double ei = lane_read(er, i);
                 const double f = s * ei;
                 const double bb = c * ei;
-                r = fast_hypot(f, g);
+                // r = |(f, g)| and 1/r: in the normal range from the hardware reciprocal square
+                // root refined by one Newton step (no correctly rounded sqrt and divide in the
+                // chain); hypot and a division otherwise
+                const double ss = f * f + g * g;
+                double ir;
+                if (ss > 1e-290 && ss < 1e290) {
+                    double y = __builtin_amdgcn_rsq(ss);
+                    y = y * (1.5 - 0.5 * ss * y * y);
+                    ir = y;
+                    r = ss * y;
+                } else {
+                    r = hypot(f, g);
+                    ir = r != 0.0 ? 1.0 / r : 0.0;
+                }
                 if (lane == i + 1) er = r;
                 if (r == 0.0) {
                     const double di1 = lane_read(dr, i + 1) - p;
@@ -109,7 +123,6 @@ __device__ int wave_tridiag_ql(int n, double* d, double* e, double* Q) {
                     early = true;
                     break;
                 }
-                const double ir = 1.0 / r;
                 s = f * ir;
                 c = g * ir;
                 g = lane_read(dr, i + 1) - p;
