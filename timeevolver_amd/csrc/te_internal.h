@@ -100,6 +100,7 @@ struct VdCsr {
 };
 
 constexpr int kSlice = 64;
+constexpr int kSellGroup = 4;       // slots a lane loads (and gathers) before using any
 constexpr int kSellWindow = 512;
 
 // What the sliced-ELL kernels read.

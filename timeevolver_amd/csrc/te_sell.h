@@ -10,7 +10,7 @@ namespace te {
 
 __device__ __forceinline__ void sell_load_group(const int* cp, const unsigned short* vp, int* c, unsigned* v) {
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
+    for (int q = 0; q < kSellGroup; q++) {
         c[q] = __builtin_nontemporal_load(cp + q * kSlice);
         v[q] = __builtin_nontemporal_load(vp + q * kSlice);
     }
@@ -19,14 +19,15 @@ __device__ __forceinline__ void sell_load_group(const int* cp, const unsigned sh
 // Slice s of a sliced-ELL value-dictionary matrix by the calling wavefront (lane = row of the
 // slice): y_r = diag_r x_r + sum_j vt[vidx_j] x[col_j] (diagonal first, then column order: the
 // same sums as the row-block kernel), handed to epi.row. Slots past a row's length are loaded
-// (col 0, value 0) and not added. Four slots are loaded before any is used, and the next group's
-// slots load while the current group's gathers are in flight. vt: the value table in LDS.
+// (col 0, value 0) and not added. A group of kSellGroup slots is loaded before any is used, and the
+// next group's slots load while the current group's gathers are in flight. vt: the value table in
+// LDS.
 template <class Epi>
 __device__ __forceinline__ void sell_slice(const SellView& m, const double2* __restrict__ x, const double2* vt, int s,
                                            int lane, Epi& epi) {
     const int o = m.off[s];
     const int ls = (m.off[s + 1] - o) / kSlice;                   // slots of this slice
-    const int ng = (ls + 3) >> 2;                                  // groups of 4 slots
+    const int ng = (ls + kSellGroup - 1) / kSellGroup;             // groups of slots
     const int64_t r = (int64_t)s * kSlice + lane;
     const unsigned lr = m.lr[r];
     const int nl = (int)(lr & 255u), rfirst = ls - (int)(lr >> 8);  // slots [0,nl) and [rfirst,ls)
@@ -39,8 +40,8 @@ __device__ __forceinline__ void sell_slice(const SellView& m, const double2* __r
     }
     const int* cp = m.col + o + lane;
     const unsigned short* vp = m.vidx + o + lane;
-    int c[4];
-    unsigned v[4];
+    int c[kSellGroup];
+    unsigned v[kSellGroup];
     sell_load_group(cp, vp, c, v);
     double sr = d.x * xr.x - d.y * xr.y, si = d.x * xr.y + d.y * xr.x;
     // Group g's gathers and group g+1's column loads are in flight together (the last group
@@ -49,31 +50,31 @@ __device__ __forceinline__ void sell_slice(const SellView& m, const double2* __r
     // bounds, valid columns, never added.
     auto accumulate = [&](const unsigned* vv, const double2* xv, int g) {
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
+        for (int q = 0; q < kSellGroup; q++) {
             const double2 p = cmul(vt[vv[q]], xv[q]);
-            const int j = 4 * g + q;
+            const int j = kSellGroup * g + q;
             const bool in = (j < nl) | ((j >= rfirst) & (j < ls));   // selects, not branches
             sr = in ? sr + p.x : sr;
             si = in ? si + p.y : si;
         }
     };
     for (int g = 0; g + 1 < ng; g++) {
-        double2 xv[4];
-        unsigned vv[4];
+        double2 xv[kSellGroup];
+        unsigned vv[kSellGroup];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
+        for (int q = 0; q < kSellGroup; q++) {
             xv[q] = x[c[q]];
             vv[q] = v[q];
         }
-        cp += 4 * kSlice;
-        vp += 4 * kSlice;
+        cp += kSellGroup * kSlice;
+        vp += kSellGroup * kSlice;
         sell_load_group(cp, vp, c, v);
         accumulate(vv, xv, g);
     }
     if (ng > 0) {
-        double2 xv[4];
+        double2 xv[kSellGroup];
 #pragma unroll
-        for (int q = 0; q < 4; q++) xv[q] = x[c[q]];
+        for (int q = 0; q < kSellGroup; q++) xv[q] = x[c[q]];
         accumulate(v, xv, ng - 1);
     }
     if (r < m.n) epi.row((int)r, make_double2(sr, si), xr, pre);

@@ -1074,8 +1074,8 @@ static int build_sell(te_ctx* ctx, te_matrix* A) {
     }
     if (rc == TE_OK && total < INT32_MAX) {
         TE_HIP(ctx, hipMalloc(&v.sell_off, sizeof(int) * (ns + 1)));
-        // a slice's last group of 4 slots may run up to 3 slots past its end: zero padding
-        const int64_t pad = 4 * kSlice;
+        // a slice's last group of slots may run up to kSellGroup - 1 slots past its end: zero padding
+        const int64_t pad = (int64_t)kSellGroup * kSlice;
         TE_HIP(ctx, hipMalloc(&v.sell_col, sizeof(int) * (total + pad)));
         TE_HIP(ctx, hipMalloc(&v.sell_vidx, sizeof(unsigned short) * (total + pad)));
         TE_HIP(ctx, hipMemsetAsync(v.sell_col + total, 0, sizeof(int) * pad, st));
