@@ -65,11 +65,13 @@ def build_hip(verbose=False):
 
 
 def build_cpp(verbose=False):
-    """The reference-compatible C++ layer (cpp/), if present."""
+    """The reference-compatible C++ layer (cpp/) and its examples; result files are HDF5 when
+    the Makefile finds the HDF5 library, CSV otherwise."""
     mk = os.path.join(HERE, "cpp", "Makefile")
     if not os.path.exists(mk):
         return None
-    r = subprocess.run(["make", "-s", "-C", os.path.dirname(mk), "libtimeevolver.so", "examples/oscillators"],
+    r = subprocess.run(["make", "-s", "-C", os.path.dirname(mk), "libtimeevolver.so", "examples/oscillators",
+                        "examples/blackhole"],
                        capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("C++ layer build failed:\n%s%s" % (r.stdout, r.stderr))
