@@ -10,13 +10,15 @@ a "step" is one complete timeEvolve() from psi0 to T with all inputs resident in
 
 Prints one JSON line. `value` is the time-to-T in seconds (lower is better). `roofline` is
 the fused Lanczos SpMV kernel: algorithmic bytes per launch = the matrix bytes its storage
-format reads once (value dictionary: 6 per stored off-diagonal slot, sliced-ELL padding
-included, + row pointers or slice offsets and row lengths + 16 D diagonal; plain CSR:
-20 nnz + 4 (D+1)) + 16 D (x = v_j, read once, also for <v_j, w>) + 16 D
-(w written) + 16 D (v_{j-1}), divided by its mean duration from HIP events recorded around
-every launch on the library's stream during the timed runs. `csr_equivalent_GBps` prices the
-same launches at the reference's CSR bytes (20 nnz + 4 (D+1) + 48 D). `cpu_baseline` times the
-oracle's CPU restatement of the reference evolver on this host, on a bounded sample (the
+format reads once (sliced ELL: 4 per stored slot, padding included, + 4 per slot of a slice
+for its base column + slice offsets + 16 D diagonal; value-dictionary rows: 6 per off-diagonal
+entry + row pointers + 16 D; plain CSR: 20 nnz + 4 (D+1)) + 16 D (x = v_j, read once, also for
+<v_j, w>) + 16 D (w written) + 16 D (v_{j-1}), divided by its mean duration from HIP events
+recorded around every launch on the library's stream during the timed runs.
+`csr_equivalent_GBps` prices the same launches at the reference's CSR bytes
+(20 nnz + 4 (D+1) + 48 D); `spmv_unit` compares the launch time with the time the 60 %-of-peak
+bar allows for one CSR SpMV (20 nnz + 4 (D+1) + 32 D bytes at 4.8 TB/s). `cpu_baseline` times
+the oracle's CPU restatement of the reference evolver on this host, on a bounded sample (the
 first Krylov step of the same workload) and extrapolates to the full time-to-T.
 """
 import argparse
@@ -125,6 +127,7 @@ def main():
     b_spmv = b_matrix + 32 * D                      # y = A x: matrix, x once, y written
     b_apply = b_matrix + 48 * D                     # + v_{j-1}
     b_csr = 20 * H.nnz + 4 * (D + 1) + 48 * D
+    b_unit = 20 * H.nnz + 4 * (D + 1) + 32 * D      # one CSR SpMV, y = A x
     achieved = b_apply / t_apply / 1e9
     kname = {0: "te::lanczos_apply_kernel", 1: "te::lanczos_apply_stream_kernel",
              2: "te::lanczos_apply_vd_kernel", 3: "te::lanczos_apply_vd_kernel",
@@ -155,6 +158,9 @@ def main():
         "build_s": round(build_s, 3),
         "spmv_GBps": round(b_spmv / t_apply / 1e9, 1),
         "csr_equivalent_GBps": round(b_csr / t_apply / 1e9, 1),
+        "spmv_unit": {"bytes": b_unit, "ms_allowed_at_60pct_peak": round(b_unit / (0.6 * PEAK_HBM_GBS * 1e9) * 1e3, 4),
+                      "apply_ms_per_launch": round(t_apply * 1e3, 4),
+                      "within": bool(t_apply <= b_unit / (0.6 * PEAK_HBM_GBS * 1e9))},
         "storage": {"format": ["csr-lanes", "csr-rowblocks", "value-dictionary",
                                "value-dictionary, locality order",
                                "value-dictionary sliced ELL, locality order"][fmt], "n_values": n_values,

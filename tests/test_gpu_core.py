@@ -116,6 +116,37 @@ def _spmv_check(oracle_lib, M, D, v, c, r, seed=5):
         assert np.linalg.norm(y - yr) <= 1e-14 * np.linalg.norm(yr)
 
 
+def _aligned_slots(rows):
+    """Slots of the sliced-ELL layout: per slice of 64 rows (each row's off-diagonal columns
+    ascending), the next slot takes each row's next column if it lies within 65536 of the
+    smallest next column over the slice. Returns 64 * (number of slots)."""
+    total = 0
+    for s0 in range(0, len(rows), 64):
+        sl = rows[s0:s0 + 64]
+        head = [0] * len(sl)
+        while True:
+            nxt = [row[h] for row, h in zip(sl, head) if h < len(row)]
+            if not nxt:
+                break
+            lo = min(nxt)
+            head = [h + (h < len(row) and row[h] - lo < 65536) for row, h in zip(sl, head)]
+            total += 64
+    return total
+
+
+def _sell_rows(c, r, order):
+    """Off-diagonal columns of each row in the row order `order` (new -> old), columns renumbered
+    in the same order and ascending."""
+    inv = np.empty_like(order)
+    inv[order] = np.arange(len(order))
+    off = c != r
+    rr, cc = inv[r[off].astype(np.int64)], inv[c[off].astype(np.int64)]
+    idx = np.lexsort((cc, rr))
+    rr, cc = rr[idx], cc[idx]
+    bounds = np.searchsorted(rr, np.arange(len(order) + 1))
+    return [cc[bounds[i]:bounds[i + 1]].tolist() for i in range(len(order))]
+
+
 def _distinct_offdiag(v, c, r):
     off = c != r
     bits = np.ascontiguousarray(v[off]).view(np.uint64).reshape(-1, 2)
@@ -142,8 +173,9 @@ def test_value_dictionary_storage(gpu, oracle_lib, monkeypatch, L):
         assert fmt == want and nv == n_distinct
         assert nbytes == 6 * n_off + 4 * (ob.D + 1) + 16 * ob.D
         _spmv_check(oracle_lib, M, ob.D, v, c, r)
-    # sliced ELL (default): 64-row slices padded to their longest row, rows in the basis order
-    # (COO matrix) or in the lexicographic order of the states (built operator)
+    # sliced ELL (default): 64-row slices, rows in the basis order (COO matrix) or in the
+    # lexicographic order of the states (built operator); 4-byte words, a 4-byte base per slot
+    # of a slice; below 65536 rows a slice has as many slots as its longest row
     ns = (ob.D + 63) // 64
     rowlen = np.bincount(r[c != r].astype(np.int64), minlength=ob.D)
     states = ob.states().astype(np.int64)
@@ -155,7 +187,8 @@ def test_value_dictionary_storage(gpu, oracle_lib, monkeypatch, L):
         lens = np.zeros(ns * 64, np.int64)
         lens[:ob.D] = rowlen[order]
         slots = 64 * int(lens.reshape(ns, 64).max(1).sum())
-        assert nbytes == 6 * slots + 128 * ns + 4 * (ns + 1) + 16 * ob.D
+        assert slots == _aligned_slots(_sell_rows(c, r, order))
+        assert nbytes == 4 * slots + 4 * (slots // 64) + 4 * (ns + 1) + 16 * ob.D
         _spmv_check(oracle_lib, M, ob.D, v, c, r)
     monkeypatch.setenv("TE_SPMV_VD", "0")
     P = capi.Matrix.from_coo(gpu, v, c, r, ob.D)
@@ -170,6 +203,38 @@ def test_value_dictionary_storage(gpu, oracle_lib, monkeypatch, L):
         assert rv["n_steps"] == rp["n_steps"] and rv["statusCode"] == rp["statusCode"]
         assert np.linalg.norm(rv["state"] - rp["state"]) <= 1e-12
         assert abs(rv["err"] - rp["err"]) <= 1e-12
+
+
+def test_sliced_ell_slots_beyond_16_bit_span(gpu, oracle_lib):
+    """Rows whose far entries (70,000 and 140,000 columns away) are present in some rows of a
+    slice and not in others: the slots that hold them get padding in the other rows, since one
+    slot spans fewer than 65536 columns. Slot count as the alignment rule gives, larger than the
+    longest rows' count; the product matches the oracle's."""
+    rng = np.random.default_rng(7)
+    D = 150_000
+    table = rng.normal(size=40) + 1j * rng.normal(size=40)
+    rows, cols = [], []
+    i = np.arange(D)
+    for d, p in ((1, 1.0), (-1, 1.0), (3, 0.7), (70_000, 0.5), (-70_000, 0.5), (140_000, 0.3)):
+        keep = rng.random(D) < p
+        rows.append(i[keep])
+        cols.append((i[keep] + d) % D)
+    rows.append(i)
+    cols.append(i)
+    r = np.concatenate(rows)
+    c = np.concatenate(cols)
+    key = np.unique(r * D + c, return_index=True)[1]   # one entry per (row, column)
+    r, c = r[key].astype(np.uint64), c[key].astype(np.uint64)
+    v = table[rng.integers(0, len(table), len(r))]
+    A = capi.Matrix.from_coo(gpu, v, c, r, D)
+    fmt, nv, nbytes = A.storage()
+    assert fmt == 4 and nv <= len(table)
+    ns = (D + 63) // 64
+    slots = _aligned_slots(_sell_rows(c, r, np.arange(D)))
+    longest = 64 * int(np.bincount(r[c != r].astype(np.int64), minlength=ns * 64).reshape(ns, 64).max(1).sum())
+    assert slots > longest
+    assert nbytes == 4 * slots + 4 * (slots // 64) + 4 * (ns + 1) + 16 * D
+    _spmv_check(oracle_lib, A, D, v, c, r)
 
 
 def test_locality_order_evolution(gpu, oracle_lib, monkeypatch):

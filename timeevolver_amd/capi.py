@@ -189,7 +189,8 @@ class Matrix:
 
     def storage(self):
         """(format, n_values, stream_bytes): 0 CSR lanes-per-row, 1 CSR row blocks,
-        2 value-dictionary row blocks; bytes of matrix data one product reads."""
+        2 value-dictionary row blocks, 3 the same in the locality order, 4 value-dictionary
+        sliced ELL; bytes of matrix data one product reads."""
         f, nv, b = _c.c_int(), _c.c_size_t(), _c.c_size_t()
         self.ctx.check(self.ctx.L.te_matrix_storage(self.h, _c.byref(f), _c.byref(nv), _c.byref(b)), "storage")
         return f.value, nv.value, b.value

@@ -80,19 +80,20 @@ struct VdCsr {
     int* perm = nullptr;
     int* iperm = nullptr;
     // Sliced-ELL form of the same entries (when every row has at most 255 off-diagonal entries):
-    // slices of kSlice consecutive rows, each padded to its longest row and stored slot-major,
-    // slot j of row s*kSlice + l at sell_off[s] + j*kSlice + l, so a wavefront reads every slot
-    // of its 64 rows with one coalesced load. A row's entries left of the diagonal fill its
-    // first slots, those right of it its last slots, padding in between: the far neighbours of
-    // the slice's rows, which sit at the same offsets for rows that share their leading modes,
-    // then share slots and their gathers from x share cache lines. Optionally (TE_SELL_SIGMA=1)
-    // rows are ordered by decreasing length within windows of kSellWindow (part of perm): less
-    // padding, worse gathers.
+    // slices of kSlice consecutive rows stored slot-major, slot j of row s*kSlice + l at word
+    // sell_off[s] + j*kSlice + l, so a wavefront reads every slot of its 64 rows with one
+    // coalesced load. Each row's entries keep their column order; they are dealt to slots so
+    // that the columns of one slot (over the slice's rows) lie within kSellSpan of each other:
+    // slot j starts at column sell_base[sell_off[s] / kSlice + j], and a word holds the 16-bit
+    // offset from it and, above, the 16-bit value index (kSellPadIdx: no entry, not added; the
+    // offset is then 0, a column of the slot). The far neighbours of the slice's rows, which sit
+    // at the same offsets for rows that share their leading modes, share slots and their gathers
+    // share cache lines. Optionally (TE_SELL_SIGMA=1) rows are ordered by decreasing length
+    // within windows of kSellWindow (part of perm): less padding, worse gathers.
     bool sell = false;
     int* sell_off = nullptr;            // n_slices + 1
-    int* sell_col = nullptr;
-    unsigned short* sell_vidx = nullptr;
-    unsigned short* sell_lr = nullptr;  // n_slices * kSlice: left count | right count << 8
+    unsigned* sell_word = nullptr;      // sell_entries + kSellGroup * kSlice (zero padding)
+    int* sell_base = nullptr;           // sell_entries / kSlice + kSellGroup (zero padding)
     int n_slices = 0;
     int64_t sell_entries = 0;
     int sell_grid = 0;
@@ -102,13 +103,15 @@ struct VdCsr {
 constexpr int kSlice = 64;
 constexpr int kSellGroup = 4;       // slots a lane loads (and gathers) before using any
 constexpr int kSellWindow = 512;
+constexpr int kSellSpan = 1 << 16;  // columns one slot of a slice may span (16-bit offsets)
+constexpr unsigned kSellPadIdx = 0xffffu;   // value index of a padding slot
+static_assert(kVdMaxValues < (int)kSellPadIdx, "value indices must stay below the padding index");
 
 // What the sliced-ELL kernels read.
 struct SellView {
     const int* off;
-    const int* col;
-    const unsigned short* vidx;
-    const unsigned short* lr;
+    const unsigned* word;
+    const int* base;
     const double2* vtab;
     int n_values;
     const double2* diag;

@@ -8,20 +8,22 @@
 
 namespace te {
 
-__device__ __forceinline__ void sell_load_group(const int* cp, const unsigned short* vp, int* c, unsigned* v) {
+// A group of kSellGroup slots: the lane's words (streamed) and the slots' base columns (the same
+// for the whole wavefront: scalar loads).
+__device__ __forceinline__ void sell_load_group(const unsigned* wp, const int* bp, unsigned* w, int* b) {
 #pragma unroll
     for (int q = 0; q < kSellGroup; q++) {
-        c[q] = __builtin_nontemporal_load(cp + q * kSlice);
-        v[q] = __builtin_nontemporal_load(vp + q * kSlice);
+        w[q] = __builtin_nontemporal_load(wp + q * kSlice);
+        b[q] = bp[q];
     }
 }
 
 // Slice s of a sliced-ELL value-dictionary matrix by the calling wavefront (lane = row of the
 // slice): y_r = diag_r x_r + sum_j vt[vidx_j] x[col_j] (diagonal first, then column order: the
-// same sums as the row-block kernel), handed to epi.row. Slots past a row's length are loaded
-// (col 0, value 0) and not added. A group of kSellGroup slots is loaded before any is used, and the
-// next group's slots load while the current group's gathers are in flight. vt: the value table in
-// LDS.
+// same sums as the row-block kernel), handed to epi.row. Padding slots gather the slot's base
+// column and are not added. A group of kSellGroup slots is loaded before any is used, and the
+// next group's slots load while the current group's gathers are in flight. vt: the value table
+// in LDS.
 template <class Epi>
 __device__ __forceinline__ void sell_slice(const SellView& m, const double2* __restrict__ x, const double2* vt, int s,
                                            int lane, Epi& epi) {
@@ -29,8 +31,6 @@ __device__ __forceinline__ void sell_slice(const SellView& m, const double2* __r
     const int ls = (m.off[s + 1] - o) / kSlice;                   // slots of this slice
     const int ng = (ls + kSellGroup - 1) / kSellGroup;             // groups of slots
     const int64_t r = (int64_t)s * kSlice + lane;
-    const unsigned lr = m.lr[r];
-    const int nl = (int)(lr & 255u), rfirst = ls - (int)(lr >> 8);  // slots [0,nl) and [rfirst,ls)
     double2 xr = make_double2(0.0, 0.0), d = xr;
     typename Epi::Pre pre{};
     if (r < m.n) {
@@ -38,44 +38,45 @@ __device__ __forceinline__ void sell_slice(const SellView& m, const double2* __r
         d = load_stream(&m.diag[r]);
         pre = epi.load((int)r);
     }
-    const int* cp = m.col + o + lane;
-    const unsigned short* vp = m.vidx + o + lane;
-    int c[kSellGroup];
-    unsigned v[kSellGroup];
-    sell_load_group(cp, vp, c, v);
+    const unsigned* wp = m.word + o + lane;
+    const int* bp = m.base + o / kSlice;
+    unsigned w[kSellGroup];
+    int b[kSellGroup];
+    sell_load_group(wp, bp, w, b);
     double sr = d.x * xr.x - d.y * xr.y, si = d.x * xr.y + d.y * xr.x;
-    // Group g's gathers and group g+1's column loads are in flight together (the last group
-    // is peeled so that the loop body is straight-line code for the wait counters). Slots
-    // past the slice end (last group) read the next slice's entries or the zero padding: in
+    // Group g's gathers and group g+1's word loads are in flight together (the last group is
+    // peeled so that the loop body is straight-line code for the wait counters). Slots past the
+    // slice end (last group) read the next slice's words and bases or the zero padding: in
     // bounds, valid columns, never added.
-    auto accumulate = [&](const unsigned* vv, const double2* xv, int g) {
+    auto accumulate = [&](const unsigned* ww, const double2* xv, int g) {
 #pragma unroll
         for (int q = 0; q < kSellGroup; q++) {
-            const double2 p = cmul(vt[vv[q]], xv[q]);
-            const int j = kSellGroup * g + q;
-            const bool in = (j < nl) | ((j >= rfirst) & (j < ls));   // selects, not branches
+            const unsigned vi = ww[q] >> 16;
+            const bool in = (kSellGroup * g + q < ls) & (vi != kSellPadIdx);   // selects, not branches
+            const double2 p = cmul(vt[in ? vi : 0u], xv[q]);
             sr = in ? sr + p.x : sr;
             si = in ? si + p.y : si;
         }
     };
+    auto gather = [&](double2* xv) {
+#pragma unroll
+        for (int q = 0; q < kSellGroup; q++) xv[q] = x[b[q] + (int)(w[q] & 0xffffu)];
+    };
     for (int g = 0; g + 1 < ng; g++) {
         double2 xv[kSellGroup];
-        unsigned vv[kSellGroup];
+        unsigned ww[kSellGroup];
+        gather(xv);
 #pragma unroll
-        for (int q = 0; q < kSellGroup; q++) {
-            xv[q] = x[c[q]];
-            vv[q] = v[q];
-        }
-        cp += kSellGroup * kSlice;
-        vp += kSellGroup * kSlice;
-        sell_load_group(cp, vp, c, v);
-        accumulate(vv, xv, g);
+        for (int q = 0; q < kSellGroup; q++) ww[q] = w[q];
+        wp += kSellGroup * kSlice;
+        bp += kSellGroup;
+        sell_load_group(wp, bp, w, b);
+        accumulate(ww, xv, g);
     }
     if (ng > 0) {
         double2 xv[kSellGroup];
-#pragma unroll
-        for (int q = 0; q < kSellGroup; q++) xv[q] = x[c[q]];
-        accumulate(v, xv, ng - 1);
+        gather(xv);
+        accumulate(w, xv, ng - 1);
     }
     if (r < m.n) epi.row((int)r, make_double2(sr, si), xr, pre);
 }

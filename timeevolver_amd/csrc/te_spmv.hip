@@ -593,8 +593,8 @@ static VdView vd_view(const te_matrix* A) {
 
 static SellView sell_view(const te_matrix* A) {
     const VdCsr& v = A->vd;
-    return SellView{v.sell_off, v.sell_col, v.sell_vidx, v.sell_lr, v.vtab,       v.n_values,
-                    v.diag,     v.n_slices, A->n_rows,  v.sell_chunk};
+    return SellView{v.sell_off, v.sell_word, v.sell_base, v.vtab,        v.n_values,
+                    v.diag,     v.n_slices,  A->n_rows,  v.sell_chunk};
 }
 
 SellView sell_view_of(const te_matrix* A) { return sell_view(A); }
@@ -991,48 +991,52 @@ __global__ __launch_bounds__(kSellWindow) void sigma_sort_kernel(const int64_t* 
     out[w0 + rank] = base ? base[r] : (int)r;
 }
 
-__global__ void sell_len_kernel(const int* __restrict__ vptr, int64_t n, int n_slices, int64_t* slen) {
-    const int s2 = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s2 > n_slices) return;
-    if (s2 == n_slices) {
-        slen[s2] = 0;
+// Slot alignment of the slices, one wavefront per slice (lane l: row s*kSlice + l). The next
+// slot of a slice takes, from every row, the row's next entry (rows in column order) if that
+// lies within kSellSpan columns of the smallest next entry over the slice; rows whose next entry
+// lies further right get a padding word in it. Count pass (FILL false): slen[s] = the slice's
+// slots * kSlice. Fill pass: the words and slot bases at the offsets soff (exclusive scan of
+// slen). A slot's padding words point at its base column, a column of one of its entries.
+template <bool FILL>
+__global__ __launch_bounds__(256) void sell_align_kernel(const int* __restrict__ vptr, const int* __restrict__ vcol,
+                                                         const unsigned short* __restrict__ vvidx, int64_t n,
+                                                         int n_slices, int64_t* slen, const int64_t* __restrict__ soff,
+                                                         int* sell_off, unsigned* word, int* base) {
+    const int s = blockIdx.x * (blockDim.x / kSlice) + (int)(threadIdx.x / kSlice);
+    const int lane = threadIdx.x % kSlice;
+    if (s > n_slices) return;                                  // whole wavefronts
+    if (s == n_slices) {
+        if (!FILL && lane == 0) slen[s] = 0;
         return;
     }
-    int mx = 0;
-    for (int l = 0; l < kSlice; l++) {
-        const int64_t r = (int64_t)s2 * kSlice + l;
-        if (r < n) mx = max(mx, vptr[r + 1] - vptr[r]);
+    const int64_t r = (int64_t)s * kSlice + lane;
+    int k = r < n ? vptr[r] : 0;
+    const int kend = r < n ? vptr[r + 1] : 0;
+    int64_t o = 0;
+    if (FILL) {
+        o = soff[s];
+        if (lane == 0) {
+            sell_off[s] = (int)o;
+            if (s == n_slices - 1) sell_off[n_slices] = (int)soff[n_slices];
+        }
     }
-    slen[s2] = (int64_t)mx * kSlice;
-}
-
-// Row r of slice s: entries left of the diagonal in slots [0, nl), right of it in [L - nr, L),
-// both in column order; padding slots gather x[r] (col r, value index 0) and are not added.
-__global__ void sell_fill_kernel(const int* __restrict__ vptr, const int* __restrict__ vcol,
-                                 const unsigned short* __restrict__ vvidx, int64_t n, int n_slices,
-                                 const int64_t* __restrict__ soff, int* sell_off, int* scol, unsigned short* svidx,
-                                 unsigned short* lr) {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= (int64_t)n_slices * kSlice) return;
-    const int s2 = (int)(r / kSlice), l = (int)(r % kSlice);
-    if (l == 0) {
-        sell_off[s2] = (int)soff[s2];
-        if (s2 == n_slices - 1) sell_off[n_slices] = (int)soff[n_slices];
-    }
-    const int len = r < n ? vptr[r + 1] - vptr[r] : 0;
-    const int k0 = r < n ? vptr[r] : 0;
-    int nl = 0;
-    while (nl < len && vcol[k0 + nl] < (int)r) nl++;
-    const int nr = len - nl;
-    lr[r] = (unsigned short)(nl | (nr << 8));
-    const int64_t o = soff[s2];
-    const int L = (int)((soff[s2 + 1] - o) / kSlice);
-    const int pad_col = r < n ? (int)r : 0;
-    for (int j = 0; j < L; j++) {
-        const int64_t idx = o + (int64_t)j * kSlice + l;
-        const int k = j < nl ? j : (j >= L - nr ? nl + (j - (L - nr)) : -1);
-        scol[idx] = k >= 0 ? vcol[k0 + k] : pad_col;
-        svidx[idx] = k >= 0 ? vvidx[k0 + k] : (unsigned short)0;
+    for (int j = 0;; j++) {
+        const bool has = k < kend;
+        const int c = has ? vcol[k] : INT_MAX;
+        int lo = c;
+#pragma unroll
+        for (int d = kSlice / 2; d >= 1; d >>= 1) lo = min(lo, __shfl_xor(lo, d));
+        if (lo == INT_MAX) {                                   // the same for all lanes
+            if (!FILL && lane == 0) slen[s] = (int64_t)j * kSlice;
+            return;
+        }
+        const bool take = has && (unsigned)(c - lo) < (unsigned)kSellSpan;
+        if (FILL) {
+            word[o + (int64_t)j * kSlice + lane] =
+                take ? ((unsigned)vvidx[k] << 16) | (unsigned)(c - lo) : (kSellPadIdx << 16);
+            if (lane == 0) base[o / kSlice + j] = lo;
+        }
+        k += take ? 1 : 0;
     }
 }
 
@@ -1047,9 +1051,8 @@ void free_vd(te_matrix* A) {
     if (v.perm) hipFree(v.perm);
     if (v.iperm) hipFree(v.iperm);
     if (v.sell_off) hipFree(v.sell_off);
-    if (v.sell_col) hipFree(v.sell_col);
-    if (v.sell_vidx) hipFree(v.sell_vidx);
-    if (v.sell_lr) hipFree(v.sell_lr);
+    if (v.sell_word) hipFree(v.sell_word);
+    if (v.sell_base) hipFree(v.sell_base);
     v = VdCsr{};
     A->use_vd = false;
 }
@@ -1065,7 +1068,9 @@ static int build_sell(te_ctx* ctx, te_matrix* A) {
     int64_t *slen = nullptr, *soff = nullptr;
     TE_HIP(ctx, hipMalloc(&slen, sizeof(int64_t) * (ns + 1)));
     TE_HIP(ctx, hipMalloc(&soff, sizeof(int64_t) * (ns + 1)));
-    hipLaunchKernelGGL(sell_len_kernel, dim3((unsigned)((ns + 256) / 256)), dim3(256), 0, st, v.ptr, n, ns, slen);
+    const unsigned ga = (unsigned)((ns + 1 + 256 / kSlice - 1) / (256 / kSlice));
+    hipLaunchKernelGGL(sell_align_kernel<false>, dim3(ga), dim3(256), 0, st, v.ptr, v.col, v.vidx, n, ns, slen,
+                       nullptr, nullptr, nullptr, nullptr);
     int rc = exclusive_scan64(ctx, slen, soff, ns + 1);
     int64_t total = 0;
     if (rc == TE_OK) {
@@ -1074,28 +1079,30 @@ static int build_sell(te_ctx* ctx, te_matrix* A) {
     }
     if (rc == TE_OK && total < INT32_MAX) {
         TE_HIP(ctx, hipMalloc(&v.sell_off, sizeof(int) * (ns + 1)));
-        // a slice's last group of slots may run up to kSellGroup - 1 slots past its end: zero padding
+        // a slice's last group of slots may run up to kSellGroup - 1 slots past its end: zero
+        // padding (words: column base + 0, value index 0, never added)
         const int64_t pad = (int64_t)kSellGroup * kSlice;
-        TE_HIP(ctx, hipMalloc(&v.sell_col, sizeof(int) * (total + pad)));
-        TE_HIP(ctx, hipMalloc(&v.sell_vidx, sizeof(unsigned short) * (total + pad)));
-        TE_HIP(ctx, hipMemsetAsync(v.sell_col + total, 0, sizeof(int) * pad, st));
-        TE_HIP(ctx, hipMemsetAsync(v.sell_vidx + total, 0, sizeof(unsigned short) * pad, st));
-        TE_HIP(ctx, hipMalloc(&v.sell_lr, sizeof(unsigned short) * (size_t)ns * kSlice));
-        hipLaunchKernelGGL(sell_fill_kernel, dim3((unsigned)((ns64 * kSlice + 255) / 256)), dim3(256), 0, st, v.ptr,
-                           v.col, v.vidx, n, ns, soff, v.sell_off, v.sell_col, v.sell_vidx, v.sell_lr);
+        const int64_t nslots = total / kSlice;
+        TE_HIP(ctx, hipMalloc(&v.sell_word, sizeof(unsigned) * (total + pad)));
+        TE_HIP(ctx, hipMalloc(&v.sell_base, sizeof(int) * (nslots + kSellGroup)));
+        TE_HIP(ctx, hipMemsetAsync(v.sell_word + total, 0, sizeof(unsigned) * pad, st));
+        TE_HIP(ctx, hipMemsetAsync(v.sell_base + nslots, 0, sizeof(int) * kSellGroup, st));
+        hipLaunchKernelGGL(sell_align_kernel<true>, dim3(ga), dim3(256), 0, st, v.ptr, v.col, v.vidx, n, ns, slen,
+                           soff, v.sell_off, v.sell_word, v.sell_base);
         TE_HIP(ctx, hipStreamSynchronize(st));
         v.n_slices = ns;
         v.sell_entries = total;
-        // Two resident blocks per CU (8 waves): the waves of an XCD sweep its slices together,
-        // and a narrower window of slices in flight keeps more of x's lines in the XCD's L2.
-        // On the L = N = 14 ring 2 beat 1, 3 and 4 (the occupancy limit).
+        // Three resident blocks per CU (12 waves): the waves of an XCD sweep its slices together,
+        // and a narrow window of slices in flight keeps more of x's lines in the XCD's L2. On the
+        // L = N = 14 ring (apply, µs): 1296, 826, 806, 867, 885 for 1..5 blocks per CU (5 is the
+        // occupancy limit); groups of 8 slots in place of 4 were slower at every count.
         int per_cu = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lanczos_apply_sell_kernel, kBlock, 0) !=
                 hipSuccess ||
             per_cu < 1)
             per_cu = 1;
         const char* gmul = std::getenv("TE_SELL_BLOCKS_PER_CU");    // launch-shape sweeps
-        const int want = gmul ? std::max(1, std::atoi(gmul)) : 2;
+        const int want = gmul ? std::max(1, std::atoi(gmul)) : 3;
         const int cap = std::max(8, ctx->num_cus * std::min(want, per_cu));
         const int blocks = (ns + kBlock / 64 - 1) / (kBlock / 64);
         v.sell_grid = (int)((std::min(blocks, cap) + 7) / 8 * 8);
