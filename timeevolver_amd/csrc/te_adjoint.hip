@@ -54,10 +54,12 @@ __global__ void start_norm_kernel(const double2* res, KrylovScalars* ks, double*
 
 // Node states of one basis: out[g][r][q'] = sum_j U[j][r] coef[node0 + 8 g + q'][j] for the
 // nodes of this pass (zero past the step's node count; out holds the pass's blocks); end[r] =
-// sum_j U[j][r] cend[j] when end is given. LPR lanes share a row, lane h of it forming nodes
-// [16 h, 16 h + 16) of the pass: the basis is read once per pass of up to 16 LPR nodes (the
-// lanes of a row load the same address), the coefficients are LDS broadcasts, and a lane writes
-// two 128-byte blocks.
+// sum_j U[j][r] cend[j] when end is given. A wavefront forms one 8-node block for a tile of 256
+// rows, kMatRows rows a lane: every coefficient is an LDS broadcast (one address for the whole
+// wavefront) that feeds 4 kMatRows FP64 FMAs, and the basis rows load two columns ahead. The
+// blocks of one tile go to neighbouring wavefronts, which read the same basis rows together
+// (the pass re-reads the basis once per block, from L2: splitting a block over two wavefronts
+// doubled that and was slower). A lane writes one 128-byte block per row.
 struct MatArgs {
     const double2* U;
     int64_t ld, n;
@@ -70,11 +72,12 @@ struct MatArgs {
     double2* end;
 };
 
-constexpr int kLaneNodes = 16;     // nodes formed by one lane
+constexpr int kMatRows = 4;        // rows formed by one lane
+constexpr int kMatTile = 64 * kMatRows;
 
-template <int LPR>
+template <int NG>                  // node groups in the pass (8 NG nodes)
 __global__ __launch_bounds__(256) void materialise_nodes_kernel(MatArgs a) {
-    constexpr int NQ = LPR * kLaneNodes;
+    constexpr int NQ = NG * kNodeGroup;
     __shared__ double2 cs[kMaxKrylov * (NQ + 1)];        // cs[j * (NQ + 1) + q], q = NQ: cend
     const int nn = a.step->n_nodes;
     const int nloc = min(NQ, nn - a.node0);
@@ -86,47 +89,79 @@ __global__ __launch_bounds__(256) void materialise_nodes_kernel(MatArgs a) {
         cs[i] = c;
     }
     __syncthreads();
-    const int h = threadIdx.x % LPR;
     const int ng = (nloc + kNodeGroup - 1) / kNodeGroup;        // blocks written by this pass
-    const int64_t rows_per_grid = (int64_t)gridDim.x * (blockDim.x / LPR);
-    for (int64_t r = (int64_t)blockIdx.x * (blockDim.x / LPR) + threadIdx.x / LPR; r < a.n; r += rows_per_grid) {
-        double2 acc[kLaneNodes];
+    const int lane = threadIdx.x & 63;
+    const int waves = blockDim.x >> 6;
+    const int64_t items = (a.n + kMatTile - 1) / kMatTile * NG;   // (tile, block) pairs
+    for (int64_t it = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * waves + (threadIdx.x >> 6))); it < items;
+         it += (int64_t)gridDim.x * waves) {
+        const int64_t tile = it / NG;
+        const int g = (int)(it % NG);
+        const bool with_end = a.end && g == 0;
+        if (g >= ng && !with_end) continue;
+        int r[kMatRows];                                   // rows < 2^31 (te_adjoint_create)
 #pragma unroll
-        for (int q = 0; q < kLaneNodes; q++) acc[q] = make_double2(0.0, 0.0);
-        double2 ae = make_double2(0.0, 0.0);
+        for (int i = 0; i < kMatRows; i++) r[i] = (int)(tile * kMatTile) + i * 64 + lane;
+        double2 acc[kMatRows][kNodeGroup], ae[kMatRows];
+#pragma unroll
+        for (int i = 0; i < kMatRows; i++) {
+            ae[i] = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int q = 0; q < kNodeGroup; q++) acc[i][q] = make_double2(0.0, 0.0);
+        }
+        auto load_u = [&](int j, double2* u) {
+#pragma unroll
+            for (int i = 0; i < kMatRows; i++)
+                u[i] = (j < a.m && r[i] < a.n) ? a.U[(size_t)j * a.ld + r[i]] : make_double2(0.0, 0.0);
+        };
+        double2 u0[kMatRows], u1[kMatRows];
+        load_u(0, u0);
+        load_u(1, u1);
         for (int j = 0; j < a.m; j++) {
-            const double2 u = a.U[(size_t)j * a.ld + r];
-            const double2* cj = cs + j * (NQ + 1) + h * kLaneNodes;
+            double2 u2[kMatRows];
+            load_u(j + 2, u2);
+            const double2* cj = cs + j * (NQ + 1) + g * kNodeGroup;
 #pragma unroll
-            for (int q = 0; q < kLaneNodes; q++) {
+            for (int q = 0; q < kNodeGroup; q++) {
                 const double2 c = cj[q];
-                acc[q].x += u.x * c.x - u.y * c.y;
-                acc[q].y += u.x * c.y + u.y * c.x;
+#pragma unroll
+                for (int i = 0; i < kMatRows; i++) {
+                    acc[i][q].x += u0[i].x * c.x - u0[i].y * c.y;
+                    acc[i][q].y += u0[i].x * c.y + u0[i].y * c.x;
+                }
             }
-            if (a.end && h == 0) {
+            if (with_end) {
                 const double2 c = cs[j * (NQ + 1) + NQ];
-                ae.x += u.x * c.x - u.y * c.y;
-                ae.y += u.x * c.y + u.y * c.x;
+#pragma unroll
+                for (int i = 0; i < kMatRows; i++) {
+                    ae[i].x += u0[i].x * c.x - u0[i].y * c.y;
+                    ae[i].y += u0[i].x * c.y + u0[i].y * c.x;
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < kMatRows; i++) {
+                u0[i] = u1[i];
+                u1[i] = u2[i];
             }
         }
 #pragma unroll
-        for (int gl = 0; gl < kLaneNodes / kNodeGroup; gl++) {
-            const int g = h * (kLaneNodes / kNodeGroup) + gl;
+        for (int i = 0; i < kMatRows; i++) {
+            if (r[i] >= a.n) continue;
             if (g < ng) {
-                double2* o = a.out + ((size_t)g * a.n + r) * kNodeGroup;
+                double2* o = a.out + ((size_t)g * a.n + r[i]) * kNodeGroup;
 #pragma unroll
-                for (int q = 0; q < kNodeGroup; q++) o[q] = acc[gl * kNodeGroup + q];
+                for (int q = 0; q < kNodeGroup; q++) o[q] = acc[i][q];
             }
+            if (with_end) a.end[r[i]] = ae[i];
         }
-        if (a.end && h == 0) a.end[r] = ae;
     }
 }
 
-// Lanes per row for a pass of `nodes` nodes (at most kMatNodes).
+// A pass of `nodes` nodes (at most kMatNodes): 2, 4 or 8 groups of 8.
 static void launch_materialise(const MatArgs& m, int nodes, int grid, hipStream_t s) {
-    if (nodes <= kLaneNodes) hipLaunchKernelGGL(materialise_nodes_kernel<1>, dim3(grid), dim3(256), 0, s, m);
-    else if (nodes <= 2 * kLaneNodes) hipLaunchKernelGGL(materialise_nodes_kernel<2>, dim3(grid), dim3(256), 0, s, m);
-    else hipLaunchKernelGGL(materialise_nodes_kernel<4>, dim3(grid), dim3(256), 0, s, m);
+    if (nodes <= 2 * kNodeGroup) hipLaunchKernelGGL(materialise_nodes_kernel<2>, dim3(grid), dim3(256), 0, s, m);
+    else if (nodes <= 4 * kNodeGroup) hipLaunchKernelGGL(materialise_nodes_kernel<4>, dim3(grid), dim3(256), 0, s, m);
+    else hipLaunchKernelGGL(materialise_nodes_kernel<8>, dim3(grid), dim3(256), 0, s, m);
 }
 
 // One group of 8 nodes: for every row r, L_q = w_q conj(lambda_q(r)),
@@ -323,7 +358,8 @@ struct te_adjoint {
     // forward
     std::vector<double2*> bases;    // K = 0: one (m0 + 1) x n basis per step
     double2* U = nullptr;           // K >= 1
-    std::vector<double2*> ckpt;     // K >= 1: psi(t_k), k % K == 0
+    std::vector<double2*> ckpt;     // K >= 1: psi(t_k), k % K == 0 (kept across runs, grown as needed)
+    size_t n_ckpt = 0;              // checkpoints written by the last forward
     double2* psiT = nullptr;
     double2* scratch = nullptr;     // currentVec of the last step
     double2* psi0 = nullptr;
@@ -453,6 +489,8 @@ extern "C" int te_adjoint_create(te_ctx* ctx, te_matrix* H, int n_params, te_mat
     if (n_params > kAdjMaxLabels)
         return fail(ctx, TE_ERR_UNSUPPORTED, "at most %d generators", kAdjMaxLabels);
     if (H->n_rows != H->n_cols || H->n_rows == 0) return fail(ctx, TE_ERR_FATAL, "Invalid Hilbertspace dimension");
+    if (H->n_rows > (size_t)INT32_MAX - 1024)                 // row indices of the node passes are 32-bit
+        return fail(ctx, TE_ERR_UNSUPPORTED, "dimension above 2^31 - 1024");
     if (!observable->diagonal || observable->n_rows != H->n_rows)
         return fail(ctx, TE_ERR_UNSUPPORTED, "the loss observable must be a diagonal matrix of H's dimension");
     if (!(params->t > 0)) return fail(ctx, TE_ERR_ARG, "t must be > 0");
@@ -676,8 +714,7 @@ extern "C" void te_adjoint_destroy(te_adjoint* a) { te::adj_free(a); }
 
 namespace te {
 
-// One Lanczos recurrence of m iterations on U from U[:,0] (normalised by s_0 in ks).
-static void lanczos(te_adjoint* a, double2* U, KrylovScalars* ks, int m, double2 alpha) {
+static LanczosArgs lanczos_args(te_adjoint* a, double2* U, KrylovScalars* ks, int m, double2 alpha) {
     LanczosArgs la{};
     la.n = a->n;
     la.row_block = a->H->row_block;
@@ -693,10 +730,51 @@ static void lanczos(te_adjoint* a, double2* U, KrylovScalars* ks, int m, double2
     la.ks = ks;
     la.partials = a->rs.partials;
     la.ticket = a->rs.ticket;
+    return la;
+}
+
+// One Lanczos recurrence of m iterations on U from U[:,0] (normalised by s_0 in ks).
+static void lanczos(te_adjoint* a, double2* U, KrylovScalars* ks, int m, double2 alpha) {
+    LanczosArgs la = lanczos_args(a, U, ks, m, alpha);
     for (int j = 0; j < m; j++) {
         la.j = j;
         launch_lanczos_apply(la, a->H, a->ctx->stream);
         launch_lanczos_project(la, a->pgrid, a->ctx->stream);
+    }
+}
+
+// Two recurrences of m iterations with the same alpha in lockstep (U1 with ks1, U2 with ks2):
+// each iteration's applies share one pass over H, their projections one launch. The second
+// recurrence's block partials sit after the first's in the reduction scratch.
+static void lanczos2(te_adjoint* a, double2* U1, KrylovScalars* ks1, double2* U2, KrylovScalars* ks2, int m,
+                     double2 alpha) {
+    LanczosArgs la = lanczos_args(a, U1, ks1, m, alpha);
+    LanczosArgs lb = lanczos_args(a, U2, ks2, m, alpha);
+    lb.partials = a->rs.partials + (size_t)a->rs.grid * 32;
+    lb.ticket = a->rs.ticket + 1;
+    // launch shapes of the paired kernels (blocks per CU; launch-shape sweeps:
+    // TE_ADJ_APPLY2_BLOCKS_PER_CU, TE_ADJ_PROJECT2_BLOCKS_PER_CU)
+    // (on the L = N = 12 ring, K = 1: 2 and 1 took 74.3 ms a run, 3 and 2 (the single kernels'
+    // shapes) 78.5 ms, 4 and more blocks per CU were slower still)
+    static const int apply_per_cu = [] {
+        const char* v = std::getenv("TE_ADJ_APPLY2_BLOCKS_PER_CU");
+        return v ? std::max(0, std::atoi(v)) : 2;
+    }();
+    static const int project_per_cu = [] {
+        const char* v = std::getenv("TE_ADJ_PROJECT2_BLOCKS_PER_CU");
+        return v ? std::max(1, std::atoi(v)) : 1;
+    }();
+    const int cap = 32 * a->rs.grid;                  // each vector's partials region
+    auto shape = [&](int per_cu, int64_t items) {
+        const int64_t g = std::min<int64_t>((int64_t)a->ctx->num_cus * per_cu, std::max<int64_t>(items, 8));
+        return (int)std::min<int64_t>(cap / 8 * 8, (g + 7) / 8 * 8);
+    };
+    const int agrid = apply_per_cu > 0 ? shape(apply_per_cu, ((int64_t)a->H->vd.n_slices + 3) / 4) : 0;
+    const int pgrid = shape(project_per_cu, (a->n + kBlock - 1) / kBlock);
+    for (int j = 0; j < m; j++) {
+        la.j = lb.j = j;
+        launch_lanczos_apply2(la, lb, a->H, agrid, a->ctx->stream);
+        launch_lanczos_project2(la, lb, pgrid, a->ctx->stream);
     }
 }
 
@@ -731,8 +809,7 @@ static int forward(te_adjoint* a, te_result* R) {
     C.trace_cap = kAdjMaxSteps;
     a->t_steps.clear();
     a->m_steps.clear();
-    for (double2* p : a->ckpt) hipFree(p);
-    a->ckpt.clear();
+    a->n_ckpt = 0;
     auto basis = [&](size_t k) -> double2* {
         if (a->K >= 1) return a->U;
         while (a->bases.size() <= k) {
@@ -776,10 +853,12 @@ static int forward(te_adjoint* a, te_result* R) {
     while (h.index_samples < n_samples) {
         if (k >= (size_t)kAdjMaxSteps) return fail(ctx, TE_ERR_UNSUPPORTED, "more than %d Krylov steps", kAdjMaxSteps);
         if (a->K >= 1 && k % a->K == 0) {                 // checkpoint psi(t_k)
-            double2* c = nullptr;
-            TE_HIP(ctx, hipMalloc(&c, sizeof(double2) * n));
-            a->ckpt.push_back(c);
-            TE_HIP(ctx, hipMemcpyAsync(c, U, sizeof(double2) * n, hipMemcpyDeviceToDevice, s));
+            if (a->n_ckpt == a->ckpt.size()) {
+                double2* c = nullptr;
+                TE_HIP(ctx, hipMalloc(&c, sizeof(double2) * n));
+                a->ckpt.push_back(c);
+            }
+            TE_HIP(ctx, hipMemcpyAsync(a->ckpt[a->n_ckpt++], U, sizeof(double2) * n, hipMemcpyDeviceToDevice, s));
         }
         lanczos(a, U, a->ks, m, alpha);
         launch_step_control(C, a->st, a->ks, s);
@@ -847,8 +926,23 @@ static int backward(te_adjoint* a) {
     a->info.nodes = 0;
     a->info.max_phase = 0;
     a->info.capped = 0;
-    const int mg = std::max(8, std::min<int>((int)((n + 255) / 256), ctx->num_cus * 8)) / 8 * 8;
-    const int tg = a->grid;
+    // launch shapes of the node passes (blocks per CU; launch-shape sweeps: TE_ADJ_MAT_BLOCKS_PER_CU,
+    // TE_ADJ_TERM_BLOCKS_PER_CU, 0: the streaming kernels' grid, also the largest). On the
+    // L = N = 12 ring, K = 1: 2 blocks per CU for the term-wise pass took 72.4 ms a run against
+    // 74.0 ms with the streaming grid (4); 2, 4 or 8 for the materialisation within 0.1 ms.
+    static const int mat_per_cu = [] {
+        const char* v = std::getenv("TE_ADJ_MAT_BLOCKS_PER_CU");
+        return v ? std::max(1, std::atoi(v)) : 8;
+    }();
+    static const int term_per_cu = [] {
+        const char* v = std::getenv("TE_ADJ_TERM_BLOCKS_PER_CU");
+        return v ? std::max(0, std::atoi(v)) : 2;
+    }();
+    const int mg = std::max(8, std::min<int>((int)((n + 255) / 256), ctx->num_cus * mat_per_cu)) / 8 * 8;
+    const int tg = term_per_cu > 0            // at most a->grid: the size of the block partials
+                       ? std::min(a->grid, std::max(8, std::min<int>((int)((n + 255) / 256),
+                                                                     ctx->num_cus * term_per_cu)) / 8 * 8)
+                       : a->grid;
     // nodes per materialisation pass (launch-shape sweeps: TE_ADJ_PASS_NODES = 16, 32 or 64)
     static const int pass_nodes = [] {
         const char* v = std::getenv("TE_ADJ_PASS_NODES");
@@ -857,9 +951,19 @@ static int backward(te_adjoint* a) {
     }();
     for (int k = nsteps - 1; k >= 0; k--) {
         const double hstep = a->t_steps[k];
-        // lambda: Lanczos of +i e H from lambda(t_{k+1})
+        // lambda: Lanczos of +i e H from lambda(t_{k+1}); with K >= 1 also psi's, in lockstep
         start_norm(a, a->Vl, a->ksl, a->scale + 1);
-        lanczos(a, a->Vl, a->ksl, a->m0, alpha_b);
+        if (a->K == 0) {
+            lanczos(a, a->Vl, a->ksl, a->m0, alpha_b);
+        } else {
+            // psi(t_{k+1}): the forward checkpoint when one is kept there, else the state
+            // back-evolved by the previous backward step
+            if (k + 1 < nsteps && (k + 1) % a->K == 0)
+                TE_HIP(ctx, hipMemcpyAsync(a->Vp, a->ckpt[(k + 1) / a->K], sizeof(double2) * n,
+                                           hipMemcpyDeviceToDevice, s));
+            start_norm(a, a->Vp, a->ksp, a->scale);
+            lanczos2(a, a->Vl, a->ksl, a->Vp, a->ksp, a->m0, alpha_b);
+        }
         AdjCoefArgs ca{};
         ca.ks[1] = a->ksl;
         ca.m[1] = a->m0;          // replaced by m_hbd below when the recurrence broke down
@@ -873,13 +977,6 @@ static int backward(te_adjoint* a) {
             ca.from_end[0] = 0;
             ca.scale[0] = nullptr;
         } else {
-            // psi(t_{k+1}): the forward checkpoint when one is kept there, else the state
-            // back-evolved by the previous backward step
-            if (k + 1 < nsteps && (k + 1) % a->K == 0)
-                TE_HIP(ctx, hipMemcpyAsync(a->Vp, a->ckpt[(k + 1) / a->K], sizeof(double2) * n,
-                                           hipMemcpyDeviceToDevice, s));
-            start_norm(a, a->Vp, a->ksp, a->scale);
-            lanczos(a, a->Vp, a->ksp, a->m0, alpha_b);
             Upsi = a->Vp;
             ca.ks[0] = a->ksp;
             ca.m[0] = a->m0;

@@ -153,6 +153,12 @@ struct BatchCtl {
 
 void launch_lanczos_apply(LanczosArgs a, const te_matrix* A, hipStream_t s);
 void launch_lanczos_project(const LanczosArgs& a, int grid, hipStream_t s);
+// Iteration j of two recurrences on the same matrix (a.j == b.j, same n and m; own bases,
+// scalars and partials, a's ticket): one pass over the matrix words for both when A is held as
+// sliced ELL (grid blocks, a multiple of 8; 0: the single apply's grid), else one apply after
+// the other. Results as from two single launches of the same grid.
+void launch_lanczos_apply2(LanczosArgs a, LanczosArgs b, const te_matrix* A, int grid, hipStream_t s);
+void launch_lanczos_project2(const LanczosArgs& a, const LanczosArgs& b, int grid, hipStream_t s);
 void launch_step_control(const StepConsts& c, StepState* st, KrylovScalars* ks, hipStream_t s);
 // One block per instance (c, st, ks, bc: n_inst entries each, device memory).
 void launch_batch_step_control(const StepConsts* c, StepState* st, KrylovScalars* ks, BatchCtl* bc, int n_inst,

@@ -19,51 +19,72 @@ __device__ __forceinline__ void sell_load_group(const unsigned* wp, const int* b
 }
 
 // Slice s of a sliced-ELL value-dictionary matrix by the calling wavefront (lane = row of the
-// slice): y_r = diag_r x_r + sum_j vt[vidx_j] x[col_j] (diagonal first, then column order: the
-// same sums as the row-block kernel), handed to epi.row. Padding slots gather the slot's base
-// column and are not added. A group of kSellGroup slots is loaded before any is used, and the
-// next group's slots load while the current group's gathers are in flight. vt: the value table
-// in LDS.
-template <class Epi>
-__device__ __forceinline__ void sell_slice(const SellView& m, const double2* __restrict__ x, const double2* vt, int s,
-                                           int lane, Epi& epi) {
+// slice), applied to NV vectors x[v] at once (one pass over the slice's words): y_r = diag_r x_r
+// + sum_j vt[vidx_j] x[col_j] (diagonal first, then column order: the same sums as the
+// row-block kernel, and for each vector the same as a pass of its own), handed to epi[v].row.
+// Padding slots gather the slot's base column and are not added. A group of kSellGroup slots is
+// loaded before any is used, and the next group's slots load while the current group's gathers
+// are in flight. vt: the value table in LDS.
+template <int NV, class Epi>
+__device__ __forceinline__ void sell_slice_n(const SellView& m, const double2* const* x, const double2* vt, int s,
+                                             int lane, Epi* epi) {
     const int o = m.off[s];
     const int ls = (m.off[s + 1] - o) / kSlice;                   // slots of this slice
     const int ng = (ls + kSellGroup - 1) / kSellGroup;             // groups of slots
     const int64_t r = (int64_t)s * kSlice + lane;
-    double2 xr = make_double2(0.0, 0.0), d = xr;
-    typename Epi::Pre pre{};
+    double2 xr[NV], d = make_double2(0.0, 0.0);
+    typename Epi::Pre pre[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        xr[v] = make_double2(0.0, 0.0);
+        pre[v] = typename Epi::Pre{};
+    }
     if (r < m.n) {
-        xr = x[r];
+#pragma unroll
+        for (int v = 0; v < NV; v++) xr[v] = x[v][r];
         d = load_stream(&m.diag[r]);
-        pre = epi.load((int)r);
+#pragma unroll
+        for (int v = 0; v < NV; v++) pre[v] = epi[v].load((int)r);
     }
     const unsigned* wp = m.word + o + lane;
     const int* bp = m.base + o / kSlice;
     unsigned w[kSellGroup];
     int b[kSellGroup];
     sell_load_group(wp, bp, w, b);
-    double sr = d.x * xr.x - d.y * xr.y, si = d.x * xr.y + d.y * xr.x;
+    double sr[NV], si[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        sr[v] = d.x * xr[v].x - d.y * xr[v].y;
+        si[v] = d.x * xr[v].y + d.y * xr[v].x;
+    }
     // Group g's gathers and group g+1's word loads are in flight together (the last group is
     // peeled so that the loop body is straight-line code for the wait counters). Slots past the
     // slice end (last group) read the next slice's words and bases or the zero padding: in
     // bounds, valid columns, never added.
-    auto accumulate = [&](const unsigned* ww, const double2* xv, int g) {
+    auto accumulate = [&](const unsigned* ww, const double2 (*xv)[kSellGroup], int g) {
 #pragma unroll
         for (int q = 0; q < kSellGroup; q++) {
             const unsigned vi = ww[q] >> 16;
             const bool in = (kSellGroup * g + q < ls) & (vi != kSellPadIdx);   // selects, not branches
-            const double2 p = cmul(vt[in ? vi : 0u], xv[q]);
-            sr = in ? sr + p.x : sr;
-            si = in ? si + p.y : si;
+            const double2 a = vt[in ? vi : 0u];
+#pragma unroll
+            for (int v = 0; v < NV; v++) {
+                const double2 p = cmul(a, xv[v][q]);
+                sr[v] = in ? sr[v] + p.x : sr[v];
+                si[v] = in ? si[v] + p.y : si[v];
+            }
         }
     };
-    auto gather = [&](double2* xv) {
+    auto gather = [&](double2 (*xv)[kSellGroup]) {
 #pragma unroll
-        for (int q = 0; q < kSellGroup; q++) xv[q] = x[b[q] + (int)(w[q] & 0xffffu)];
+        for (int q = 0; q < kSellGroup; q++) {
+            const int c = b[q] + (int)(w[q] & 0xffffu);
+#pragma unroll
+            for (int v = 0; v < NV; v++) xv[v][q] = x[v][c];
+        }
     };
     for (int g = 0; g + 1 < ng; g++) {
-        double2 xv[kSellGroup];
+        double2 xv[NV][kSellGroup];
         unsigned ww[kSellGroup];
         gather(xv);
 #pragma unroll
@@ -74,11 +95,22 @@ __device__ __forceinline__ void sell_slice(const SellView& m, const double2* __r
         accumulate(ww, xv, g);
     }
     if (ng > 0) {
-        double2 xv[kSellGroup];
+        double2 xv[NV][kSellGroup];
         gather(xv);
         accumulate(w, xv, ng - 1);
     }
-    if (r < m.n) epi.row((int)r, make_double2(sr, si), xr, pre);
+    if (r < m.n) {
+#pragma unroll
+        for (int v = 0; v < NV; v++) epi[v].row((int)r, make_double2(sr[v], si[v]), xr[v], pre[v]);
+    }
+}
+
+// One vector: sell_slice_n with NV = 1.
+template <class Epi>
+__device__ __forceinline__ void sell_slice(const SellView& m, const double2* __restrict__ x, const double2* vt, int s,
+                                           int lane, Epi& epi) {
+    const double2* xs[1] = {x};
+    sell_slice_n<1>(m, xs, vt, s, lane, &epi);
 }
 
 // Lanczos apply epilogue on one row: w = c1 (A u_j)_r + c2 u_{j-1}[r] (non-temporal store),

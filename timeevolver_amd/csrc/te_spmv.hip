@@ -124,10 +124,10 @@ __global__ __launch_bounds__(kBlock) void lanczos_apply_kernel(LanczosArgs a) {
     }
 }
 
-__global__ __launch_bounds__(kBlock) void lanczos_project_kernel(LanczosArgs a) {
-    KrylovScalars* ks = a.ks;
-    if (ks->broken) return;
-    __shared__ double2 sh[kBlock / 64];
+// Rows [lo, hi) of the projection w' = w - c3 u_j (stored unless j + 1 == m); returns their
+// contribution to |w'|^2.
+__device__ __forceinline__ double project_rows(const LanczosArgs& a, int64_t lo, int64_t hi) {
+    const KrylovScalars* ks = a.ks;
     const int j = a.j;
     const double sj = ks->s[j];
     const double2 aj = ks->a[j];
@@ -135,8 +135,6 @@ __global__ __launch_bounds__(kBlock) void lanczos_project_kernel(LanczosArgs a) 
     const double2* __restrict__ uj = a.U + (size_t)j * a.ld;
     double2* __restrict__ w = a.U + (size_t)(j + 1) * a.ld;
     const bool store = j + 1 < a.m;
-    int64_t lo, hi;
-    block_rows(a.n, lo, hi);
     double nrm = 0.0;
     int64_t row = lo + threadIdx.x;
     // four rows per thread in flight: all eight loads are issued before any is used
@@ -163,22 +161,65 @@ __global__ __launch_bounds__(kBlock) void lanczos_project_kernel(LanczosArgs a) 
         if (store) w[row] = wr;
         nrm += wr.x * wr.x + wr.y * wr.y;
     }
+    return nrm;
+}
+
+// The last block's finish of a projection from beta^2 = tot.x.
+__device__ __forceinline__ void project_finish(const LanczosArgs& a, double2 tot) {
+    KrylovScalars* ks = a.ks;
+    const int j = a.j;
+    const double beta = sqrt(tot.x);
+    if (beta < a.tolRate) {            // lucky breakdown (core/krylovTimeEvolver.cpp:551-555)
+        ks->broken = 1;
+        ks->m_hbd = j + 1;
+        ks->h = beta;
+    } else if (j + 1 == a.m) {
+        ks->h = beta;                  // :566-567
+    } else {
+        ks->beta[j] = beta;            // :558-565
+        ks->s[j + 1] = 1.0 / beta;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void lanczos_project_kernel(LanczosArgs a) {
+    if (a.ks->broken) return;
+    __shared__ double2 sh[kBlock / 64];
+    int64_t lo, hi;
+    block_rows(a.n, lo, hi);
+    const double nrm = project_rows(a, lo, hi);
     double2 bs = block_sum2<kBlock>(make_double2(nrm, 0.0), sh);
     if (threadIdx.x == 0) a.partials[blockIdx.x] = bs;
     if (last_block_arrive(a.ticket)) {
         double2 tot = reduce_partials<kBlock>(a.partials, gridDim.x, 1, sh);
         if (threadIdx.x == 0) {
-            const double beta = sqrt(tot.x);
-            if (beta < a.tolRate) {            // lucky breakdown (core/krylovTimeEvolver.cpp:551-555)
-                ks->broken = 1;
-                ks->m_hbd = j + 1;
-                ks->h = beta;
-            } else if (j + 1 == a.m) {
-                ks->h = beta;                  // :566-567
-            } else {
-                ks->beta[j] = beta;            // :558-565
-                ks->s[j + 1] = 1.0 / beta;
-            }
+            project_finish(a, tot);
+            *a.ticket = 0;
+        }
+    }
+}
+
+// The projections of two recurrences in one launch (a and b: own partials; a's ticket); one that
+// broke down is skipped. Each gets the row ranges, partials and reduction order of its own launch.
+__global__ __launch_bounds__(kBlock) void lanczos_project2_kernel(LanczosArgs a, LanczosArgs b) {
+    const bool la = !a.ks->broken, lb = !b.ks->broken;
+    if (!la && !lb) return;
+    __shared__ double2 sh[kBlock / 64];
+    int64_t lo, hi;
+    block_rows(a.n, lo, hi);
+    const double n0 = la ? project_rows(a, lo, hi) : 0.0;
+    const double n1 = lb ? project_rows(b, lo, hi) : 0.0;
+    const double2 bs0 = block_sum2<kBlock>(make_double2(n0, 0.0), sh);
+    const double2 bs1 = block_sum2<kBlock>(make_double2(n1, 0.0), sh);
+    if (threadIdx.x == 0) {
+        a.partials[blockIdx.x] = bs0;
+        b.partials[blockIdx.x] = bs1;
+    }
+    if (last_block_arrive(a.ticket)) {
+        const double2 t0 = reduce_partials<kBlock>(a.partials, gridDim.x, 1, sh);
+        const double2 t1 = reduce_partials<kBlock>(b.partials, gridDim.x, 1, sh);
+        if (threadIdx.x == 0) {
+            if (la) project_finish(a, t0);
+            if (lb) project_finish(b, t1);
             *a.ticket = 0;
         }
     }
@@ -446,14 +487,13 @@ __global__ __launch_bounds__(kBlock) void lanczos_apply_vd_kernel(LanczosArgs a)
 }
 
 // ---------------------------------------------------------------------------------------
-// Sliced-ELL value-dictionary kernels: one lane per row, a wavefront per slice of kSlice rows.
-// Row r: y_r = diag_r x_r + sum_j vtab[vidx_j] x[col_j] (diagonal first, then column order: the
-// same sums as the row-block kernel). Slots past a row's length are loaded (col 0, value 0) and
-// not added. Four slots are loaded before any is used, and the wavefronts of a CU work on
-// different slices, so gathers from x overlap without LDS staging or barriers.
+// Sliced-ELL value-dictionary kernels: one lane per row, a wavefront per slice of kSlice rows
+// (sell_slice_n). The sweep stages the value table in LDS and hands each wavefront its slices
+// in the schedule order; the wavefronts of a CU work on different slices, so gathers from x
+// overlap without LDS staging or barriers.
 // ---------------------------------------------------------------------------------------
-template <class Epi>
-__device__ __forceinline__ void sell_rows(const SellView& m, const double2* __restrict__ x, double2* vt, Epi& epi) {
+template <class Slice>
+__device__ __forceinline__ void sell_sweep(const SellView& m, double2* vt, Slice&& slice) {
     for (int i = threadIdx.x; i < m.n_values; i += blockDim.x) vt[i] = m.vtab[i];
     __syncthreads();
     const int lane = threadIdx.x & 63;
@@ -481,7 +521,7 @@ __device__ __forceinline__ void sell_rows(const SellView& m, const double2* __re
             if (s >= end) break;
         }
         s = __builtin_amdgcn_readfirstlane(s);
-        sell_slice(m, x, vt, s, lane, epi);
+        slice(s, lane);
     }
 }
 
@@ -489,13 +529,11 @@ __global__ __launch_bounds__(kBlock) void spmv_sell_kernel(SellView m, const dou
                                                            double2* __restrict__ y, double2 alpha) {
     __shared__ double2 vt[kVdMaxValues];
     SpmvEpiVd epi{alpha, y};
-    sell_rows(m, x, vt, epi);
+    sell_sweep(m, vt, [&](int s, int lane) { sell_slice(m, x, vt, s, lane, epi); });
 }
 
-__global__ __launch_bounds__(kBlock) void lanczos_apply_sell_kernel(LanczosArgs a) {
-    KrylovScalars* ks = a.ks;
-    if (ks->broken) return;
-    __shared__ double2 vt[kVdMaxValues];
+__device__ __forceinline__ ApplyEpiVd apply_epi(const LanczosArgs& a) {
+    const KrylovScalars* ks = a.ks;
     const int j = a.j;
     const double sj = ks->s[j];
     ApplyEpiVd epi;
@@ -504,13 +542,63 @@ __global__ __launch_bounds__(kBlock) void lanczos_apply_sell_kernel(LanczosArgs 
     epi.up = j > 0 ? a.U + (size_t)(j - 1) * a.ld : nullptr;
     epi.w = a.U + (size_t)(j + 1) * a.ld;
     epi.dot = make_double2(0.0, 0.0);
-    sell_rows(a.sell, a.U + (size_t)j * a.ld, vt, epi);
+    return epi;
+}
+
+// The last block's finish of an apply: alpha_j = s_j sum of the block partials.
+__device__ __forceinline__ void apply_finish(const LanczosArgs& a, double2 tot) {
+    const double sj = a.ks->s[a.j];
+    a.ks->a[a.j] = make_double2(sj * tot.x, sj * tot.y);
+}
+
+__device__ __forceinline__ void apply_sell_body(const LanczosArgs& a, double2* vt) {
+    ApplyEpiVd epi = apply_epi(a);
+    const double2* x = a.U + (size_t)a.j * a.ld;
+    sell_sweep(a.sell, vt, [&](int s, int lane) { sell_slice(a.sell, x, vt, s, lane, epi); });
     double2 bs = block_sum2<kBlock>(epi.dot, vt);       // the table's LDS is free again
     if (threadIdx.x == 0) a.partials[blockIdx.x] = bs;
     if (last_block_arrive(a.ticket)) {
         double2 tot = reduce_partials<kBlock>(a.partials, gridDim.x, 1, vt);
         if (threadIdx.x == 0) {
-            ks->a[j] = make_double2(sj * tot.x, sj * tot.y);
+            apply_finish(a, tot);
+            *a.ticket = 0;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void lanczos_apply_sell_kernel(LanczosArgs a) {
+    if (a.ks->broken) return;
+    __shared__ double2 vt[kVdMaxValues];
+    apply_sell_body(a, vt);
+}
+
+// Two Lanczos recurrences on the same matrix, iteration j of each, in one pass over its words
+// (a and b: own bases, scalars and partials; a's ticket). A recurrence that broke down is
+// skipped; with one left the single-vector pass runs. Each vector gets the sums, partials and
+// reduction order of its own apply.
+__global__ __launch_bounds__(kBlock) void lanczos_apply_sell2_kernel(LanczosArgs a, LanczosArgs b) {
+    const bool la = !a.ks->broken, lb = !b.ks->broken;
+    __shared__ double2 vt[kVdMaxValues];
+    if (!la || !lb) {
+        if (la) apply_sell_body(a, vt);
+        if (lb) apply_sell_body(b, vt);
+        return;
+    }
+    ApplyEpiVd epi[2] = {apply_epi(a), apply_epi(b)};
+    const double2* xs[2] = {a.U + (size_t)a.j * a.ld, b.U + (size_t)b.j * b.ld};
+    sell_sweep(a.sell, vt, [&](int s, int lane) { sell_slice_n<2>(a.sell, xs, vt, s, lane, epi); });
+    const double2 bs0 = block_sum2<kBlock>(epi[0].dot, vt);
+    const double2 bs1 = block_sum2<kBlock>(epi[1].dot, vt);
+    if (threadIdx.x == 0) {
+        a.partials[blockIdx.x] = bs0;
+        b.partials[blockIdx.x] = bs1;
+    }
+    if (last_block_arrive(a.ticket)) {
+        const double2 t0 = reduce_partials<kBlock>(a.partials, gridDim.x, 1, vt);
+        const double2 t1 = reduce_partials<kBlock>(b.partials, gridDim.x, 1, vt);
+        if (threadIdx.x == 0) {
+            apply_finish(a, t0);
+            apply_finish(b, t1);
             *a.ticket = 0;
         }
     }
@@ -1278,6 +1366,21 @@ int prepare_spmv(te_ctx* ctx, te_matrix* A, const int* perm) {
 
 void launch_lanczos_project(const LanczosArgs& a, int grid, hipStream_t s) {
     hipLaunchKernelGGL(lanczos_project_kernel, dim3(grid), dim3(kBlock), 0, s, a);
+}
+
+void launch_lanczos_apply2(LanczosArgs a, LanczosArgs b, const te_matrix* A, int grid, hipStream_t s) {
+    if (A->use_vd && A->vd.sell) {
+        a.sell = b.sell = sell_view(A);
+        hipLaunchKernelGGL(lanczos_apply_sell2_kernel, dim3(grid > 0 ? grid : A->vd.sell_grid), dim3(kBlock), 0, s, a,
+                           b);
+        return;
+    }
+    launch_lanczos_apply(a, A, s);
+    launch_lanczos_apply(b, A, s);
+}
+
+void launch_lanczos_project2(const LanczosArgs& a, const LanczosArgs& b, int grid, hipStream_t s) {
+    hipLaunchKernelGGL(lanczos_project2_kernel, dim3(grid), dim3(kBlock), 0, s, a, b);
 }
 
 void launch_zdotc(const double2* x, const double2* y, int64_t n, ReduceScratch& rs, int slot, hipStream_t s) {
