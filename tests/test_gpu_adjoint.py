@@ -64,6 +64,26 @@ def test_adjoint_vs_dense_frechet_L6(gpu, K):
     assert rel <= 1e-6
 
 
+@pytest.mark.parametrize("K", [1, 2])
+def test_adjoint_paired_recurrences_break_down_apart(gpu, K):
+    """Uniform couplings on a 4-mode ring with 2 particles (D = 10) and the Fock start |2,0,0,0>
+    keep psi in the subspace symmetric under the reflection about mode 0, while the loss n_1
+    breaks that symmetry: psi's Lanczos recurrence breaks down after 6 iterations and lambda's
+    after 7 (plain Lanczos on the dense matrix), so for one iteration the paired apply and
+    projection go on with lambda alone. Gradient against the dense Frechet derivative."""
+    L, N = 4, 2
+    T = 2.0
+    theta = np.array([0.3] * L + [0.7] * L)
+    prob = ring_problem(gpu, L, N, 0.5, T, tol=1e-8, m=40, obs_mode=1, checkpoint_interval=K, theta0=theta)
+    psi0 = fock_state(prob.basis.D)
+    assert list(prob.basis.states()[0]) == [N, 0, 0, 0]
+    loss, grad, res, info = prob.loss_and_grad(theta, psi0)
+    l_ref, g_ref = _dense_grad(prob, theta, psi0, T)
+    assert res["statusCode"] == 1                      # the forward broke down too (lucky)
+    assert abs(loss - l_ref) <= 2 * N * res["err"] + 1e-12
+    assert np.linalg.norm(grad - g_ref) <= 1e-6 * np.linalg.norm(g_ref)
+
+
 def test_adjoint_random_state_and_loss_matches_plain(gpu):
     """A random initial state, a different observable mode; the loss is the plain evolver's <n_2>(T)."""
     L = N = 5
