@@ -924,6 +924,10 @@ __device__ __forceinline__ void block_rows_k(int64_t n, int64_t& lo, int64_t& hi
     hi = lo + chunk < n ? lo + chunk : n;
 }
 
+// basis columns a thread loads before it uses any: 8, or 4 with 8 samples a launch (registers)
+template <int SC>
+constexpr int combine_burst() { return SC >= 8 ? 4 : 8; }
+
 template <int SC, int NOB>
 __global__ __launch_bounds__(kBlock) void combine_kernel(CombineArgs a) {
     __shared__ double2 cf[(1 + SC) * kMaxKrylov];
@@ -953,8 +957,7 @@ __global__ __launch_bounds__(kBlock) void combine_kernel(CombineArgs a) {
         double2 as[SC];
 #pragma unroll
         for (int s = 0; s < SC; s++) as[s] = make_double2(0.0, 0.0);
-        for (int j = 0; j < m; j++) {
-            const double2 u = a.U[(size_t)j * a.ld + row];
+        auto add = [&](int j, double2 u) {
             if (a.write_w) {
                 const double2 t = cmul(u, cf[j]);
                 aw.x += t.x;
@@ -966,7 +969,19 @@ __global__ __launch_bounds__(kBlock) void combine_kernel(CombineArgs a) {
                 as[s].x += t.x;
                 as[s].y += t.y;
             }
+        };
+        // a burst of basis columns in flight per thread: all loads of a burst are issued before
+        // any is used (the sums still run over j in order)
+        constexpr int B = combine_burst<SC>();
+        int j = 0;
+        for (; j + B <= m; j += B) {
+            double2 u[B];
+#pragma unroll
+            for (int q = 0; q < B; q++) u[q] = a.U[(size_t)(j + q) * a.ld + row];
+#pragma unroll
+            for (int q = 0; q < B; q++) add(j + q, u[q]);
         }
+        for (; j < m; j++) add(j, a.U[(size_t)j * a.ld + row]);
         if (a.write_w) a.U0[row] = aw;
         if (a.write_last && S > 0) {
             double2 v = as[0];
@@ -976,12 +991,15 @@ __global__ __launch_bounds__(kBlock) void combine_kernel(CombineArgs a) {
             a.last_out[row] = v;
         }
         if (NOB > 0) {
+            double dg[NOB > 0 ? NOB : 1];                 // each observable's diagonal read once a row
+#pragma unroll
+            for (int o = 0; o < (NOB > 0 ? NOB : 1); o++) dg[o] = o < a.nobs ? a.diag[(size_t)o * a.n + row] : 0.0;
 #pragma unroll
             for (int s = 0; s < SC; s++) {
                 const double pr = as[s].x * as[s].x + as[s].y * as[s].y;
 #pragma unroll
                 for (int o = 0; o < (NOB > 0 ? NOB : 1); o++)
-                    if (o < a.nobs) acc[o * SC + s] += a.diag[(size_t)o * a.n + row] * pr;
+                    if (o < a.nobs) acc[o * SC + s] += dg[o] * pr;
             }
         }
     }
