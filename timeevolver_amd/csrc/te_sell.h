@@ -9,12 +9,17 @@
 namespace te {
 
 // A group of kSellGroup slots: the lane's words (streamed) and the slots' base columns (the same
-// for the whole wavefront: scalar loads).
-__device__ __forceinline__ void sell_load_group(const unsigned* wp, const int* bp, unsigned* w, int* b) {
+// for the whole wavefront: scalar loads). Slots from `valid` on lie past the slice's end: not
+// loaded (padding word, base 0: column 0).
+__device__ __forceinline__ void sell_load_group(const unsigned* wp, const int* bp, int valid, unsigned* w, int* b) {
 #pragma unroll
     for (int q = 0; q < kSellGroup; q++) {
-        w[q] = __builtin_nontemporal_load(wp + q * kSlice);
-        b[q] = bp[q];
+        w[q] = kSellPadIdx << 16;
+        b[q] = 0;
+        if (q < valid) {                              // the same for the whole wavefront
+            w[q] = __builtin_nontemporal_load(wp + q * kSlice);
+            b[q] = bp[q];
+        }
     }
 }
 
@@ -50,7 +55,7 @@ __device__ __forceinline__ void sell_slice_n(const SellView& m, const double2* c
     const int* bp = m.base + o / kSlice;
     unsigned w[kSellGroup];
     int b[kSellGroup];
-    sell_load_group(wp, bp, w, b);
+    sell_load_group(wp, bp, ls, w, b);
     double sr[NV], si[NV];
 #pragma unroll
     for (int v = 0; v < NV; v++) {
@@ -59,8 +64,7 @@ __device__ __forceinline__ void sell_slice_n(const SellView& m, const double2* c
     }
     // Group g's gathers and group g+1's word loads are in flight together (the last group is
     // peeled so that the loop body is straight-line code for the wait counters). Slots past the
-    // slice end (last group) read the next slice's words and bases or the zero padding: in
-    // bounds, valid columns, never added.
+    // slice end (last group) are neither loaded nor gathered.
     auto accumulate = [&](const unsigned* ww, const double2 (*xv)[kSellGroup], int g) {
 #pragma unroll
         for (int q = 0; q < kSellGroup; q++) {
@@ -75,28 +79,28 @@ __device__ __forceinline__ void sell_slice_n(const SellView& m, const double2* c
             }
         }
     };
-    auto gather = [&](double2 (*xv)[kSellGroup]) {
+    auto gather = [&](double2 (*xv)[kSellGroup], int valid) {
 #pragma unroll
         for (int q = 0; q < kSellGroup; q++) {
             const int c = b[q] + (int)(w[q] & 0xffffu);
 #pragma unroll
-            for (int v = 0; v < NV; v++) xv[v][q] = x[v][c];
+            for (int v = 0; v < NV; v++) xv[v][q] = q < valid ? x[v][c] : make_double2(0.0, 0.0);
         }
     };
     for (int g = 0; g + 1 < ng; g++) {
         double2 xv[NV][kSellGroup];
         unsigned ww[kSellGroup];
-        gather(xv);
+        gather(xv, kSellGroup);
 #pragma unroll
         for (int q = 0; q < kSellGroup; q++) ww[q] = w[q];
         wp += kSellGroup * kSlice;
         bp += kSellGroup;
-        sell_load_group(wp, bp, w, b);
+        sell_load_group(wp, bp, ls - kSellGroup * (g + 1), w, b);
         accumulate(ww, xv, g);
     }
     if (ng > 0) {
         double2 xv[NV][kSellGroup];
-        gather(xv);
+        gather(xv, ls - kSellGroup * (ng - 1));
         accumulate(w, xv, ng - 1);
     }
     if (r < m.n) {
