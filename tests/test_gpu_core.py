@@ -440,6 +440,39 @@ def test_ring_random_state_vs_oracle(gpu, oracle_lib):
     _check_pair(ro, rg, 6)
 
 
+def test_complex_hermitian_ring_vs_oracle(gpu, oracle_lib):
+    """A ring threaded by a flux: hops t_k a_{k+1}^dag a_k + conj(t_k) a_k^dag a_{k+1} with
+    t_k = J_k exp(i phi_k), so H is Hermitian with complex off-diagonal values (its value table,
+    the builder's storage of non-real terms and every product run on complex values). Built
+    bit-exactly and evolved against the oracle from a random state."""
+    from timeevolver_amd.hamiltonian import OpTerm, createFourPointB
+    L = N = 6
+    rng = np.random.default_rng(5)
+    eps, J = ring_couplings(L, 13)
+    phi = rng.uniform(-np.pi, np.pi, L)
+    terms = [createNumberOperator(k, eps[k]) for k in range(L)]
+    for k in range(L):
+        t = J[k] * np.exp(1j * phi[k])
+        terms.append(OpTerm(complex(t), [-1, 1], [k, (k + 1) % L], [0, 0]))
+        terms.append(OpTerm(complex(np.conj(t)), [-1, 1], [(k + 1) % L, k], [0, 0]))
+    terms += [createFourPointB(k, k, k, k, 0, 0, 0, 0, 0.25) for k in range(L)]
+    terms = as_tuples(terms)
+    ob = oracle_lib.Basis.basis(N, L)
+    H = oracle_lib.build_operator(ob, terms)
+    assert np.abs(H[0].imag).max() > 0.1                       # genuinely complex entries
+    Hd = np.zeros((ob.D, ob.D), complex)
+    np.add.at(Hd, (H[2].astype(np.int64), H[1].astype(np.int64)), H[0])
+    assert np.abs(Hd - Hd.conj().T).max() == 0.0              # Hermitian as stored
+    oobs = [("sparse", oracle_lib.build_operator(ob, as_tuples([createNumberOperator(k, 1)]))) for k in range(L)]
+    psi = random_state(ob.D, 12)
+    ro = oracle_lib.evolve(ob.D, H, psi, 4.0, 0.1, oobs, tol=1e-8, m=40)
+    b, Hg, gobs = _gpu_model(gpu, "basis", (N, L), terms, [[createNumberOperator(k, 1)] for k in range(L)])
+    v, c, r = Hg.to_coo()
+    assert np.array_equal(c, H[1]) and np.array_equal(v.view(np.uint64), H[0].view(np.uint64))
+    rg = capi.Evolver(gpu, Hg, psi, 4.0, 0.1, gobs, tol=1e-8, m=40).run()
+    _check_pair(ro, rg, N)
+
+
 def test_fast_integration_vs_oracle(gpu, oracle_lib):
     ro, rg = _ring_pair(gpu, oracle_lib, 6, 6, fock_state, 2.0, 0.1, fastIntegration=True)
     _check_pair(ro, rg, 6)
