@@ -287,6 +287,7 @@ __global__ __launch_bounds__(kBlock) void batch_project_kernel(BatchDev a, int j
 // last sample is the one that overwrites column 0, after every pass has read it.
 template <int NOB>
 __global__ __launch_bounds__(kBlock) void batch_combine_kernel(BatchDev a) {
+    constexpr int kBatchBurst = 8;
     __shared__ double2 cf[(1 + kBatchSC) * kMaxKrylov];
     __shared__ double2 sh[kBlock / 64];
     int u0, u1, us;
@@ -329,8 +330,7 @@ __global__ __launch_bounds__(kBlock) void batch_combine_kernel(BatchDev a) {
                 double2 as[kBatchSC];
 #pragma unroll
                 for (int s = 0; s < kBatchSC; s++) as[s] = make_double2(0.0, 0.0);
-                for (int jj = 0; jj < m; jj++) {
-                    const double2 uv = a.U[(size_t)jj * a.ld + gr];
+                auto add = [&](int jj, double2 uv) {
                     if (ww) {
                         const double2 t = cmul(uv, cf[jj]);
                         aw.x += t.x;
@@ -344,7 +344,17 @@ __global__ __launch_bounds__(kBlock) void batch_combine_kernel(BatchDev a) {
                             as[s].y += t.y;
                         }
                     }
+                };
+                // kBatchBurst basis columns in flight per thread (sums still in order of jj)
+                int jj = 0;
+                for (; jj + kBatchBurst <= m; jj += kBatchBurst) {
+                    double2 uv[kBatchBurst];
+#pragma unroll
+                    for (int q = 0; q < kBatchBurst; q++) uv[q] = a.U[(size_t)(jj + q) * a.ld + gr];
+#pragma unroll
+                    for (int q = 0; q < kBatchBurst; q++) add(jj + q, uv[q]);
                 }
+                for (; jj < m; jj++) add(jj, a.U[(size_t)jj * a.ld + gr]);
                 if (ww) a.U[gr] = aw;
                 if (lastpass && sc > 0) {
                     double2 v = as[0];
@@ -353,12 +363,15 @@ __global__ __launch_bounds__(kBlock) void batch_combine_kernel(BatchDev a) {
                         if (s == sc - 1) v = as[s];
                     a.sampled[gr] = v;
                 }
+                double dg[NOB > 0 ? NOB : 1];              // each observable's diagonal read once a row
+#pragma unroll
+                for (int o = 0; o < NOB; o++) dg[o] = o < a.nobs ? a.diag[(size_t)o * a.ld + gr] : 0.0;
 #pragma unroll
                 for (int s = 0; s < kBatchSC; s++) {
                     const double pr = as[s].x * as[s].x + as[s].y * as[s].y;
 #pragma unroll
                     for (int o = 0; o < NOB; o++)
-                        if (o < a.nobs && s < sc) acc[o * kBatchSC + s] += a.diag[(size_t)o * a.ld + gr] * pr;
+                        if (o < a.nobs && s < sc) acc[o * kBatchSC + s] += dg[o] * pr;
                 }
             }
 #pragma unroll
