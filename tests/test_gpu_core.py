@@ -237,6 +237,33 @@ def test_sliced_ell_slots_beyond_16_bit_span(gpu, oracle_lib):
     _spmv_check(oracle_lib, A, D, v, c, r)
 
 
+def test_scattered_columns_keep_row_blocks(gpu, oracle_lib):
+    """A Hermitian matrix whose rows reach columns anywhere in 0..D (D = 1,000,000, no
+    locality): one 16-bit span per slot would pad the aligned slices about sixfold (host count
+    of the alignment rule on the first 300 slices), so the matrix keeps the value-dictionary
+    row blocks (format 2); products and a short evolution match the oracle."""
+    rng = np.random.default_rng(11)
+    D = 1_000_000
+    rows = np.repeat(np.arange(D), 3)
+    cols = rng.integers(0, D, len(rows))
+    keep = rows != cols
+    rows, cols = rows[keep], cols[keep]
+    key = np.unique(rows * D + cols)
+    rows, cols = key // D, key % D
+    vals = np.array([0.3, -0.7, 0.2 + 0.4j, 1.1j])[rng.integers(0, 4, len(rows))]
+    r = np.concatenate([rows, cols, np.arange(D)]).astype(np.uint64)
+    c = np.concatenate([cols, rows, np.arange(D)]).astype(np.uint64)
+    v = np.concatenate([vals, np.conj(vals), rng.normal(size=D)]).astype(complex)
+    A = capi.Matrix.from_coo(gpu, v, c, r, D)
+    assert A.storage()[0] == 2
+    _spmv_check(oracle_lib, A, D, v, c, r)
+    psi = random_state(D, 2)
+    ro = oracle_lib.evolve(D, (v, c, r), psi, 0.3, 0.1, [], tol=1e-8, m=30)
+    rg = capi.Evolver(gpu, A, psi, 0.3, 0.1, [], tol=1e-8, m=30).run()
+    assert rg["statusCode"] == ro["statusCode"]
+    assert np.linalg.norm(rg["state"] - ro["state"]) <= ro["err"] + rg["err"] + 1e-10
+
+
 def test_locality_order_evolution(gpu, oracle_lib, monkeypatch):
     """An operator built on a basis evolves in the lexicographic order of the states; every
     observable kind (diagonal, non-diagonal sparse, vector, host callback), the sampled state

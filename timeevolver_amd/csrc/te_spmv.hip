@@ -1083,23 +1083,31 @@ __global__ __launch_bounds__(kSellWindow) void sigma_sort_kernel(const int64_t* 
 // slot of a slice takes, from every row, the row's next entry (rows in column order) if that
 // lies within kSellSpan columns of the smallest next entry over the slice; rows whose next entry
 // lies further right get a padding word in it. Count pass (FILL false): slen[s] = the slice's
-// slots * kSlice. Fill pass: the words and slot bases at the offsets soff (exclusive scan of
-// slen). A slot's padding words point at its base column, a column of one of its entries.
+// slots * kSlice, smax[s] = its longest row * kSlice (the slots without the span limit). Fill
+// pass: the words and slot bases at the offsets soff (exclusive scan of slen). A slot's padding
+// words point at its base column, a column of one of its entries.
 template <bool FILL>
 __global__ __launch_bounds__(256) void sell_align_kernel(const int* __restrict__ vptr, const int* __restrict__ vcol,
                                                          const unsigned short* __restrict__ vvidx, int64_t n,
-                                                         int n_slices, int64_t* slen, const int64_t* __restrict__ soff,
-                                                         int* sell_off, unsigned* word, int* base) {
+                                                         int n_slices, int64_t* slen, int64_t* smax,
+                                                         const int64_t* __restrict__ soff, int* sell_off,
+                                                         unsigned* word, int* base) {
     const int s = blockIdx.x * (blockDim.x / kSlice) + (int)(threadIdx.x / kSlice);
     const int lane = threadIdx.x % kSlice;
     if (s > n_slices) return;                                  // whole wavefronts
     if (s == n_slices) {
-        if (!FILL && lane == 0) slen[s] = 0;
+        if (!FILL && lane == 0) slen[s] = smax[s] = 0;
         return;
     }
     const int64_t r = (int64_t)s * kSlice + lane;
     int k = r < n ? vptr[r] : 0;
     const int kend = r < n ? vptr[r + 1] : 0;
+    if (!FILL) {
+        int len = kend - k;
+#pragma unroll
+        for (int d = kSlice / 2; d >= 1; d >>= 1) len = max(len, __shfl_xor(len, d));
+        if (lane == 0) smax[s] = (int64_t)len * kSlice;
+    }
     int64_t o = 0;
     if (FILL) {
         o = soff[s];
@@ -1153,19 +1161,28 @@ static int build_sell(te_ctx* ctx, te_matrix* A) {
     const int64_t ns64 = (n + kSlice - 1) / kSlice;
     if (ns64 >= INT32_MAX / kSlice) return TE_OK;
     const int ns = (int)ns64;
-    int64_t *slen = nullptr, *soff = nullptr;
+    int64_t *slen = nullptr, *soff = nullptr, *smax = nullptr, *smax_off = nullptr;
     TE_HIP(ctx, hipMalloc(&slen, sizeof(int64_t) * (ns + 1)));
     TE_HIP(ctx, hipMalloc(&soff, sizeof(int64_t) * (ns + 1)));
+    TE_HIP(ctx, hipMalloc(&smax, sizeof(int64_t) * (ns + 1)));
+    TE_HIP(ctx, hipMalloc(&smax_off, sizeof(int64_t) * (ns + 1)));
     const unsigned ga = (unsigned)((ns + 1 + 256 / kSlice - 1) / (256 / kSlice));
-    hipLaunchKernelGGL(sell_align_kernel<false>, dim3(ga), dim3(256), 0, st, v.ptr, v.col, v.vidx, n, ns, slen,
+    hipLaunchKernelGGL(sell_align_kernel<false>, dim3(ga), dim3(256), 0, st, v.ptr, v.col, v.vidx, n, ns, slen, smax,
                        nullptr, nullptr, nullptr, nullptr);
     int rc = exclusive_scan64(ctx, slen, soff, ns + 1);
-    int64_t total = 0;
+    if (rc == TE_OK) rc = exclusive_scan64(ctx, smax, smax_off, ns + 1);
+    int64_t total = 0, total_max = 0;
     if (rc == TE_OK) {
         TE_HIP(ctx, hipMemcpyAsync(&total, soff + ns, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        TE_HIP(ctx, hipMemcpyAsync(&total_max, smax_off + ns, sizeof(int64_t), hipMemcpyDeviceToHost, st));
         TE_HIP(ctx, hipStreamSynchronize(st));
     }
-    if (rc == TE_OK && total < INT32_MAX) {
+    hipFree(smax);
+    hipFree(smax_off);
+    // Rows whose columns scatter beyond the span (no locality) would make the aligned slots pad
+    // without bound: past 1.5x the slots of a layout padded only to each slice's longest row, the
+    // matrix keeps the value-dictionary row-block kernels.
+    if (rc == TE_OK && total < INT32_MAX && total <= total_max + total_max / 2) {
         TE_HIP(ctx, hipMalloc(&v.sell_off, sizeof(int) * (ns + 1)));
         // a slice's last group of slots may run up to kSellGroup - 1 slots past its end: zero
         // padding (words: column base + 0, value index 0, never added)
@@ -1176,7 +1193,7 @@ static int build_sell(te_ctx* ctx, te_matrix* A) {
         TE_HIP(ctx, hipMemsetAsync(v.sell_word + total, 0, sizeof(unsigned) * pad, st));
         TE_HIP(ctx, hipMemsetAsync(v.sell_base + nslots, 0, sizeof(int) * kSellGroup, st));
         hipLaunchKernelGGL(sell_align_kernel<true>, dim3(ga), dim3(256), 0, st, v.ptr, v.col, v.vidx, n, ns, slen,
-                           soff, v.sell_off, v.sell_word, v.sell_base);
+                           nullptr, soff, v.sell_off, v.sell_word, v.sell_base);
         TE_HIP(ctx, hipStreamSynchronize(st));
         v.n_slices = ns;
         v.sell_entries = total;
