@@ -111,6 +111,45 @@ def _check_reference_layout(ours, ref):
             assert (a["type"], a["space"]) == (typ, space), (ds, name)
 
 
+@pytest.mark.parametrize("hdf5", [False, True])
+def test_writers_clean_under_sanitizers(tmp_path, hdf5):
+    """The result writers (both flavours) with AddressSanitizer, leak detection and
+    UndefinedBehaviorSanitizer: host data only, so the whole layer runs on the CPU."""
+    if hdf5 and not HAVE_HDF5:
+        pytest.skip("HDF5 library or h5dump not present")
+    _build_layer()
+    exe = str(tmp_path / "save_output_san")
+    src = [os.path.join(ROOT, "tests", "cpp", "save_output.cpp")]
+    src += sorted(os.path.join(CPP, "src", f) for f in os.listdir(os.path.join(CPP, "src")) if f.endswith(".cpp"))
+    cmd = ["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+           "-fno-omit-frame-pointer", "-I", os.path.join(CPP, "include")] + src + ["-o", exe,
+           "-L", os.path.join(ROOT, "timeevolver_amd"), "-ltimeevolver_hip",
+           "-Wl,-rpath," + os.path.join(ROOT, "timeevolver_amd")]
+    if hdf5:
+        # the HDF5 prefix also holds an older libstdc++: reach libhdf5 through a directory of its own
+        libdir = tmp_path / "h5lib"
+        libdir.mkdir()
+        real = os.path.realpath(os.path.join(HDF5_DIR, "lib", "libhdf5.so"))
+        os.symlink(real, libdir / os.path.basename(real))
+        soname = [f for f in os.listdir(os.path.join(HDF5_DIR, "lib")) if f.startswith("libhdf5.so.")
+                  and f.count(".") == 2]
+        for name in soname:
+            if not (libdir / name).exists():
+                os.symlink(real, libdir / name)
+        cmd += ["-DUSE_HDF", "-idirafter", HDF5_DIR + "/include", os.path.join(HDF5_DIR, "lib", "libhdf5.so"),
+                "-Wl,-rpath," + str(libdir)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0", UBSAN_OPTIONS="print_stacktrace=1")
+    out = tmp_path / "out"
+    out.mkdir()
+    for args in (["generic"], ["result", os.path.join(GOLD, "blackhole_N20_Nm2_K4.txt")] +
+                 BH_RUNS["blackhole_N20_Nm2_K4.txt"]):
+        r = subprocess.run([exe, str(out)] + args, capture_output=True, text=True, env=env, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        assert "Sanitizer" not in r.stderr and "runtime error" not in r.stderr, r.stderr
+
+
 @pytest.mark.skipif(not HAVE_HDF5, reason="HDF5 library or h5dump not present")
 def test_result_h5_matches_reference_layout(tmp_path):
     exe = _build_writer(tmp_path, hdf5=True)
