@@ -168,12 +168,13 @@ int te_evolver_state_device(te_evolver* ev, const void** state_dev);
 int te_evolver_step_trace(te_evolver* ev, double* t_steps, size_t cap, size_t* n);
 
 /* Kernel timing of te_evolver_run, for measurement harnesses (no reference counterpart):
- * HIP events recorded around the first 4096 launches of each kind on the context's stream.
- * kind: 0 Lanczos apply (fused SpMV, the dominant kernel), 1 Lanczos project,
+ * HIP events recorded around the first 4096 launches of each kind on the context's stream,
+ * for the kinds whose bit is set in kinds (bit k: kind k; 0 turns timing off; events cost a
+ * few µs each). kind: 0 Lanczos apply (fused SpMV, the dominant kernel), 1 Lanczos project,
  * 2 step controller, 3 combine (state update, samples, observables); 4..6 the controller's
  * phases (eigen-decomposition, step-size search, sample coefficients) from the device clock,
  * with *launches the number of Krylov steps. */
-int te_evolver_set_profiling(te_evolver* ev, int enable);
+int te_evolver_set_profiling(te_evolver* ev, int kinds);
 int te_evolver_kernel_time(const te_evolver* ev, int kind, double* total_ms, size_t* launches);
 
 /* ---- batched evolution (no reference counterpart: the reference runs one krylovTimeEvolver

@@ -372,8 +372,17 @@ class Evolver:
             "trace": trace[:nt.value],
         }
 
-    def set_profiling(self, on=True):
-        self.ctx.check(self.ctx.L.te_evolver_set_profiling(self.h, int(on)), "set_profiling")
+    KINDS = ("apply", "project", "control", "combine")
+
+    def set_profiling(self, kinds=KINDS):
+        """Time the launches of the given kinds (names from KINDS) with HIP events in the next
+        runs; False/None/() turns timing off, True times every kind."""
+        if kinds is True:
+            kinds = self.KINDS
+        mask = 0
+        for k in (kinds or ()):
+            mask |= 1 << self.KINDS.index(k)
+        self.ctx.check(self.ctx.L.te_evolver_set_profiling(self.h, mask), "set_profiling")
 
     def kernel_times(self):
         """{kind: (total_ms, launches)} of the last run (0 apply, 1 project, 2 control, 3 combine)."""

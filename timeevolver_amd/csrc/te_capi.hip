@@ -528,7 +528,7 @@ struct te_evolver {
     double2* tmp_ref = nullptr;
     double2* state_out = nullptr;
     // optional kernel timing: HIP event pairs on the context stream, per kernel kind
-    bool profiling = false;
+    int prof_kinds = 0;              // bit k: time launches of kind k
     std::vector<hipEvent_t> prof_ev[4];
     size_t prof_used[4] = {0, 0, 0, 0};
     double prof_ms[4] = {0, 0, 0, 0};
@@ -562,11 +562,11 @@ static void evolver_free(te_evolver* ev) {
 constexpr size_t kProfPairs = 4096;
 
 static void prof_begin(te_evolver* ev, int kind) {
-    if (ev->profiling && ev->prof_used[kind] + 2 <= ev->prof_ev[kind].size())
+    if (((ev->prof_kinds >> kind) & 1) && ev->prof_used[kind] + 2 <= ev->prof_ev[kind].size())
         hipEventRecord(ev->prof_ev[kind][ev->prof_used[kind]], ev->ctx->stream);
 }
 static void prof_end(te_evolver* ev, int kind) {
-    if (ev->profiling && ev->prof_used[kind] + 2 <= ev->prof_ev[kind].size()) {
+    if (((ev->prof_kinds >> kind) & 1) && ev->prof_used[kind] + 2 <= ev->prof_ev[kind].size()) {
         hipEventRecord(ev->prof_ev[kind][ev->prof_used[kind] + 1], ev->ctx->stream);
         ev->prof_used[kind] += 2;
     }
@@ -1052,10 +1052,10 @@ extern "C" int te_evolver_run(te_evolver* ev, double* obs_values, double* evolve
     return TE_OK;
 }
 
-extern "C" int te_evolver_set_profiling(te_evolver* ev, int enable) {
-    if (!ev) return TE_ERR_ARG;
-    ev->profiling = enable != 0;
-    if (ev->profiling && ev->prof_ev[0].empty()) {
+extern "C" int te_evolver_set_profiling(te_evolver* ev, int kinds) {
+    if (!ev || kinds < 0 || kinds > 15) return TE_ERR_ARG;
+    ev->prof_kinds = kinds;
+    if (kinds && ev->prof_ev[0].empty()) {
         hipSetDevice(ev->ctx->device);
         for (auto& v : ev->prof_ev) {
             v.resize(2 * te::kProfPairs);
