@@ -53,8 +53,8 @@ int stream_grid(const te_ctx* ctx, int64_t work_items) {
 }
 
 int project_grid(const te_ctx* ctx, int64_t rows) {
-    // Two blocks per CU, each streaming one long contiguous range: on the L = N = 14 ring the
-    // projection ran 187 us at 2 against 230 us at 4 and 289 us at 8 blocks per CU.
+    // Two blocks per CU sweeping one front of 1024-row chunks: on the L = N = 14 ring the
+    // projection ran 177 us at 2 against 202 us at 1 and 208 us at 4 blocks per CU.
     static const int per_cu = [] {
         const char* v = std::getenv("TE_PROJECT_BLOCKS_PER_CU");    // launch-shape sweeps
         return v ? std::max(1, std::atoi(v)) : 2;

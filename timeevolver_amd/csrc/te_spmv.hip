@@ -181,12 +181,28 @@ __device__ __forceinline__ void project_finish(const LanczosArgs& a, double2 tot
     }
 }
 
+// The projection's rows in chunks of kProjectChunk, block b taking chunks nch-1-b,
+// nch-1-b-grid, ...: all blocks sweep one front from the last rows to the first. The apply's
+// front ends on the last rows, so the projection starts on the w rows still held in the MALL,
+// and ends on the u_{j+1} rows the next apply gathers first (5 % off the projection, 1 % off the
+// apply on the L = N = 14 ring, against a contiguous range per block). Returns this block's
+// contribution to |w'|^2 (the sum over its chunks in that order).
+constexpr int64_t kProjectChunk = 4 * kBlock;
+
+__device__ __forceinline__ double project_front(const LanczosArgs& a) {
+    const int64_t nch = (a.n + kProjectChunk - 1) / kProjectChunk;
+    double nrm = 0.0;
+    for (int64_t k = blockIdx.x; k < nch; k += gridDim.x) {
+        const int64_t lo = (nch - 1 - k) * kProjectChunk;
+        nrm += project_rows(a, lo, lo + kProjectChunk < a.n ? lo + kProjectChunk : a.n);
+    }
+    return nrm;
+}
+
 __global__ __launch_bounds__(kBlock) void lanczos_project_kernel(LanczosArgs a) {
     if (a.ks->broken) return;
     __shared__ double2 sh[kBlock / 64];
-    int64_t lo, hi;
-    block_rows(a.n, lo, hi);
-    const double nrm = project_rows(a, lo, hi);
+    const double nrm = project_front(a);
     double2 bs = block_sum2<kBlock>(make_double2(nrm, 0.0), sh);
     if (threadIdx.x == 0) a.partials[blockIdx.x] = bs;
     if (last_block_arrive(a.ticket)) {
@@ -204,10 +220,8 @@ __global__ __launch_bounds__(kBlock) void lanczos_project2_kernel(LanczosArgs a,
     const bool la = !a.ks->broken, lb = !b.ks->broken;
     if (!la && !lb) return;
     __shared__ double2 sh[kBlock / 64];
-    int64_t lo, hi;
-    block_rows(a.n, lo, hi);
-    const double n0 = la ? project_rows(a, lo, hi) : 0.0;
-    const double n1 = lb ? project_rows(b, lo, hi) : 0.0;
+    const double n0 = la ? project_front(a) : 0.0;
+    const double n1 = lb ? project_front(b) : 0.0;
     const double2 bs0 = block_sum2<kBlock>(make_double2(n0, 0.0), sh);
     const double2 bs1 = block_sum2<kBlock>(make_double2(n1, 0.0), sh);
     if (threadIdx.x == 0) {
