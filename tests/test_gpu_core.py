@@ -61,7 +61,7 @@ def _ring_terms(L, seed=11, U=0.5):
     return ring_bose_hubbard_terms(L, eps, J, U)
 
 
-@pytest.mark.parametrize("L,N", [(3, 3), (6, 6), (8, 8), (10, 10)])
+@pytest.mark.parametrize("L,N", [(3, 3), (6, 6), (8, 8), (10, 10), (12, 12)])
 def test_ring_operator_bitexact(gpu, oracle_lib, L, N):
     terms = as_tuples(_ring_terms(L))
     gb = capi.Basis.basis(gpu, N, L)
@@ -460,6 +460,15 @@ def test_ring_small_config_vs_oracle(gpu, oracle_lib):
     """Ring Bose-Hubbard L = N = 8 (D = 6435), Fock |8,0,...>, T = 10, m = 40, tol = 1e-8, step 0.1."""
     ro, rg = _ring_pair(gpu, oracle_lib, 8, 8, fock_state, 10.0, 0.1)
     _check_pair(ro, rg, 8)
+
+
+def test_ring_medium_config_vs_oracle(gpu, oracle_lib):
+    """Ring Bose-Hubbard L = N = 12 (D = 1,352,078), seeded random complex initial state, T = 10,
+    m = 40, tol = 1e-8, step 0.1 (101 samples of the 12 n_k); the basis and CSR it runs on are
+    the ones test_ring_operator_bitexact[12-12] checks bit for bit."""
+    ro, rg = _ring_pair(gpu, oracle_lib, 12, 12, lambda D: random_state(D, 9), 10.0, 0.1)
+    _check_pair(ro, rg, 12)
+    assert rg["statusCode"] == 10
 
 
 def test_ring_random_state_vs_oracle(gpu, oracle_lib):
