@@ -915,14 +915,11 @@ void launch_adjoint_coef(const AdjCoefArgs& a, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------------------
-// V w, V y_s and diagonal observables in one pass over the Krylov basis.
+// V w, V y_s and diagonal observables in one pass over the Krylov basis. All blocks sweep one
+// front (tiles of kBlock rows dealt round-robin), and the basis columns are loaded
+// non-temporally: m column streams read this way run at 5.5 TB/s on the MI355X against 4.9-5.1
+// for a contiguous row range per block with ordinary loads.
 // ---------------------------------------------------------------------------------------
-__device__ __forceinline__ void block_rows_k(int64_t n, int64_t& lo, int64_t& hi) {
-    const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
-    const int lb = xcd_block_id();
-    lo = (int64_t)lb * chunk;
-    hi = lo + chunk < n ? lo + chunk : n;
-}
 
 // basis columns a thread loads before it uses any: 8, or 4 with 8 samples a launch (registers)
 template <int SC>
@@ -950,9 +947,7 @@ __global__ __launch_bounds__(kBlock) void combine_kernel(CombineArgs a) {
     double acc[NOB > 0 ? NOB * SC : 1];
 #pragma unroll
     for (int i = 0; i < (NOB > 0 ? NOB * SC : 1); i++) acc[i] = 0.0;
-    int64_t lo, hi;
-    block_rows_k(a.n, lo, hi);
-    for (int64_t row = lo + tid; row < hi; row += kBlock) {
+    for (int64_t row = (int64_t)blockIdx.x * kBlock + tid; row < a.n; row += (int64_t)gridDim.x * kBlock) {
         double2 aw = make_double2(0.0, 0.0);
         double2 as[SC];
 #pragma unroll
@@ -977,11 +972,11 @@ __global__ __launch_bounds__(kBlock) void combine_kernel(CombineArgs a) {
         for (; j + B <= m; j += B) {
             double2 u[B];
 #pragma unroll
-            for (int q = 0; q < B; q++) u[q] = a.U[(size_t)(j + q) * a.ld + row];
+            for (int q = 0; q < B; q++) u[q] = load_stream(&a.U[(size_t)(j + q) * a.ld + row]);
 #pragma unroll
             for (int q = 0; q < B; q++) add(j + q, u[q]);
         }
-        for (; j < m; j++) add(j, a.U[(size_t)j * a.ld + row]);
+        for (; j < m; j++) add(j, load_stream(&a.U[(size_t)j * a.ld + row]));
         if (a.write_w) a.U0[row] = aw;
         if (a.write_last && S > 0) {
             double2 v = as[0];

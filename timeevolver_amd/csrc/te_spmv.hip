@@ -137,13 +137,15 @@ __device__ __forceinline__ double project_rows(const LanczosArgs& a, int64_t lo,
     const bool store = j + 1 < a.m;
     double nrm = 0.0;
     int64_t row = lo + threadIdx.x;
-    // four rows per thread in flight: all eight loads are issued before any is used
+    // four rows per thread in flight: all eight loads are issued before any is used. Both are
+    // read non-temporally (3 % off the launch); u_{j+1} is stored with ordinary stores, so the
+    // rows the next apply gathers first stay in the MALL.
     for (; row + 3 * kBlock < hi; row += 4 * kBlock) {
         double2 u[4], wr[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            u[q] = uj[row + q * kBlock];
-            wr[q] = w[row + q * kBlock];
+            u[q] = load_stream(&uj[row + q * kBlock]);
+            wr[q] = load_stream(&w[row + q * kBlock]);
         }
 #pragma unroll
         for (int q = 0; q < 4; q++) {
@@ -154,8 +156,8 @@ __device__ __forceinline__ double project_rows(const LanczosArgs& a, int64_t lo,
         }
     }
     for (; row < hi; row += kBlock) {
-        const double2 u = uj[row];
-        double2 wr = w[row];
+        const double2 u = load_stream(&uj[row]);
+        double2 wr = load_stream(&w[row]);
         wr.x -= c3.x * u.x - c3.y * u.y;
         wr.y -= c3.x * u.y + c3.y * u.x;
         if (store) w[row] = wr;
