@@ -65,6 +65,20 @@ int project_grid(const te_ctx* ctx, int64_t rows) {
     return (int)((g + 7) / 8 * 8);
 }
 
+int combine_grid(const te_ctx* ctx, int64_t rows) {
+    // Two blocks per CU on one front of kBlock-row tiles: on the L = N = 14 ring (40 basis
+    // columns, 14 observables) the combine ran 37.5-38.3 ms a run at 2 against 40.1 at 4 and
+    // 41-42 at 1 block per CU.
+    static const int per_cu = [] {
+        const char* v = std::getenv("TE_COMBINE_BLOCKS_PER_CU");    // launch-shape sweeps
+        return v ? std::max(1, std::atoi(v)) : 2;
+    }();
+    const int cap = std::max(8, ctx->num_cus * per_cu);
+    int64_t g = (rows + kBlock - 1) / kBlock;
+    g = std::max<int64_t>(8, std::min<int64_t>(g, cap));
+    return (int)((g + 7) / 8 * 8);
+}
+
 SpmvShape choose_shape(const te_ctx* ctx, int64_t n_rows, int64_t nnz) {
     SpmvShape s;
     const double mean = n_rows > 0 ? (double)nnz / (double)n_rows : 1.0;
@@ -624,7 +638,7 @@ static int evolver_create(te_ctx* ctx, te_matrix* H, const void* psi0, bool psi0
     ev->n_samples = (long long)std::floor(params->t / params->sampling_step) + 1;          // :57
     ev->tolRate = params->tol / params->t;                                                // :287
     ev->numErrEst = (double)n * nrm1 * DBL_EPSILON;                                       // :289
-    ev->grid_stream = stream_grid(ctx, n);
+    ev->grid_stream = combine_grid(ctx, n);
     ev->grid_project = project_grid(ctx, n);
 
     TE_TRY(dmalloc(ctx, &ev->psi0, n));

@@ -166,11 +166,14 @@ struct te_matrix {
 
 namespace te {
 
-// Grid of the row-range streaming kernels (projection, combine) for work_items rows: a
-// multiple of 8 (one share per XCD), at most TE_STREAM_BLOCKS_PER_CU (default 4) blocks per CU.
+// Grid of the row-range streaming kernels (the adjoint's passes, the row-block SpMV) for
+// work_items rows: a multiple of 8 (one share per XCD), at most TE_STREAM_BLOCKS_PER_CU
+// (default 4) blocks per CU.
 int stream_grid(const te_ctx* ctx, int64_t work_items);
 // Grid of the Lanczos projection (TE_PROJECT_BLOCKS_PER_CU, default 2, blocks per CU).
 int project_grid(const te_ctx* ctx, int64_t rows);
+// Grid of the evolver's combine (TE_COMBINE_BLOCKS_PER_CU, default 2, blocks per CU).
+int combine_grid(const te_ctx* ctx, int64_t rows);
 // SpMV launch shape for a matrix (lanes per row from the mean row length, grid from the CUs).
 SpmvShape choose_shape(const te_ctx* ctx, int64_t n_rows, int64_t nnz);
 // Derives the row blocks of the stream kernel on the device and picks the kernel (call after
