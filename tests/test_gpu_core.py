@@ -188,7 +188,7 @@ def test_value_dictionary_storage(gpu, oracle_lib, monkeypatch, L):
         lens[:ob.D] = rowlen[order]
         slots = 64 * int(lens.reshape(ns, 64).max(1).sum())
         assert slots == _aligned_slots(_sell_rows(c, r, order))
-        assert nbytes == 4 * slots + 4 * (slots // 64) + 4 * (ns + 1) + 16 * ob.D
+        assert nbytes == 4 * slots + 4 * (slots // 64) + 4 * (ns + 1) + 8 * ob.D   # real diagonal
         _spmv_check(oracle_lib, M, ob.D, v, c, r)
     monkeypatch.setenv("TE_SPMV_VD", "0")
     P = capi.Matrix.from_coo(gpu, v, c, r, ob.D)
@@ -235,6 +235,41 @@ def test_sliced_ell_slots_beyond_16_bit_span(gpu, oracle_lib):
     assert slots > longest
     assert nbytes == 4 * slots + 4 * (slots // 64) + 4 * (ns + 1) + 16 * D
     _spmv_check(oracle_lib, A, D, v, c, r)
+
+
+def test_sliced_ell_diagonal_kinds(gpu, oracle_lib, monkeypatch):
+    """The sliced-ELL form streams the diagonal as 8-byte reals when every imaginary part is +0.0
+    bit for bit, and as 16-byte complex values otherwise: a complex diagonal, or a single -0.0
+    imaginary part (multiplying by (re, +0.0) instead of (re, -0.0) could change the sign of a
+    zero). Products match the oracle in every case, and the value-dictionary row-block kernel
+    bit for bit (same rows, diagonal first, then the entries in column order)."""
+    L = N = 8
+    ob = oracle_lib.Basis.basis(N, L)
+    v, c, r = oracle_lib.build_operator(ob, as_tuples(_ring_terms(L)))
+    on = c == r
+    assert np.all(np.ascontiguousarray(v[on].imag).view(np.uint64) == 0)
+    D = ob.D
+    ns = (D + 63) // 64
+    slots = _aligned_slots(_sell_rows(c, r, np.arange(D)))
+    fixed = 4 * slots + 4 * (slots // 64) + 4 * (ns + 1)
+    v_cplx = v.copy()
+    v_cplx[on] = v[on] * (1.0 + 0.25j)
+    v_negz = v.copy()
+    k = int(np.flatnonzero(on)[D // 2])
+    v_negz[k] = complex(v[k].real, -0.0)
+    x = random_state(D, 4)
+    for vals, diag_bytes in ((v, 8), (v_cplx, 16), (v_negz, 16)):
+        A = capi.Matrix.from_coo(gpu, vals, c, r, D)
+        fmt, _, nbytes = A.storage()
+        assert fmt == 4 and nbytes == fixed + diag_bytes * D
+        _spmv_check(oracle_lib, A, D, vals, c, r)
+        monkeypatch.setenv("TE_SPMV_SELL", "0")
+        B = capi.Matrix.from_coo(gpu, vals, c, r, D)
+        monkeypatch.delenv("TE_SPMV_SELL")
+        assert B.storage()[0] == 2
+        for alpha in (1.0, 0.2 - 0.7j):
+            ya, yb = A.spmv(x, alpha), B.spmv(x, alpha)
+            assert np.array_equal(ya.view(np.uint64), yb.view(np.uint64))
 
 
 def test_scattered_columns_keep_row_blocks(gpu, oracle_lib):

@@ -98,6 +98,10 @@ struct VdCsr {
     int64_t sell_entries = 0;
     int sell_grid = 0;
     int sell_chunk = 0;                 // SellView::chunk
+    // The real parts of diag when every imaginary part is +0.0 (bit pattern 0), else nullptr:
+    // the sliced-ELL kernels then stream 8 bytes a row for the diagonal and multiply by
+    // (re, +0.0) held in registers, the same arithmetic on the same operands.
+    double* diag_re = nullptr;
 };
 
 constexpr int kSlice = 64;
@@ -119,6 +123,7 @@ struct SellView {
     int64_t n;
     int chunk;      // 0: XCD x sweeps the x-th eighth of the slices; > 0: chunks of `chunk` slices
                     // dealt round-robin to the XCDs, so all XCDs sweep one front together
+    const double* diag_re;   // real diagonal (VdCsr::diag_re) or nullptr: read diag
 };
 
 // What the kernels read of a VdCsr.

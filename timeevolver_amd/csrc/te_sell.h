@@ -47,7 +47,7 @@ __device__ __forceinline__ void sell_slice_n(const SellView& m, const double2* c
     if (r < m.n) {
 #pragma unroll
         for (int v = 0; v < NV; v++) xr[v] = x[v][r];
-        d = load_stream(&m.diag[r]);
+        d = m.diag_re ? make_double2(__builtin_nontemporal_load(&m.diag_re[r]), 0.0) : load_stream(&m.diag[r]);
 #pragma unroll
         for (int v = 0; v < NV; v++) pre[v] = epi[v].load((int)r);
     }

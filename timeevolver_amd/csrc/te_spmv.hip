@@ -698,7 +698,7 @@ static VdView vd_view(const te_matrix* A) {
 static SellView sell_view(const te_matrix* A) {
     const VdCsr& v = A->vd;
     return SellView{v.sell_off, v.sell_word, v.sell_base, v.vtab,        v.n_values,
-                    v.diag,     v.n_slices,  A->n_rows,  v.sell_chunk};
+                    v.diag,     v.n_slices,  A->n_rows,  v.sell_chunk, v.diag_re};
 }
 
 SellView sell_view_of(const te_matrix* A) { return sell_view(A); }
@@ -1159,6 +1159,7 @@ void free_vd(te_matrix* A) {
     if (v.vidx) hipFree(v.vidx);
     if (v.vtab) hipFree(v.vtab);
     if (v.diag) hipFree(v.diag);
+    if (v.diag_re) hipFree(v.diag_re);
     if (v.blocks) hipFree(v.blocks);
     if (v.perm) hipFree(v.perm);
     if (v.iperm) hipFree(v.iperm);
@@ -1170,6 +1171,46 @@ void free_vd(te_matrix* A) {
 }
 
 // Sliced-ELL form of A->vd (rows of at most 255 off-diagonal entries).
+// re[i] = d[i].x; *complex_flag = 1 if any d[i].y is not +0.0 (bit pattern 0).
+__global__ void diag_real_kernel(const double2* __restrict__ d, int64_t n, double* __restrict__ re, int* complex_flag) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double2 v = d[i];
+    re[i] = v.x;
+    if (__double_as_longlong(v.y) != 0) *complex_flag = 1;
+}
+
+// The real diagonal of a sliced-ELL matrix (VdCsr::diag_re), when it has one.
+static int build_diag_re(te_ctx* ctx, VdCsr& v, int64_t n) {
+    if (v.diag_re) hipFree(v.diag_re);
+    v.diag_re = nullptr;
+    if (n <= 0) return TE_OK;
+    double* re = nullptr;
+    int* flag = nullptr;
+    TE_HIP(ctx, hipMalloc(&re, sizeof(double) * n));
+    if (hipMalloc(&flag, sizeof(int)) != hipSuccess) {
+        hipFree(re);
+        return fail(ctx, TE_ERR_HIP, "hipMalloc failed for the diagonal flag");
+    }
+    int h = 0;
+    hipError_t e = hipMemsetAsync(flag, 0, sizeof(int), ctx->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(diag_real_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, v.diag, n, re,
+                           flag);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    hipFree(flag);
+    if (e != hipSuccess || h != 0) {
+        hipFree(re);
+        if (e != hipSuccess) return fail(ctx, TE_ERR_HIP, "real-diagonal check failed: %s", hipGetErrorString(e));
+        return TE_OK;
+    }
+    v.diag_re = re;
+    return TE_OK;
+}
+
 static int build_sell(te_ctx* ctx, te_matrix* A) {
     VdCsr& v = A->vd;
     hipStream_t st = ctx->stream;
@@ -1233,6 +1274,7 @@ static int build_sell(te_ctx* ctx, te_matrix* A) {
         const char* chunk = std::getenv("TE_SELL_CHUNK");          // sweep-schedule sweeps
         v.sell_chunk = chunk ? std::max(0, std::atoi(chunk)) : 256;
         v.sell = true;
+        rc = build_diag_re(ctx, v, n);
     }
     hipFree(slen);
     hipFree(soff);
