@@ -176,7 +176,7 @@ def main():
         "kernel_launches_per_run": kernel_launches,
         "roofline": {"kernel": kname + " (fused SpMV + axpy + dot)", "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": measured_traffic(kname, D),
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": measured_traffic(kname, D, b_apply),
                      "bytes_per_launch": b_apply, "mean_launch_us": round(t_apply * 1e6, 2),
                      "launches_timed": apply_n},
     }
@@ -339,10 +339,11 @@ def bench_adjoint(np, capi, ctx, args):
     return out
 
 
-def measured_traffic(kernel, D):
+def measured_traffic(kernel, D, algorithmic_bytes):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc summary
     (profiles/traffic.json: FETCH_SIZE and WRITE_SIZE scaled by a plain-copy calibration), when
-    it was measured on this workload; None otherwise."""
+    it was measured on this workload with this storage (same algorithmic bytes per launch);
+    None otherwise."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(p) as f:
@@ -350,6 +351,8 @@ def measured_traffic(kernel, D):
     except (OSError, ValueError):
         return None
     if not t or ("D=%d" % D) not in t.get("workload", ""):
+        return None
+    if t.get("algorithmic_bytes") != algorithmic_bytes:
         return None
     return t["fetch_bytes"] + t["write_bytes"]
 
