@@ -61,8 +61,9 @@ int te_matrix_info(const te_matrix* A, size_t* n_rows, size_t* n_cols, size_t* n
  * diagonal), 3 the same in a locality order of the rows (lexicographic order of the basis
  * states for operators from te_operator_build), 4 its sliced-ELL form (64-row slices stored
  * slot-major; 4-byte slots of a 16-bit column offset and a 16-bit value index, one base column
- * per slot of a slice). With format 3, and with 4 in the locality order, te_spmv reads the
- * CSR row blocks in the basis order. stream_bytes: bytes of matrix data one product reads. */
+ * per slot of a slice; the dense diagonal as 8-byte reals when every imaginary part is +0.0,
+ * else 16-byte complex values). With format 3, and with 4 in the locality order, te_spmv reads
+ * the CSR row blocks in the basis order. stream_bytes: bytes of matrix data one product reads. */
 int te_matrix_storage(const te_matrix* A, int* format, size_t* n_values, size_t* stream_bytes);
 /* smatrix::norm1 (core/matrixDataTypes.cpp:67-79): maximal absolute column sum. */
 int te_matrix_norm1(te_matrix* A, double* out);
