@@ -1268,11 +1268,12 @@ static int build_sell(te_ctx* ctx, te_matrix* A) {
         const int cap = std::max(8, ctx->num_cus * std::min(want, per_cu));
         const int blocks = (ns + kBlock / 64 - 1) / (kBlock / 64);
         v.sell_grid = (int)((std::min(blocks, cap) + 7) / 8 * 8);
-        // All XCDs sweep one front, 256 slices (16,384 rows) per chunk: on the L = N = 14 ring
+        // All XCDs sweep one front, 512 slices (32,768 rows) per chunk: on the L = N = 14 ring
         // 6 % faster than a contiguous eighth per XCD (the far hops' windows of x are then shared
-        // in the MALL instead of eight separate ones); 128..2048 within 1 %.
+        // in the MALL instead of eight separate ones); apply 383.0-383.5 ms a run at 512 slices
+        // a chunk, 385.0-385.4 at 256, 384.1 at 1024, 385.2 at 128.
         const char* chunk = std::getenv("TE_SELL_CHUNK");          // sweep-schedule sweeps
-        v.sell_chunk = chunk ? std::max(0, std::atoi(chunk)) : 256;
+        v.sell_chunk = chunk ? std::max(0, std::atoi(chunk)) : 512;
         v.sell = true;
         rc = build_diag_re(ctx, v, n);
     }
