@@ -194,8 +194,7 @@ struct TermArgs {
     int group;
     double* zsum;
     double* partials;           // gridDim.x x n_labels
-    unsigned int* ticket;
-    double* out;                // n_labels running sums
+    double* out;                // n_labels running sums (termwise_finish_kernel adds to them)
 };
 
 template <int LCAP, bool TABLE>
@@ -274,14 +273,20 @@ __global__ __launch_bounds__(256) void termwise_kernel(TermArgs a) {
         const double2 s = block_sum2<256>(make_double2(acc[l * 256 + tid], 0.0), sh);
         if (tid == 0) a.partials[(size_t)blockIdx.x * a.n_labels + l] = s.x;
     }
-    if (last_block_arrive(a.ticket)) {
-        for (int l = 0; l < a.n_labels; l++) {
-            double t = 0.0;
-            for (int b = tid; b < (int)gridDim.x; b += 256) t += a.partials[(size_t)b * a.n_labels + l];
-            const double2 s = block_sum2<256>(make_double2(t, 0.0), sh);
-            if (tid == 0) a.out[l] += s.x;
-        }
-        if (tid == 0) *a.ticket = 0;
+}
+
+// One block after termwise_kernel: out[l] += the nparts blocks' partials of label l, summed in
+// block order (in a launch of its own: a last-block reduction inside the term-wise pass would
+// cost each of its blocks an agent-scope release, an L2 write-back on this multi-XCD part).
+__global__ __launch_bounds__(256) void termwise_finish_kernel(const double* __restrict__ partials, int nparts,
+                                                              int n_labels, double* out) {
+    __shared__ double2 sh[4];
+    const int tid = threadIdx.x;
+    for (int l = 0; l < n_labels; l++) {
+        double t = 0.0;
+        for (int b = tid; b < nparts; b += 256) t += partials[(size_t)b * n_labels + l];
+        const double2 s = block_sum2<256>(make_double2(t, 0.0), sh);
+        if (tid == 0) out[l] += s.x;
     }
 }
 
@@ -321,6 +326,7 @@ static void launch_termwise_t(const TermArgs& ta, int grid, hipStream_t s) {
 static void launch_termwise(const TermArgs& ta, int grid, hipStream_t s) {
     if (ta.vtab) launch_termwise_t<true>(ta, grid, s);
     else launch_termwise_t<false>(ta, grid, s);
+    hipLaunchKernelGGL(termwise_finish_kernel, dim3(1), dim3(256), 0, s, ta.partials, grid, ta.n_labels, ta.out);
 }
 
 }  // namespace te
@@ -1040,7 +1046,6 @@ static int backward(te_adjoint* a) {
                 ta.group = node0 / kNodeGroup + g;
                 ta.zsum = a->zsum;
                 ta.partials = a->partials;
-                ta.ticket = a->ticket;
                 ta.out = a->offacc;
                 launch_termwise(ta, tg, s);
             }

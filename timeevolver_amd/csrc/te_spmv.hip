@@ -201,23 +201,17 @@ __device__ __forceinline__ double project_front(const LanczosArgs& a) {
     return nrm;
 }
 
+// Ends at each block's partial of |w'|^2, as the sliced-ELL apply does (lanczos_finish_kernel).
 __global__ __launch_bounds__(kBlock) void lanczos_project_kernel(LanczosArgs a) {
     if (a.ks->broken) return;
     __shared__ double2 sh[kBlock / 64];
     const double nrm = project_front(a);
     double2 bs = block_sum2<kBlock>(make_double2(nrm, 0.0), sh);
     if (threadIdx.x == 0) a.partials[blockIdx.x] = bs;
-    if (last_block_arrive(a.ticket)) {
-        double2 tot = reduce_partials<kBlock>(a.partials, gridDim.x, 1, sh);
-        if (threadIdx.x == 0) {
-            project_finish(a, tot);
-            *a.ticket = 0;
-        }
-    }
 }
 
-// The projections of two recurrences in one launch (a and b: own partials; a's ticket); one that
-// broke down is skipped. Each gets the row ranges, partials and reduction order of its own launch.
+// The projections of two recurrences in one launch (a and b: own partials); one that broke down
+// is skipped. Each gets the row ranges and partials of its own launch.
 __global__ __launch_bounds__(kBlock) void lanczos_project2_kernel(LanczosArgs a, LanczosArgs b) {
     const bool la = !a.ks->broken, lb = !b.ks->broken;
     if (!la && !lb) return;
@@ -229,15 +223,6 @@ __global__ __launch_bounds__(kBlock) void lanczos_project2_kernel(LanczosArgs a,
     if (threadIdx.x == 0) {
         a.partials[blockIdx.x] = bs0;
         b.partials[blockIdx.x] = bs1;
-    }
-    if (last_block_arrive(a.ticket)) {
-        const double2 t0 = reduce_partials<kBlock>(a.partials, gridDim.x, 1, sh);
-        const double2 t1 = reduce_partials<kBlock>(b.partials, gridDim.x, 1, sh);
-        if (threadIdx.x == 0) {
-            if (la) project_finish(a, t0);
-            if (lb) project_finish(b, t1);
-            *a.ticket = 0;
-        }
     }
 }
 
@@ -561,25 +546,23 @@ __device__ __forceinline__ ApplyEpiVd apply_epi(const LanczosArgs& a) {
     return epi;
 }
 
-// The last block's finish of an apply: alpha_j = s_j sum of the block partials.
+// The finish of an apply: alpha_j = s_j sum of the block partials.
 __device__ __forceinline__ void apply_finish(const LanczosArgs& a, double2 tot) {
     const double sj = a.ks->s[a.j];
     a.ks->a[a.j] = make_double2(sj * tot.x, sj * tot.y);
 }
 
+// The sliced-ELL apply ends at each block's partial of <u_j, w>; lanczos_finish_kernel sums
+// them in block order in a launch of its own. A last-block reduction inside the apply costs
+// every block an agent-scope release, which on this multi-XCD part writes back the XCD's L2
+// (a one-front stream of the projection's shape, 20 M rows: 240 us with the in-kernel
+// reduction, 182 us with partials and a separate one-block finish).
 __device__ __forceinline__ void apply_sell_body(const LanczosArgs& a, double2* vt) {
     ApplyEpiVd epi = apply_epi(a);
     const double2* x = a.U + (size_t)a.j * a.ld;
     sell_sweep(a.sell, vt, [&](int s, int lane) { sell_slice(a.sell, x, vt, s, lane, epi); });
     double2 bs = block_sum2<kBlock>(epi.dot, vt);       // the table's LDS is free again
     if (threadIdx.x == 0) a.partials[blockIdx.x] = bs;
-    if (last_block_arrive(a.ticket)) {
-        double2 tot = reduce_partials<kBlock>(a.partials, gridDim.x, 1, vt);
-        if (threadIdx.x == 0) {
-            apply_finish(a, tot);
-            *a.ticket = 0;
-        }
-    }
 }
 
 __global__ __launch_bounds__(kBlock) void lanczos_apply_sell_kernel(LanczosArgs a) {
@@ -589,9 +572,9 @@ __global__ __launch_bounds__(kBlock) void lanczos_apply_sell_kernel(LanczosArgs 
 }
 
 // Two Lanczos recurrences on the same matrix, iteration j of each, in one pass over its words
-// (a and b: own bases, scalars and partials; a's ticket). A recurrence that broke down is
-// skipped; with one left the single-vector pass runs. Each vector gets the sums, partials and
-// reduction order of its own apply.
+// (a and b: own bases, scalars and partials). A recurrence that broke down is skipped; with
+// one left the single-vector pass runs. Each vector gets the sums and partials of its own
+// apply (and, in lanczos_finish_kernel, its reduction order).
 __global__ __launch_bounds__(kBlock) void lanczos_apply_sell2_kernel(LanczosArgs a, LanczosArgs b) {
     const bool la = !a.ks->broken, lb = !b.ks->broken;
     __shared__ double2 vt[kVdMaxValues];
@@ -609,13 +592,29 @@ __global__ __launch_bounds__(kBlock) void lanczos_apply_sell2_kernel(LanczosArgs
         a.partials[blockIdx.x] = bs0;
         b.partials[blockIdx.x] = bs1;
     }
-    if (last_block_arrive(a.ticket)) {
-        const double2 t0 = reduce_partials<kBlock>(a.partials, gridDim.x, 1, vt);
-        const double2 t1 = reduce_partials<kBlock>(b.partials, gridDim.x, 1, vt);
+}
+
+// One block, launched after a sliced-ELL apply (PROJECT false: alpha_j) or a projection
+// (PROJECT true: beta_j, s_{j+1} or the breakdown) of one recurrence, or of two (`two`: b with
+// its own partials): sums the launch's nparts block partials in block order. A recurrence that
+// had broken down before that launch wrote no partials and is skipped here too (only this
+// kernel sets the flag, after both were read).
+template <bool PROJECT>
+__global__ __launch_bounds__(kBlock) void lanczos_finish_kernel(LanczosArgs a, LanczosArgs b, int two, int nparts) {
+    __shared__ double2 sh[kBlock / 64];
+    const bool la = !a.ks->broken, lb = two && !b.ks->broken;
+    if (la) {
+        const double2 t = reduce_partials<kBlock>(a.partials, nparts, 1, sh);
         if (threadIdx.x == 0) {
-            apply_finish(a, t0);
-            apply_finish(b, t1);
-            *a.ticket = 0;
+            if (PROJECT) project_finish(a, t);
+            else apply_finish(a, t);
+        }
+    }
+    if (lb) {
+        const double2 t = reduce_partials<kBlock>(b.partials, nparts, 1, sh);
+        if (threadIdx.x == 0) {
+            if (PROJECT) project_finish(b, t);
+            else apply_finish(b, t);
         }
     }
 }
@@ -742,6 +741,7 @@ void launch_lanczos_apply(LanczosArgs a, const te_matrix* A, hipStream_t s) {
         const VdCsr& v = A->vd;
         a.sell = sell_view(A);
         hipLaunchKernelGGL(lanczos_apply_sell_kernel, dim3(v.sell_grid), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL(lanczos_finish_kernel<false>, dim3(1), dim3(kBlock), 0, s, a, a, 0, v.sell_grid);
         return;
     }
     if (A->use_vd) {
@@ -1442,13 +1442,15 @@ int prepare_spmv(te_ctx* ctx, te_matrix* A, const int* perm) {
 
 void launch_lanczos_project(const LanczosArgs& a, int grid, hipStream_t s) {
     hipLaunchKernelGGL(lanczos_project_kernel, dim3(grid), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(lanczos_finish_kernel<true>, dim3(1), dim3(kBlock), 0, s, a, a, 0, grid);
 }
 
 void launch_lanczos_apply2(LanczosArgs a, LanczosArgs b, const te_matrix* A, int grid, hipStream_t s) {
     if (A->use_vd && A->vd.sell) {
         a.sell = b.sell = sell_view(A);
-        hipLaunchKernelGGL(lanczos_apply_sell2_kernel, dim3(grid > 0 ? grid : A->vd.sell_grid), dim3(kBlock), 0, s, a,
-                           b);
+        const int g = grid > 0 ? grid : A->vd.sell_grid;
+        hipLaunchKernelGGL(lanczos_apply_sell2_kernel, dim3(g), dim3(kBlock), 0, s, a, b);
+        hipLaunchKernelGGL(lanczos_finish_kernel<false>, dim3(1), dim3(kBlock), 0, s, a, b, 1, g);
         return;
     }
     launch_lanczos_apply(a, A, s);
@@ -1457,6 +1459,7 @@ void launch_lanczos_apply2(LanczosArgs a, LanczosArgs b, const te_matrix* A, int
 
 void launch_lanczos_project2(const LanczosArgs& a, const LanczosArgs& b, int grid, hipStream_t s) {
     hipLaunchKernelGGL(lanczos_project2_kernel, dim3(grid), dim3(kBlock), 0, s, a, b);
+    hipLaunchKernelGGL(lanczos_finish_kernel<true>, dim3(1), dim3(kBlock), 0, s, a, b, 1, grid);
 }
 
 void launch_zdotc(const double2* x, const double2* y, int64_t n, ReduceScratch& rs, int slot, hipStream_t s) {
