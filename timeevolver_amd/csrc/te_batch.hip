@@ -14,13 +14,13 @@
 //    the project/combine passes walk vector units (up to vrows rows of one instance); each unit
 //    first reads its instance's flags and skips the instance when it is finished, broken down,
 //    or past its Krylov dimension;
-//  * reductions are per instance and deterministic: a unit writes its block partial, the last
-//    unit of the instance to arrive (per-instance ticket) sums the instance's partials in unit
-//    order and resets the ticket;
+//  * reductions are per instance and deterministic: a unit writes its block partial and
+//    batch_finish_kernel (one block per instance, a launch of its own) sums the instance's
+//    partials in unit order;
 //  * the step controller runs one block per instance (launch_batch_step_control);
 //  * one round (m_max Lanczos iterations, controller, combine, observable sums, live count) is
 //    captured once as a HIP graph and replayed until no instance is live. The graph holds
-//    kernel nodes only (tickets are reset by the kernels themselves).
+//    kernel nodes only.
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -37,14 +37,15 @@ namespace te {
 
 constexpr int kBatchRB = 4;          // row blocks (kStreamChunk entries + rows each) per CSR SpMV unit
 // Slices (kSlice rows each) per sliced-ELL SpMV unit: the batch's slices per block of the grid,
-// clamped to [16, 128]. Each unit ends with a block reduction and a ticket and may restage its
-// instance's value table, so units should be large, but a small batch still needs units for
-// every block: on 512 ring instances with L in {6..10} (181k slices) the batched run took
-// 0.381 s with 16-slice units, 0.281 s with 32, 0.247 s with 64, 0.244 s with 128.
-constexpr int kBatchSlicesMin = 16, kBatchSlicesMax = 128;
+// clamped to [16, 32]. Each unit ends with a block reduction and may restage its instance's
+// value table; with the instance sums in batch_finish_kernel small units balance best: on 512
+// ring instances with L in {6..10} (181k slices, 4096-row vector units) the batched run took
+// 0.1473 / 0.1462 / 0.1470 / 0.1489 s with 16 / 24 / 32 / 48-slice units, 0.1601 s with 88.
+constexpr int kBatchSlicesMin = 16, kBatchSlicesMax = 32;
 // Rows per vector unit (projection, combine), grown when the observable partials would be large:
-// 4096 / 8192 / 16384 / 32768 rows gave 0.269 / 0.252 / 0.247 / 0.260 s on the same batch.
-constexpr int kBatchVecRows = 16384;
+// 2048 / 4096 rows gave 0.1467 / 0.1470 s with 32-slice units; 4096 / 8192 / 16384 rows
+// 0.1601 / 0.1628 / 0.1667 s with 88-slice units.
+constexpr int kBatchVecRows = 4096;
 constexpr int kBatchSC = 8;          // samples per pass of the combine kernel
 constexpr int kBatchMaxObs = 4;      // diagonal observables per instance
 constexpr int kBatchTraceCap = 4096; // recorded step sizes per instance
@@ -75,9 +76,7 @@ struct BatchDev {
     StepState* st;
     BatchCtl* bc;
     double2* spart;         // per SpMV unit
-    unsigned int* sticket;  // per instance
     double2* vpart;         // per vector unit
-    unsigned int* vticket;  // per instance
     const double2* wcoef;   // per instance, kMaxKrylov
     const double2* Y;       // per instance, n_samples x kMaxKrylov
     long long n_samples;
@@ -90,20 +89,6 @@ struct BatchDev {
     int n_inst;
     int* n_live;
 };
-
-// The last block of an instance's units to arrive (ticket) returns true in all its threads.
-__device__ __forceinline__ bool last_unit_arrive(unsigned int* ticket, unsigned int count) {
-    __shared__ bool last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned int t = atomicAdd(ticket, 1u);
-        last = (t == count - 1);
-    }
-    __syncthreads();
-    if (last) __threadfence();
-    return last;
-}
 
 // Whether instance i takes part in Lanczos iteration j of this round (uniform in a block).
 __device__ __forceinline__ bool lanczos_active(const BatchDev& a, int i, int j) {
@@ -157,13 +142,6 @@ __global__ __launch_bounds__(kBlock) void batch_apply_kernel(BatchDev a, int j) 
             stream_row_block(I.rb[b], I.rb[b + 1], I.ptr, I.col, I.val, epi.uj, prod, epi, nullptr);
         const double2 bs = block_sum2<kBlock>(epi.dot, sh);
         if (threadIdx.x == 0) a.spart[u] = bs;
-        if (last_unit_arrive(a.sticket + i, (unsigned)I.nsu)) {
-            const double2 tot = reduce_partials<kBlock>(a.spart + I.su0, I.nsu, 1, sh);
-            if (threadIdx.x == 0) {
-                ks->a[j] = make_double2(sj * tot.x, sj * tot.y);
-                a.sticket[i] = 0;
-            }
-        }
     }
 }
 
@@ -203,13 +181,6 @@ __global__ __launch_bounds__(kBlock) void batch_apply_sell_kernel(BatchDev a, in
             sell_slice(m, x, vt, __builtin_amdgcn_readfirstlane(sl), lane, epi);
         const double2 bs = block_sum2<kBlock>(epi.dot, sh);
         if (threadIdx.x == 0) a.spart[u] = bs;
-        if (last_unit_arrive(a.sticket + i, (unsigned)I.nsu)) {
-            const double2 tot = reduce_partials<kBlock>(a.spart + I.su0, I.nsu, 1, sh);
-            if (threadIdx.x == 0) {
-                ks->a[j] = make_double2(sj * tot.x, sj * tot.y);
-                a.sticket[i] = 0;
-            }
-        }
     }
 }
 
@@ -260,23 +231,42 @@ __global__ __launch_bounds__(kBlock) void batch_project_kernel(BatchDev a, int j
         }
         const double2 bs = block_sum2<kBlock>(make_double2(nrm, 0.0), sh);
         if (threadIdx.x == 0) a.vpart[u] = bs;
-        if (last_unit_arrive(a.vticket + i, (unsigned)I.nvu)) {
-            const double2 tot = reduce_partials<kBlock>(a.vpart + I.vu0, I.nvu, 1, sh);
-            if (threadIdx.x == 0) {
-                const double beta = sqrt(tot.x);
-                if (beta < a.tolRate) {
-                    ks->broken = 1;
-                    ks->m_hbd = j + 1;
-                    ks->h = beta;
-                } else if (j + 1 == m) {
-                    ks->h = beta;
-                } else {
-                    ks->beta[j] = beta;
-                    ks->s[j + 1] = 1.0 / beta;
-                }
-                a.vticket[i] = 0;
-            }
-        }
+    }
+}
+
+// One block per instance after an apply (PROJECT false: alpha_j = s_j <u_j, w>) or a projection
+// (PROJECT true: beta_j = |w'|, stored with s_{j+1} = 1 / beta_j, or h when j + 1 is the Krylov
+// dimension; breakdown when beta_j < tolRate, core/krylovTimeEvolver.cpp:551-567) of iteration j:
+// the instance's unit partials summed in unit order. In a launch of its own: a reduction by the
+// instance's last unit inside the apply or projection costs every unit an agent-scope release,
+// which on this multi-XCD part writes back the XCD's L2 (per-unit tickets made the projection
+// of 512 instances about 1.7x slower in a probe of its shape).
+template <bool PROJECT>
+__global__ __launch_bounds__(kBlock) void batch_finish_kernel(BatchDev a, int j) {
+    __shared__ double2 sh[kBlock / 64];
+    const int i = blockIdx.x;
+    if (i >= a.n_inst || !lanczos_active(a, i, j)) return;
+    const BatchInst& I = a.inst[i];
+    KrylovScalars* ks = a.ks + i;
+    const double2 tot = PROJECT ? reduce_partials<kBlock>(a.vpart + I.vu0, I.nvu, 1, sh)
+                                : reduce_partials<kBlock>(a.spart + I.su0, I.nsu, 1, sh);
+    if (threadIdx.x != 0) return;
+    if (!PROJECT) {
+        const double sj = ks->s[j];
+        ks->a[j] = make_double2(sj * tot.x, sj * tot.y);
+        return;
+    }
+    const int m = a.st[i].m;
+    const double beta = sqrt(tot.x);
+    if (beta < a.tolRate) {
+        ks->broken = 1;
+        ks->m_hbd = j + 1;
+        ks->h = beta;
+    } else if (j + 1 == m) {
+        ks->h = beta;
+    } else {
+        ks->beta[j] = beta;
+        ks->s[j + 1] = 1.0 / beta;
     }
 }
 
@@ -451,7 +441,6 @@ struct te_batch {
     int4 *sunit = nullptr, *vunit = nullptr;
     double2 *U = nullptr, *psi0 = nullptr, *sampled = nullptr, *wcoef = nullptr, *Y = nullptr;
     double2 *spart = nullptr, *vpart = nullptr;
-    unsigned int *sticket = nullptr, *vticket = nullptr;
     KrylovScalars* ks = nullptr;
     StepState* st = nullptr;
     BatchCtl* bc = nullptr;
@@ -476,7 +465,7 @@ static void batch_free(te_batch* b) {
     if (b->exec) hipGraphExecDestroy(b->exec);
     if (b->graph) hipGraphDestroy(b->graph);
     void* ptrs[] = {b->inst, b->sunit,   b->vunit,  b->U,  b->psi0, b->sampled, b->wcoef, b->Y, b->tmp,
-                    b->spart, b->vpart,  b->sticket, b->vticket, b->ks, b->st, b->bc, b->consts,
+                    b->spart, b->vpart,  b->ks, b->st, b->bc, b->consts,
                     b->diag, b->opart,   b->obs,    b->nrm, b->trace, b->n_live};
     for (void* p : ptrs)
         if (p) hipFree(p);
@@ -497,7 +486,9 @@ static void batch_round(te_batch* b, hipStream_t s) {
     for (int j = 0; j < b->m_max; j++) {
         if (b->sell) hipLaunchKernelGGL(batch_apply_sell_kernel, dim3(b->grid_s), dim3(kBlock), 0, s, d, j);
         else hipLaunchKernelGGL(batch_apply_kernel, dim3(b->grid_s), dim3(kBlock), 0, s, d, j);
+        hipLaunchKernelGGL(batch_finish_kernel<false>, dim3(b->n_inst), dim3(kBlock), 0, s, d, j);
         hipLaunchKernelGGL(batch_project_kernel, dim3(b->grid_v), dim3(kBlock), 0, s, d, j);
+        hipLaunchKernelGGL(batch_finish_kernel<true>, dim3(b->n_inst), dim3(kBlock), 0, s, d, j);
     }
     launch_batch_step_control(b->consts, b->st, b->ks, b->bc, b->n_inst, b->n_samples, s);
 }
@@ -674,8 +665,6 @@ extern "C" int te_batch_create(te_ctx* ctx, int n_inst, te_matrix* const* H, con
     TE_TRY(balloc(ctx, &b->Y, (size_t)n_inst * ns * kMaxKrylov));
     TE_TRY(balloc(ctx, &b->spart, hsu.size()));
     TE_TRY(balloc(ctx, &b->vpart, hvu.size()));
-    TE_TRY(balloc(ctx, &b->sticket, n_inst));
-    TE_TRY(balloc(ctx, &b->vticket, n_inst));
     TE_TRY(balloc(ctx, &b->ks, n_inst));
     TE_TRY(balloc(ctx, &b->st, n_inst));
     TE_TRY(balloc(ctx, &b->bc, n_inst));
@@ -693,8 +682,6 @@ extern "C" int te_batch_create(te_ctx* ctx, int n_inst, te_matrix* const* H, con
     TE_TRYH(hipMemcpyAsync(b->inst, hinst.data(), sizeof(BatchInst) * n_inst, hipMemcpyHostToDevice, s));
     TE_TRYH(hipMemcpyAsync(b->sunit, hsu.data(), sizeof(int4) * hsu.size(), hipMemcpyHostToDevice, s));
     TE_TRYH(hipMemcpyAsync(b->vunit, hvu.data(), sizeof(int4) * hvu.size(), hipMemcpyHostToDevice, s));
-    TE_TRYH(hipMemsetAsync(b->sticket, 0, sizeof(unsigned) * n_inst, s));
-    TE_TRYH(hipMemsetAsync(b->vticket, 0, sizeof(unsigned) * n_inst, s));
     TE_TRYH(hipMemsetAsync(b->psi0, 0, sizeof(double2) * total, s));
     TE_TRYH(hipMemsetAsync(b->diag, 0, sizeof(double) * std::max(n_obs, 1) * total, s));
     std::vector<StepConsts> hc(n_inst);
@@ -750,9 +737,7 @@ extern "C" int te_batch_create(te_ctx* ctx, int n_inst, te_matrix* const* H, con
     d.st = b->st;
     d.bc = b->bc;
     d.spart = b->spart;
-    d.sticket = b->sticket;
     d.vpart = b->vpart;
-    d.vticket = b->vticket;
     d.wcoef = b->wcoef;
     d.Y = b->Y;
     d.n_samples = b->n_samples;
