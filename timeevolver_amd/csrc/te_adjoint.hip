@@ -842,7 +842,6 @@ static int forward(te_adjoint* a, te_result* R) {
     cb.obs_values = a->obs;
     cb.n_samples = n_samples;
     cb.partials = a->rs.partials;
-    cb.ticket = a->rs.ticket;
     // sample 0: <O> of psi0
     {
         CombineArgs c0 = cb;

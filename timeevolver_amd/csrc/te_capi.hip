@@ -810,7 +810,6 @@ static CombineArgs combine_base(te_evolver* ev, int m) {
     a.obs_values = ev->diag_vals;
     a.n_samples = ev->n_samples;
     a.partials = ev->rs.partials;
-    a.ticket = ev->rs.ticket;
     return a;
 }
 

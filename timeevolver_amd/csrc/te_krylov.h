@@ -91,8 +91,7 @@ struct CombineArgs {
     int obs_base;                       // index of the first of these observables in obs_values
     double* obs_values;                 // total_obs x n_samples
     long long n_samples;
-    double2* partials;
-    unsigned int* ticket;
+    double2* partials;                  // grid x (observables x samples of the launch)
 };
 
 // Quadrature tables of the error integrals (tanh-sinh rows, Gauss-Legendre 15 and 7) in device
@@ -154,7 +153,7 @@ struct BatchCtl {
 void launch_lanczos_apply(LanczosArgs a, const te_matrix* A, hipStream_t s);
 void launch_lanczos_project(const LanczosArgs& a, int grid, hipStream_t s);
 // Iteration j of two recurrences on the same matrix (a.j == b.j, same n and m; own bases,
-// scalars and partials, a's ticket): one pass over the matrix words for both when A is held as
+// scalars and partials): one pass over the matrix words for both when A is held as
 // sliced ELL (grid blocks, a multiple of 8; 0: the single apply's grid), else one apply after
 // the other. Results as from two single launches of the same grid.
 void launch_lanczos_apply2(LanczosArgs a, LanczosArgs b, const te_matrix* A, int grid, hipStream_t s);

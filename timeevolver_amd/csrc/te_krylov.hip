@@ -1004,14 +1004,20 @@ __global__ __launch_bounds__(kBlock) void combine_kernel(CombineArgs a) {
         const double2 t = block_sum2<kBlock>(make_double2(acc[i], 0.0), sh);
         if (tid == 0) a.partials[(size_t)blockIdx.x * (NOB * SC) + i] = t;
     }
-    if (last_block_arrive(a.ticket)) {
-        for (int i = 0; i < NOB * SC; i++) {
-            const int o = i / SC, s = i % SC;
-            const double2 tot = reduce_partials<kBlock>(a.partials + i, gridDim.x, NOB * SC, sh);
-            if (tid == 0 && o < a.nobs && s < S)
-                a.obs_values[(size_t)(a.obs_base + o) * a.n_samples + a.y_base + s] = tot.x;
-        }
-        if (tid == 0) *a.ticket = 0;
+}
+
+// One block after combine_kernel<SC, NOB>: each observable's value at each sample, the nparts
+// blocks' partials summed in block order (a launch of its own rather than a last-block
+// reduction, which would cost every block of the combine an agent-scope release: an L2
+// write-back on this multi-XCD part).
+template <int SC, int NOB>
+__global__ __launch_bounds__(kBlock) void combine_finish_kernel(CombineArgs a, int nparts) {
+    __shared__ double2 sh[kBlock / 64];
+    for (int i = 0; i < NOB * SC; i++) {
+        const int o = i / SC, s = i % SC;
+        const double2 tot = reduce_partials<kBlock>(a.partials + i, nparts, NOB * SC, sh);
+        if (threadIdx.x == 0 && o < a.nobs && s < a.S)
+            a.obs_values[(size_t)(a.obs_base + o) * a.n_samples + a.y_base + s] = tot.x;
     }
 }
 
@@ -1039,15 +1045,21 @@ int combine_samples_per_launch(int nobs) {
     return 1;
 }
 
+template <int SC, int NOB>
+static void launch_combine_t(const CombineArgs& a, int grid, hipStream_t s) {
+    hipLaunchKernelGGL((combine_kernel<SC, NOB>), dim3(grid), dim3(kBlock), 0, s, a);
+    if constexpr (NOB > 0)
+        hipLaunchKernelGGL((combine_finish_kernel<SC, NOB>), dim3(1), dim3(kBlock), 0, s, a, grid);
+}
+
 void launch_combine(const CombineArgs& a, int grid, hipStream_t s) {
-    const dim3 g(grid), b(kBlock);
-    if (a.nobs == 0) hipLaunchKernelGGL((combine_kernel<8, 0>), g, b, 0, s, a);
-    else if (a.nobs <= 1) hipLaunchKernelGGL((combine_kernel<8, 1>), g, b, 0, s, a);
-    else if (a.nobs <= 2) hipLaunchKernelGGL((combine_kernel<8, 2>), g, b, 0, s, a);
-    else if (a.nobs <= 4) hipLaunchKernelGGL((combine_kernel<8, 4>), g, b, 0, s, a);
-    else if (a.nobs <= 8) hipLaunchKernelGGL((combine_kernel<4, 8>), g, b, 0, s, a);
-    else if (a.nobs <= 16) hipLaunchKernelGGL((combine_kernel<2, 16>), g, b, 0, s, a);
-    else hipLaunchKernelGGL((combine_kernel<1, 32>), g, b, 0, s, a);
+    if (a.nobs == 0) launch_combine_t<8, 0>(a, grid, s);
+    else if (a.nobs <= 1) launch_combine_t<8, 1>(a, grid, s);
+    else if (a.nobs <= 2) launch_combine_t<8, 2>(a, grid, s);
+    else if (a.nobs <= 4) launch_combine_t<8, 4>(a, grid, s);
+    else if (a.nobs <= 8) launch_combine_t<4, 8>(a, grid, s);
+    else if (a.nobs <= 16) launch_combine_t<2, 16>(a, grid, s);
+    else launch_combine_t<1, 32>(a, grid, s);
 }
 
 void launch_store_obs(const double2* res, int slot, int mode, double* out, hipStream_t s) {
