@@ -759,16 +759,16 @@ static void lanczos2(te_adjoint* a, double2* U1, KrylovScalars* ks1, double2* U2
     lb.partials = a->rs.partials + (size_t)a->rs.grid * 32;
     lb.ticket = a->rs.ticket + 1;
     // launch shapes of the paired kernels (blocks per CU; launch-shape sweeps:
-    // TE_ADJ_APPLY2_BLOCKS_PER_CU, TE_ADJ_PROJECT2_BLOCKS_PER_CU)
-    // (on the L = N = 12 ring, K = 1: 2 and 1 took 74.3 ms a run, 3 and 2 (the single kernels'
-    // shapes) 78.5 ms, 4 and more blocks per CU were slower still)
+    // TE_ADJ_APPLY2_BLOCKS_PER_CU, TE_ADJ_PROJECT2_BLOCKS_PER_CU). On the L = N = 12 ring, K = 1,
+    // with the partials summed by separate finish launches: backward 41.8-41.9 ms a run at 2 and
+    // 1, 41.3 at 2 and 2, 42.6-42.7 with 3 or 4 blocks per CU for the apply.
     static const int apply_per_cu = [] {
         const char* v = std::getenv("TE_ADJ_APPLY2_BLOCKS_PER_CU");
         return v ? std::max(0, std::atoi(v)) : 2;
     }();
     static const int project_per_cu = [] {
         const char* v = std::getenv("TE_ADJ_PROJECT2_BLOCKS_PER_CU");
-        return v ? std::max(1, std::atoi(v)) : 1;
+        return v ? std::max(1, std::atoi(v)) : 2;
     }();
     const int cap = 32 * a->rs.grid;                  // each vector's partials region
     auto shape = [&](int per_cu, int64_t items) {
