@@ -49,7 +49,8 @@ __device__ __forceinline__ bool last_block_arrive(unsigned int* ticket) {
     return last;
 }
 
-// Sums partials[0..n) (stride `stride` doubles2 between blocks) in the last block.
+// Sums partials[0..n) (stride `stride` doubles2 between blocks) in the calling block, in a fixed
+// order (thread t takes t, t + NT, ..., then the block sum): the same result in every block.
 template <int NT>
 __device__ __forceinline__ double2 reduce_partials(const double2* partials, int n, int stride, double2* sh) {
     double2 acc = make_double2(0.0, 0.0);

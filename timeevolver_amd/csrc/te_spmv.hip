@@ -166,7 +166,7 @@ __device__ __forceinline__ double project_rows(const LanczosArgs& a, int64_t lo,
     return nrm;
 }
 
-// The last block's finish of a projection from beta^2 = tot.x.
+// The finish of a projection from beta^2 = tot.x.
 __device__ __forceinline__ void project_finish(const LanczosArgs& a, double2 tot) {
     KrylovScalars* ks = a.ks;
     const int j = a.j;
