@@ -141,6 +141,13 @@ def main():
              4: "te::lanczos_apply_sell_kernel"}[fmt]
     kernel_ms = {k: round(kt_all[k][0], 3) for k in kt_all}
     kernel_launches = {k: kt_all[k][1] for k in kt_all}
+    # whole-run figure: algorithmic bytes of every apply, projection (48 D; 32 D for the last
+    # of a step, which stores nothing) and combine pass (the basis columns read, currentVec and
+    # the sampled state written, the observables' diagonals) over the wall time
+    n_steps_run = r["n_steps"]
+    b_run = (kernel_launches.get("apply", 0) * b_apply
+             + kernel_launches.get("project", 0) * 48 * D - n_steps_run * 16 * D
+             + kernel_launches.get("combine", 0) * (16 * args.m * D + 32 * D + 8 * L * D))
 
     line = {
         "metric": METRIC,
@@ -174,6 +181,10 @@ def main():
                     "matrix_bytes": b_matrix},
         "kernel_ms_per_run": kernel_ms,
         "kernel_launches_per_run": kernel_launches,
+        "run_roofline": {"bytes": int(b_run), "achieved_GBps": round(b_run / time_to_T / 1e9, 1),
+                         "frac": round(b_run / time_to_T / 1e9 / PEAK_HBM_GBS, 4),
+                         "basis": "algorithmic bytes of a run's applies, projections and combine passes "
+                                  "over time-to-T"},
         "roofline": {"kernel": kname + " (fused SpMV + axpy + dot)", "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": measured_traffic(kname, D, b_apply),
