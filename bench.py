@@ -259,6 +259,9 @@ def bench_batched(np, capi, ctx, args):
                         "sampling step %g, observable n_0" % (len(dims), sum(dims), T, m, tol, step),
             "value": round(steps_batch / t_batch, 1), "plain_value": round(steps_plain / t_plain, 1),
             "speedup_vs_plain": round((steps_batch / t_batch) / (steps_plain / t_plain), 2),
+            "spmv_per_s": round(sum(r["n_steps"] * r["krylovDim"] for r in rb) / t_batch, 1),
+            "spmv_per_s_note": "Krylov steps x Krylov dimension per instance (at most: a lucky breakdown "
+                               "ends a step's Lanczos early)",
             "batched_s": round(t_batch, 4), "plain_s": round(t_plain, 4), "krylov_steps": steps_batch,
             "rounds": rounds, "same_steps_and_status_as_plain": same, "max_obs_diff_vs_plain": dobs,
             "build_s": round(build_s, 2)}
