@@ -123,6 +123,21 @@ def test_adjoint_vs_finite_differences_L8(gpu):
     assert rel <= 1e-6
 
 
+@pytest.mark.parametrize("K", [0, 1])
+def test_adjoint_runs_are_bitwise_repeatable(gpu, K):
+    """Every reduction of the forward and backward passes (the Lanczos scalars summed by their
+    finish launches, the node passes' label sums, the observables) runs in a fixed order: two
+    runs of one adjoint give the same loss and gradient bit for bit."""
+    L = N = 8
+    theta = _theta(L, 17)
+    prob = ring_problem(gpu, L, N, 0.5, 1.5, tol=1e-8, m=40, checkpoint_interval=K, theta0=theta)
+    psi0 = random_state(prob.basis.D, 6)
+    l1, g1, _, _ = prob.loss_and_grad(theta, psi0)
+    l2, g2, _, _ = prob.loss_and_grad(theta, psi0)
+    assert np.float64(l1).view(np.uint64) == np.float64(l2).view(np.uint64)
+    assert np.array_equal(np.asarray(g1).view(np.uint64), np.asarray(g2).view(np.uint64))
+
+
 def test_torch_autograd_wrapper(gpu):
     import torch
     L = N = 4

@@ -501,6 +501,26 @@ def test_ring_small_config_vs_oracle(gpu, oracle_lib):
     _check_pair(ro, rg, 8)
 
 
+def test_evolver_runs_are_bitwise_repeatable(gpu):
+    """The Lanczos scalars come from block partials summed in block order by separate finish
+    launches, and the observables likewise: two runs of one evolver, and a run of a second evolver
+    on the same operator, give the same state, samples, error and step count bit for bit (ring
+    L = N = 10, D = 92,378, sliced ELL over many blocks)."""
+    L = N = 10
+    gb = capi.Basis.basis(gpu, N, L)
+    H = capi.operator_build(gpu, gb, as_tuples(_ring_terms(L)))
+    assert H.storage()[0] == 4
+    obs = [capi.operator_build(gpu, gb, as_tuples([createNumberOperator(k, 1)])) for k in range(3)]
+    psi = random_state(gb.D, 8)
+    ev = capi.Evolver(gpu, H, psi, 2.0, 0.5, obs, tol=1e-8)
+    runs = [ev.run(), ev.run(), capi.Evolver(gpu, H, psi, 2.0, 0.5, obs, tol=1e-8).run()]
+    for r in runs[1:]:
+        assert r["n_steps"] == runs[0]["n_steps"] and r["statusCode"] == runs[0]["statusCode"]
+        assert np.float64(r["err"]).view(np.uint64) == np.float64(runs[0]["err"]).view(np.uint64)
+        assert np.array_equal(r["state"].view(np.uint64), runs[0]["state"].view(np.uint64))
+        assert np.array_equal(r["observables"].view(np.uint64), runs[0]["observables"].view(np.uint64))
+
+
 def test_ring_medium_config_vs_oracle(gpu, oracle_lib):
     """Ring Bose-Hubbard L = N = 12 (D = 1,352,078), seeded random complex initial state, T = 10,
     m = 40, tol = 1e-8, step 0.1 (101 samples of the 12 n_k); the basis and CSR it runs on are
