@@ -172,7 +172,8 @@ int te_evolver_step_trace(te_evolver* ev, double* t_steps, size_t cap, size_t* n
  * HIP events recorded around the first 4096 launches of each kind on the context's stream,
  * for the kinds whose bit is set in kinds (bit k: kind k; 0 turns timing off; events cost a
  * few µs each). kind: 0 Lanczos apply (fused SpMV, the dominant kernel), 1 Lanczos project,
- * 2 step controller, 3 combine (state update, samples, observables); 4..6 the controller's
+ * 2 step controller, 3 combine (state update, samples, observables); the events of kinds 0, 1
+ * and 3 bracket the kernel and the one-block launch that sums its partials; 4..6 the controller's
  * phases (eigen-decomposition, step-size search, sample coefficients) from the device clock,
  * with *launches the number of Krylov steps. */
 int te_evolver_set_profiling(te_evolver* ev, int kinds);
