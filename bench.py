@@ -20,8 +20,9 @@ outside the timed region).
 `csr_equivalent_GBps` prices the same launches at the reference's CSR bytes
 (20 nnz + 4 (D+1) + 48 D); `spmv_unit` compares the launch time with the time the 60 %-of-peak
 bar allows for one CSR SpMV (20 nnz + 4 (D+1) + 32 D bytes at 4.8 TB/s). `cpu_baseline` times
-the oracle's CPU restatement of the reference evolver on this host, on a bounded sample (the
-first Krylov step of the same workload) and extrapolates to the full time-to-T.
+the oracle's CPU restatement of the reference evolver on this host, its SpMV and BLAS on MKL as
+the reference's build runs them (OpenMP loops when libmkl_rt is absent), on a bounded sample (the
+first two Krylov steps of the same workload) and extrapolates to the full time-to-T.
 """
 import argparse
 import json
@@ -53,7 +54,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=1, help="Krylov steps the CPU baseline runs")
+    ap.add_argument("--cpu-steps", type=int, default=2, help="Krylov steps the CPU baseline runs")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain measurement")
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--batched-only", action="store_true", help="only the batched measurement")
@@ -385,13 +386,56 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "repeats": 3}), flush=True)
 
 
+def host_description():
+    """CPU model, logical CPUs (all, and usable by this process) and RAM of this host."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    ram_gib = None
+    try:
+        with open("/proc/meminfo") as f:
+            for line in f:
+                if line.startswith("MemTotal:"):
+                    ram_gib = round(int(line.split()[1]) / 1048576.0, 1)
+                    break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"cpu_model": model, "nproc": os.cpu_count(), "usable_cpus": usable, "ram_GiB": ram_gib}
+
+
 def cpu_baseline(np, capi, H, obs, D, args, gpu_steps):
-    """The oracle (CPU restatement of core/krylovTimeEvolver.cpp) on this host's cores, on the
-    first `cpu_steps` Krylov steps of the same workload, extrapolated to all GPU steps."""
+    """The reference's CPU path on this host's cores: the oracle's restatement of
+    krylovTimeEvolver::timeEvolve (core/krylovTimeEvolver.cpp:261-512) with its products and
+    level-1/2 kernels on MKL when libmkl_rt is present, as the reference's MKL build runs them
+    (mkl_sparse_z_mv on an ILP64 handle from create_coo -> convert_csr -> order -> hints ->
+    optimize, core/matrixDataTypes.cpp:169-174, :196-224; cblas axpy/dotc/nrm2/scal/copy/gemv),
+    else on the oracle's OpenMP loops. Threads: OMP_NUM_THREADS when set (the host's share of
+    this machine), else every CPU this process may use; MKL gets the same count. Timed: the
+    first `cpu_steps` Krylov steps of the same workload with their samples (the t = 0 sample
+    included), timeEvolve's loop only (the handles' set-up is the constructor's, timed apart);
+    extrapolated to the GPU run's Krylov steps by seconds per step."""
     import oracle
     from timeevolver_amd.workloads import fock_state
 
-    log("cpu baseline: copying CSR to host")
+    host = host_description()
+    threads = int(os.environ.get("OMP_NUM_THREADS") or 0) or host["usable_cpus"]
+    oracle.set_threads(threads)
+    rc = oracle.mkl_enable(threads=threads)
+    if rc == 0:
+        spmv = "mkl"
+        backend = "MKL mkl_sparse_z_mv + cblas (%s)" % oracle.mkl_version()
+    else:
+        spmv = "openmp-loop"
+        backend = "oracle OpenMP loops (libmkl_rt %s)" % {1: "not found", 2: "lacks an entry point",
+                                                          3: "refused the ILP64 interface"}.get(rc, "error %d" % rc)
+    log("cpu baseline: %s, %d threads; copying CSR to host" % (backend, threads))
     ptr, col, val = H.to_csr()
     diags = []
     for o in obs:
@@ -399,19 +443,24 @@ def cpu_baseline(np, capi, H, obs, D, args, gpu_steps):
         d = np.zeros(D)
         d[c] = v.real        # diagonal observables: one entry per row
         diags.append(d)
-    threads = oracle.get_threads()
-    log("cpu baseline: %d Krylov step(s) on %d threads" % (args.cpu_steps, threads))
-    t = time.perf_counter()
-    r = oracle.evolve_csr(ptr, col, val, fock_state(D), args.T, args.sampling_step, diags, tol=args.tol, m=args.m,
-                          max_steps=args.cpu_steps)
-    dt = time.perf_counter() - t
+    try:
+        r = oracle.evolve_csr(ptr, col, val, fock_state(D), args.T, args.sampling_step, diags, tol=args.tol,
+                              m=args.m, max_steps=args.cpu_steps)
+    finally:
+        oracle.mkl_disable()
+    dt = r["run_s"]
     per_step = dt / max(r["n_steps"], 1)
-    log("cpu baseline: %.2f s for %d step(s)" % (dt, r["n_steps"]))
-    return {"value": round(per_step * gpu_steps, 2), "unit": "s", "cores": threads, "kind": "port",
-            "sample": "first %d Krylov step(s) (m=%d Lanczos iterations + t=0 sample of %d observables) of the same "
-                      "workload took %.2f s on %d host threads; time-to-T extrapolated x %d Krylov steps"
-                      % (r["n_steps"], args.m, len(diags), dt, threads, gpu_steps),
-            "seconds_per_krylov_step": round(per_step, 3)}
+    log("cpu baseline: %.2f s for %d Krylov step(s), %.2f s set-up" % (dt, r["n_steps"], r["setup_s"]))
+    out = {"value": round(per_step * gpu_steps, 2), "unit": "s", "cores": threads, "kind": "port",
+           "spmv": spmv, "backend": backend,
+           "sample": "first %d Krylov step(s) of the same workload (m=%d Lanczos iterations each, the samples "
+                     "they cover of %d observables, t=0 sample included): %.2f s of timeEvolve on %d threads; "
+                     "value = seconds per Krylov step x %d Krylov steps of the GPU run (extrapolated)"
+                     % (r["n_steps"], args.m, len(diags), dt, threads, gpu_steps),
+           "extrapolated": True, "seconds_per_krylov_step": round(per_step, 3), "krylov_steps_timed": r["n_steps"],
+           "samples_timed": r["numSamples"], "setup_s": round(r["setup_s"], 2)}
+    out.update(host)
+    return out
 
 
 if __name__ == "__main__":

@@ -73,6 +73,9 @@ def lib():
             "or_evolve_csr": (_i, [_l, _lp, _ip, _dp, _dp, _d, _d, _d, _d, _i, _i, _l, _i, _dp, _dp, _dp, _dp]),
             "or_evolve": (_i, [_l, _l, _dp, _lp, _lp, _dp, _d, _d, _d, _d, _i, _i, _l, _l, _i, _ip, _lp,
                                _dp, _lp, _lp, _dp, _dp, _dp, _dp, _dp, _l]),
+            "or_mkl_enable": (_i, [ctypes.c_char_p, _i]),
+            "or_mkl_version": (None, [ctypes.c_char_p, _i]),
+            "or_mkl_disable": (None, []),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -84,6 +87,24 @@ def lib():
 
 def _ptr(a, t):
     return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+def mkl_enable(path=None, threads=0):
+    """Routes the evolvers' products and level-1/2 kernels through MKL (libmkl_rt opened with
+    dlopen, ILP64, GNU OpenMP threads), as the reference's MKL build runs them. Returns 0 on
+    success, else 1 (no library), 2 (missing entry point), 3 (ILP64 refused); the loop kernels
+    stay in use then. On until mkl_disable()."""
+    return lib().or_mkl_enable(path.encode() if path else None, int(threads))
+
+
+def mkl_disable():
+    lib().or_mkl_disable()
+
+
+def mkl_version():
+    buf = ctypes.create_string_buffer(256)
+    lib().or_mkl_version(buf, 256)
+    return buf.value.decode(errors="replace")
 
 
 def set_threads(n):
@@ -321,7 +342,7 @@ def evolve_csr(ptr, col, vals, psi0, t, samplingStep, diags=(), expFactor=1.0, t
     n_samples = int(np.floor(t / samplingStep)) + 1
     obs_out = np.zeros(max(len(diags), 1) * n_samples)
     state = np.zeros(n, dtype=np.complex128) if want_state else None
-    result = np.zeros(8)
+    result = np.zeros(10)
     rc = lib().or_evolve_csr(n, _ptr(ptr, ctypes.c_long), _ptr(col, ctypes.c_int), _ptr(vals, ctypes.c_double),
                              _ptr(psi.view(np.float64), ctypes.c_double), t, samplingStep, expFactor, tol, m,
                              int(fastIntegration), max_steps, len(diags),
@@ -331,5 +352,5 @@ def evolve_csr(ptr, col, vals, psi0, t, samplingStep, diags=(), expFactor=1.0, t
     if rc != 0:
         raise RuntimeError("oracle evolution hit a fatal condition")
     return {"err": result[0], "evolvedTime": result[1], "numSamples": int(result[2]), "n_steps": int(result[3]),
-            "statusCode": int(result[6]), "state": state,
+            "statusCode": int(result[6]), "state": state, "run_s": result[8], "setup_s": result[9],
             "observables": obs_out[:len(diags) * n_samples].reshape(len(diags), n_samples)}

@@ -31,11 +31,95 @@
 #include <map>
 #include <unordered_map>
 #include <vector>
+#include <dlfcn.h>
 #include <omp.h>
 
 using cd = std::complex<double>;
 
 namespace {
+
+// ----------------------------------------------------------------------------------
+// MKL backend (CPU baseline only). The reference is built against MKL with 64-bit
+// integers (CMakeLists.txt:52 Intel10_64ilp; core/mathHeader.h:6-7 MKL_INT = size_t):
+// its spMV is mkl_sparse_z_mv (core/matrixDataTypes.cpp:169-174) on a handle made by
+// create_coo -> convert_csr -> order -> mv/memory hints -> optimize (:196-224), and the
+// Lanczos and state updates are cblas calls (core/krylovTimeEvolver.cpp:382, :421,
+// :523-571; core/krylovObservables.cpp:187-188). No MKL header is installed here, so the
+// runtime library (libmkl_rt) is opened with dlopen and the documented C interfaces are
+// declared below; the ILP64 interface and the GNU OpenMP threading layer are selected
+// before any other call. Enabled only by or_mkl_enable; every evolver entry then routes
+// its products and level-1/2 kernels through MKL.
+// ----------------------------------------------------------------------------------
+namespace mkl {
+using i64 = long long;
+struct Z { double re, im; };
+struct Descr { int type, mode, diag; };
+enum { INDEX_BASE_ZERO = 0, OP_NON_TRANSPOSE = 10, MATRIX_GENERAL = 20, FILL_FULL = 42, DIAG_NON_UNIT = 50,
+       MEMORY_AGGRESSIVE = 81, INTERFACE_ILP64 = 1, THREADING_GNU = 3, COL_MAJOR = 102, NO_TRANS = 111 };
+struct Api {
+    void* so = nullptr;
+    int (*set_interface)(int) = nullptr;
+    int (*set_threading)(int) = nullptr;
+    void (*set_threads)(int) = nullptr;
+    void (*version)(char*, int) = nullptr;
+    int (*create_coo)(void**, int, i64, i64, i64, i64*, i64*, Z*) = nullptr;
+    int (*convert_csr)(void*, int, void**) = nullptr;
+    int (*order)(void*) = nullptr;
+    int (*mv_hint)(void*, int, Descr, i64) = nullptr;
+    int (*memory_hint)(void*, int) = nullptr;
+    int (*optimize)(void*) = nullptr;
+    int (*mv)(int, Z, void*, Descr, const Z*, Z, Z*) = nullptr;
+    int (*destroy)(void*) = nullptr;
+    void (*zaxpy)(i64, const void*, const void*, i64, void*, i64) = nullptr;
+    void (*zdotc_sub)(i64, const void*, i64, const void*, i64, void*) = nullptr;
+    double (*dznrm2)(i64, const void*, i64) = nullptr;
+    void (*zscal)(i64, const void*, void*, i64) = nullptr;
+    void (*zcopy)(i64, const void*, i64, void*, i64) = nullptr;
+    void (*zgemv)(int, int, i64, i64, const void*, const void*, i64, const void*, i64, const void*, void*,
+                  i64) = nullptr;
+    char ver[256] = {0};
+};
+
+Api g_api;
+const Api* g_mkl = nullptr;              // non-null: MKL routes are on
+
+template <class F>
+bool sym(F& f, const char* name) {
+    f = reinterpret_cast<F>(dlsym(g_api.so, name));
+    return f != nullptr;
+}
+
+// Opens libmkl_rt (path given, or the loader's search path) and resolves every entry.
+int enable(const char* path, int threads) {
+    if (g_api.ver[0]) {                   // loaded before: switch back on
+        g_api.set_threads(threads > 0 ? threads : omp_get_max_threads());
+        g_mkl = &g_api;
+        return 0;
+    }
+    const char* cands[] = {path, "libmkl_rt.so.1", "libmkl_rt.so", "/opt/conda/lib/libmkl_rt.so.1"};
+    for (const char* c : cands)
+        if (c && c[0] && (g_api.so = dlopen(c, RTLD_NOW | RTLD_GLOBAL))) break;
+    if (!g_api.so) return 1;
+    bool ok = sym(g_api.set_interface, "MKL_Set_Interface_Layer") && sym(g_api.set_threading, "MKL_Set_Threading_Layer") &&
+              sym(g_api.set_threads, "MKL_Set_Num_Threads") && sym(g_api.version, "MKL_Get_Version_String") &&
+              sym(g_api.create_coo, "mkl_sparse_z_create_coo") && sym(g_api.convert_csr, "mkl_sparse_convert_csr") &&
+              sym(g_api.order, "mkl_sparse_order") && sym(g_api.mv_hint, "mkl_sparse_set_mv_hint") &&
+              sym(g_api.memory_hint, "mkl_sparse_set_memory_hint") && sym(g_api.optimize, "mkl_sparse_optimize") &&
+              sym(g_api.mv, "mkl_sparse_z_mv") && sym(g_api.destroy, "mkl_sparse_destroy") &&
+              sym(g_api.zaxpy, "cblas_zaxpy") && sym(g_api.zdotc_sub, "cblas_zdotc_sub") &&
+              sym(g_api.dznrm2, "cblas_dznrm2") && sym(g_api.zscal, "cblas_zscal") && sym(g_api.zcopy, "cblas_zcopy") &&
+              sym(g_api.zgemv, "cblas_zgemv");
+    if (!ok) return 2;
+    if (g_api.set_interface(INTERFACE_ILP64) != INTERFACE_ILP64) return 3;
+    g_api.set_threading(THREADING_GNU);
+    g_api.set_threads(threads > 0 ? threads : omp_get_max_threads());
+    g_api.version(g_api.ver, (int)sizeof(g_api.ver) - 1);
+    g_mkl = &g_api;
+    return 0;
+}
+
+Descr general() { return Descr{MATRIX_GENERAL, FILL_FULL, DIAG_NON_UNIT}; }
+}  // namespace mkl
 
 // ----------------------------------------------------------------------------------
 // Basis: helper/Basis.cpp:37-120 (basis), :131-182 (tensorBasis), :13-17 (hash table)
@@ -235,7 +319,53 @@ struct Csr {
     long n = 0, ncols = 0;
     std::vector<long> ptr, col;
     std::vector<cd> val;
+    void* mkl = nullptr;        // MKL handle (MKL backend only)
+    Csr() = default;
+    Csr(const Csr&) = delete;
+    Csr& operator=(const Csr&) = delete;
+    Csr(Csr&& o) noexcept : n(o.n), ncols(o.ncols), ptr(std::move(o.ptr)), col(std::move(o.col)),
+                            val(std::move(o.val)), mkl(o.mkl) { o.mkl = nullptr; }
+    Csr& operator=(Csr&& o) noexcept {
+        std::swap(n, o.n); std::swap(ncols, o.ncols); ptr.swap(o.ptr); col.swap(o.col); val.swap(o.val);
+        std::swap(mkl, o.mkl);
+        return *this;
+    }
+    ~Csr() { if (mkl && mkl::g_api.destroy) mkl::g_api.destroy(mkl); }
 };
+
+// The reference's smatrix::initialize (core/matrixDataTypes.cpp:196-224) on the MKL backend:
+// COO handle with 64-bit indices, converted to CSR, ordered, mv hint of 20000 calls,
+// aggressive memory hint, optimize. The COO arrays only need to live until the conversion.
+static int mkl_attach(Csr& A) {
+    using namespace mkl;
+    if (!g_mkl || A.mkl) return 0;
+    const long nnz = A.ptr[A.n];
+    if (nnz == 0) return 0;
+    std::vector<i64> row(nnz), col(nnz);
+#pragma omp parallel for schedule(static)
+    for (long r = 0; r < A.n; r++)
+        for (long k = A.ptr[r]; k < A.ptr[r + 1]; k++) {
+            row[k] = r;
+            col[k] = A.col[k];
+        }
+    void* coo = nullptr;
+    if (g_mkl->create_coo(&coo, INDEX_BASE_ZERO, A.n, A.ncols, nnz, row.data(), col.data(),
+                          reinterpret_cast<Z*>(A.val.data())) != 0)
+        return 1;
+    void* csr = nullptr;
+    int rc = g_mkl->convert_csr(coo, OP_NON_TRANSPOSE, &csr);
+    g_mkl->destroy(coo);
+    if (rc != 0) return 2;
+    g_mkl->order(csr);
+    g_mkl->mv_hint(csr, OP_NON_TRANSPOSE, general(), 20000);
+    g_mkl->memory_hint(csr, MEMORY_AGGRESSIVE);
+    if (g_mkl->optimize(csr) != 0) {
+        g_mkl->destroy(csr);
+        return 3;
+    }
+    A.mkl = csr;
+    return 0;
+}
 
 static Csr coo_to_csr(long n, long ncols, long nnz, const double* v, const long* col, const long* row) {
     Csr A;
@@ -259,6 +389,12 @@ static Csr coo_to_csr(long n, long ncols, long nnz, const double* v, const long*
 
 // y = alpha * A x (matrixDataTypes.cpp:172-173: mkl_sparse_z_mv with beta = 0).
 static void spmv(const Csr& A, cd alpha, const cd* x, cd* y) {
+    if (A.mkl) {
+        const mkl::Z a{alpha.real(), alpha.imag()}, zero{0.0, 0.0};
+        mkl::g_mkl->mv(mkl::OP_NON_TRANSPOSE, a, A.mkl, mkl::general(), reinterpret_cast<const mkl::Z*>(x), zero,
+                       reinterpret_cast<mkl::Z*>(y));
+        return;
+    }
 #pragma omp parallel for schedule(static)
     for (long r = 0; r < A.n; r++) {
         cd s(0.0, 0.0);
@@ -277,6 +413,11 @@ static double norm1_coo(long ncols, long nnz, const double* v, const long* col) 
 // ---- dense level-1 kernels with a fixed reduction order (per-thread blocks, combined
 // in thread order), so results are deterministic for a given thread count ----
 static cd zdotc(long n, const cd* x, const cd* y) {         // sum conj(x) y
+    if (mkl::g_mkl) {
+        cd r;
+        mkl::g_mkl->zdotc_sub(n, x, 1, y, 1, &r);
+        return r;
+    }
     int nt = omp_get_max_threads();
     std::vector<cd> part(nt, cd(0.0, 0.0));
 #pragma omp parallel num_threads(nt)
@@ -292,6 +433,7 @@ static cd zdotc(long n, const cd* x, const cd* y) {         // sum conj(x) y
     return s;
 }
 static double dznrm2(long n, const cd* x) {
+    if (mkl::g_mkl) return mkl::g_mkl->dznrm2(n, x, 1);
     int nt = omp_get_max_threads();
     std::vector<double> part(nt, 0.0);
 #pragma omp parallel num_threads(nt)
@@ -307,11 +449,20 @@ static double dznrm2(long n, const cd* x) {
     return std::sqrt(s);
 }
 static void zaxpy(long n, cd a, const cd* x, cd* y) {
+    if (mkl::g_mkl) {
+        mkl::g_mkl->zaxpy(n, &a, x, 1, y, 1);
+        return;
+    }
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < n; i++) y[i] += a * x[i];
 }
 // y = V w, V column-major n x m (the reference's cblas_zgemv NoTrans, :382, :421).
 static void gemv_tall(long n, int m, const cd* V, const cd* w, cd* y) {
+    if (mkl::g_mkl) {
+        const cd one(1.0, 0.0), zero(0.0, 0.0);
+        mkl::g_mkl->zgemv(mkl::COL_MAJOR, mkl::NO_TRANS, n, m, &one, V, n, w, 1, &zero, y, 1);
+        return;
+    }
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < n; i++) {
         cd s(0.0, 0.0);
@@ -690,8 +841,14 @@ struct Evolver {
                 b[j] = normy;
                 double inv = 1.0 / normy;
                 cd* vn = &V[(size_t)(j + 1) * Hsize];
+                if (mkl::g_mkl) {                   // :563-564: zscal, then zcopy
+                    const cd s(inv, 0.0);
+                    mkl::g_mkl->zscal(Hsize, &s, tmp.data(), 1);
+                    mkl::g_mkl->zcopy(Hsize, tmp.data(), 1, vn, 1);
+                } else {
 #pragma omp parallel for schedule(static)
-                for (long i = 0; i < Hsize; i++) vn[i] = tmp[i] * inv;
+                    for (long i = 0; i < Hsize; i++) vn[i] = tmp[i] * inv;
+                }
             } else {
                 *hRet = normy;
             }
@@ -1044,6 +1201,9 @@ int or_evolve(long n, long nnz, const double* hv, const long* hc, const long* hr
         obs[i].values.assign(ev.n_samples, 0.0);
     }
     ev.obs = &obs;
+    if (mkl_attach(H) != 0) return -2;
+    for (auto& o : obs)
+        if (o.kind == 0 && mkl_attach(o.A) != 0) return -2;
     Result R;
     int rc = ev.run(R);
     if (rc != 0) return rc;
@@ -1061,9 +1221,13 @@ int or_evolve(long n, long nnz, const double* hv, const long* hc, const long* hr
 // Time evolution with H given as CSR (int64 row pointers, int32 columns) and diagonal
 // observables as dense diagonals (n_obs x n): the CPU baseline on a large workload, where a
 // COO sort would dominate. max_steps bounds the number of Krylov steps (0 = all).
+// result: [err, evolvedTime, numSamples, n_steps, dim, krylovDim, statusCode, n_samples,
+// seconds of timeEvolve (the run loop: samples, Lanczos, step control, state updates),
+// seconds of the set-up before it (copies, norm1, and with MKL the handles' initialize)].
 int or_evolve_csr(long n, const long* ptr, const int* col, const double* vals, const double* psi0, double t,
                   double samplingStep, double expFactor, double tol, int m, int fastIntegration, long max_steps,
                   int n_obs, const double* diags, double* obs_out, double* state_out, double* result) {
+    const double t_setup = omp_get_wtime();
     Csr H;
     H.n = H.ncols = n;
     long nnz = ptr[n];
@@ -1096,8 +1260,13 @@ int or_evolve_csr(long n, const long* ptr, const int* col, const double* vals, c
         obs[i].values.assign(ev.n_samples, 0.0);
     }
     ev.obs = &obs;
+    if (mkl_attach(H) != 0) return -2;
+    for (auto& o : obs)
+        if (mkl_attach(o.A) != 0) return -2;
     Result R;
+    const double t_run = omp_get_wtime();
     int rc = ev.run(R);
+    const double t_end = omp_get_wtime();
     if (rc != 0) return rc;
     for (int i = 0; i < n_obs; i++)
         for (long s = 0; s < ev.n_samples; s++) obs_out[i * ev.n_samples + s] = obs[i].values[s];
@@ -1105,7 +1274,21 @@ int or_evolve_csr(long n, const long* ptr, const int* col, const double* vals, c
     result[0] = R.err; result[1] = R.evolvedTime; result[2] = (double)R.numSamples;
     result[3] = (double)R.n_steps; result[4] = (double)R.dim; result[5] = (double)R.krylovDim;
     result[6] = (double)R.statusCode; result[7] = (double)ev.n_samples;
+    result[8] = t_end - t_run; result[9] = t_run - t_setup;
     return 0;
+}
+
+// MKL backend switch (CPU baseline): 0 when libmkl_rt was opened and every entry resolved
+// (ILP64 interface, GNU OpenMP threading, `threads` threads; 0 = OpenMP's count), else the
+// loop kernels stay in use and the code says which step failed (1 no library, 2 missing
+// entry, 3 ILP64 refused). or_mkl_disable returns to the loop kernels (the library stays
+// loaded; a later or_mkl_enable switches it back on).
+int or_mkl_enable(const char* path, int threads) { return mkl::enable(path, threads); }
+void or_mkl_disable() { mkl::g_mkl = nullptr; }
+// The MKL version string ("" while the backend is off).
+void or_mkl_version(char* buf, int len) {
+    if (len <= 0) return;
+    std::snprintf(buf, (size_t)len, "%s", mkl::g_mkl ? mkl::g_api.ver : "");
 }
 
 }  // extern "C"

@@ -3,7 +3,7 @@ one batch, each checked against the oracle's restatement of the reference evolve
 the plain single-instance GPU path.
 
 Bars (as for the plain path): ||psi_batch - psi_oracle|| <= err_batch + err_oracle + 1e-10,
-observables within 2 ||O|| (err_batch + err_oracle) + 1e-10, identical sample counts and status
+observables within the requested tol, identical sample counts and status
 codes; graph replay and kernel-by-kernel launches bitwise identical; repeated runs bitwise
 identical.
 """
@@ -44,7 +44,7 @@ def _oracle_run(oracle, N, L, terms, psi, T, step, obs_modes, tol, m):
     return oracle.evolve(ob.D, Hc, psi, T, step, oobs, tol=tol, m=m)
 
 
-def _check(rb, ro, obs_norm):
+def _check(rb, ro, tol):
     bound = rb["err"] + ro["err"]
     dpsi = np.linalg.norm(rb["state"] - ro["state"])
     dobs = np.abs(rb["observables"] - ro["observables"]).max()
@@ -52,7 +52,7 @@ def _check(rb, ro, obs_norm):
           (rb["dim"], rb["n_steps"], ro["n_steps"], rb["err"], ro["err"], dpsi, dobs, rb["statusCode"],
            ro["statusCode"]))
     assert dpsi <= bound + 1e-10
-    assert dobs <= 2 * obs_norm * bound + 1e-10
+    assert dobs <= tol
     assert rb["numSamples"] == ro["numSamples"] == ro["n_samples"]
     assert rb["statusCode"] == ro["statusCode"]
     assert abs(rb["evolvedTime"] - ro["evolvedTime"]) <= 1e-12
@@ -69,7 +69,7 @@ def test_batch_uneven_vs_oracle(gpu, oracle_lib):
     assert be.rounds() == max(r["n_steps"] for r in rb)
     for (L, N, *_), (b, H, obs, terms, psi), r in zip(specs, inst, rb):
         ro = _oracle_run(oracle_lib, N, L, terms, psi, T, step, (0, 1), tol, m)
-        _check(r, ro, N)
+        _check(r, ro, tol)
 
 
 def test_batch_matches_plain_gpu(gpu):
@@ -83,7 +83,7 @@ def test_batch_matches_plain_gpu(gpu):
         rp = capi.Evolver(gpu, H, psi, T, step, obs, tol=tol, m=m).run()
         assert r["n_steps"] == rp["n_steps"] and r["statusCode"] == rp["statusCode"]
         assert np.linalg.norm(r["state"] - rp["state"]) <= r["err"] + rp["err"] + 1e-10
-        assert np.abs(r["observables"] - rp["observables"]).max() <= 2 * 8 * (r["err"] + rp["err"]) + 1e-10   # n_0 <= N <= 8
+        assert np.abs(r["observables"] - rp["observables"]).max() <= tol
 
 
 def test_batch_graph_and_repeat_bitwise(gpu):
@@ -116,7 +116,7 @@ def test_batch_edge_cases(gpu, oracle_lib):
                            tol=tol, m=m).run()
     for (L, N, *_), (b, H, obs, terms, psi), r in zip(specs, inst, rb):
         ro = _oracle_run(oracle_lib, N, L, terms, psi, T, step, (0,), tol, m)
-        _check(r, ro, max(N, 1))
+        _check(r, ro, tol)
     assert np.array_equal(rb[1]["state"], rb[3]["state"])
 
 
@@ -148,4 +148,4 @@ def test_batch_sell_and_csr_paths(gpu, monkeypatch):
         for r in (a, c):
             assert r["n_steps"] == rp["n_steps"] and r["statusCode"] == rp["statusCode"]
             assert np.linalg.norm(r["state"] - rp["state"]) <= r["err"] + rp["err"] + 1e-10
-            assert np.abs(r["observables"] - rp["observables"]).max() <= 2 * 7 * (r["err"] + rp["err"]) + 1e-10
+            assert np.abs(r["observables"] - rp["observables"]).max() <= tol

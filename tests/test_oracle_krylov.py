@@ -168,3 +168,41 @@ def test_evolve_csr_entry_matches_coo_entry(oracle_lib):
     assert r2["n_steps"] == r["n_steps"] and r2["statusCode"] == r["statusCode"]
     assert np.array_equal(r2["state"], r["state"])
     assert np.abs(r2["observables"] - r["observables"]).max() < 1e-13
+
+
+def test_mkl_backend_matches_loop_kernels(oracle_lib):
+    """The CPU baseline's MKL route (mkl_sparse_z_mv on the handle the reference's
+    smatrix::initialize builds, cblas level-1/2 calls) gives the loop kernels' evolution:
+    same steps and status, states and observables equal to rounding. Both entry points
+    (COO with sparse and vector observables; CSR with diagonal observables)."""
+    rc = oracle_lib.mkl_enable()
+    if rc != 0:
+        pytest.skip("libmkl_rt not available (code %d)" % rc)
+    oracle_lib.mkl_disable()
+    L, N = 6, 6
+    eps, J = ring_couplings(L, 11)
+    b = oracle_lib.Basis.basis(N, L)
+    H = oracle_lib.build_operator(b, as_tuples(ring_bose_hubbard_terms(L, eps, J, 0.5)))
+    obs = [("sparse", oracle_lib.build_operator(b, as_tuples([createNumberOperator(k, 1)]))) for k in range(L)]
+    obs.append(("vector", random_state(b.D, 4)))
+    psi = random_state(b.D, 3)
+    vals, cols, rows = H
+    ptr = np.cumsum(np.concatenate([[0], np.bincount(rows, minlength=b.D)])).astype(np.int64)
+    S = b.states()
+    diags = [S[:, k].astype(float) for k in range(L)]
+    runs = {}
+    for mode in ("loop", "mkl"):
+        if mode == "mkl":
+            assert oracle_lib.mkl_enable() == 0
+            assert "Math Kernel Library" in oracle_lib.mkl_version()
+        try:
+            runs[mode] = (oracle_lib.evolve(b.D, H, psi, 2.0, 0.1, obs, tol=1e-8),
+                          oracle_lib.evolve_csr(ptr, cols.astype(np.int32), vals, psi, 2.0, 0.1, diags, tol=1e-8,
+                                                want_state=True))
+        finally:
+            oracle_lib.mkl_disable()
+    for a, c in zip(runs["loop"], runs["mkl"]):
+        assert a["n_steps"] == c["n_steps"] and a["statusCode"] == c["statusCode"]
+        assert np.linalg.norm(a["state"] - c["state"]) <= 1e-12
+        assert np.abs(a["observables"] - c["observables"]).max() <= 1e-12
+        assert abs(a["err"] - c["err"]) <= 1e-12
