@@ -241,6 +241,16 @@ def bench_batched(np, capi, ctx, args):
     steps_batch = sum(r["n_steps"] for r in rb)
     rounds = be.rounds()
     log("batched: %.3f s, %d Krylov steps in %d rounds" % (t_batch, steps_batch, rounds))
+    # every launch kind timed with HIP events in one more run, outside the timed region (the
+    # rounds are then launched kernel by kernel, not replayed from the graph)
+    be.set_profiling(True)
+    ctx.synchronize()
+    t = time.perf_counter()
+    be.run(want_states=False)
+    ctx.synchronize()
+    t_prof = time.perf_counter() - t
+    kt = be.kernel_times()
+    be.set_profiling(False)
     be.close()
 
     evs = [capi.Evolver(ctx, H, p, T, step, o, tol=tol, m=m) for H, p, o in zip(Hs, psis, obs)]
@@ -266,7 +276,13 @@ def bench_batched(np, capi, ctx, args):
                                "ends a step's Lanczos early)",
             "batched_s": round(t_batch, 4), "plain_s": round(t_plain, 4), "krylov_steps": steps_batch,
             "rounds": rounds, "same_steps_and_status_as_plain": same, "max_obs_diff_vs_plain": dobs,
-            "build_s": round(build_s, 2)}
+            "build_s": round(build_s, 2),
+            "kernel_ms_per_run": {k: round(v[0], 3) for k, v in kt.items()},
+            "kernel_launches_per_run": {k: v[1] for k, v in kt.items()},
+            "profiled_run_s": round(t_prof, 4),
+            "kernel_note": "one more run with HIP events around every launch kind (apply and projection with "
+                           "their finish launches, controller, combine with observable sums and live count), "
+                           "launched kernel by kernel; outside the timed runs"}
 
 
 def bench_adjoint(np, capi, ctx, args):

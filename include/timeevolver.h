@@ -203,8 +203,18 @@ int te_batch_run(te_batch* b, double* obs_values, double* const* states, te_resu
  * one Krylov step of every live instance). */
 int te_batch_info(const te_batch* b, size_t* packed_rows, size_t* n_samples, long long* rounds);
 /* 1 (default): each round is replayed from a captured HIP graph; 0: launched kernel by kernel.
- * The results are identical. */
+ * The results are identical. A round that cannot be captured or instantiated is launched kernel
+ * by kernel (logged as a warning). */
 int te_batch_set_graph(te_batch* b, int enable);
+/* Kernel timing of te_batch_run, for measurement harnesses (no reference counterpart): HIP
+ * events recorded around the first 8192 launches of each kind on the context's stream, for the
+ * kinds whose bit is set in kinds (0 turns timing off). While timing is on, rounds are launched
+ * kernel by kernel instead of replayed from the graph (same results). kind: 0 Lanczos apply of
+ * every instance with the launch that sums its partials, 1 projection with its sums, 2 step
+ * controller (one block per instance), 3 combine (state update, samples, observable sums, live
+ * count). total_ms and launches cover the last run. */
+int te_batch_set_profiling(te_batch* b, int kinds);
+int te_batch_kernel_time(const te_batch* b, int kind, double* total_ms, size_t* launches);
 
 /* ---- adjoint gradient (no reference counterpart: the reference only evolves forward) ---- */
 typedef struct te_adjoint te_adjoint;

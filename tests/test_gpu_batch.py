@@ -93,10 +93,16 @@ def test_batch_graph_and_repeat_bitwise(gpu):
                            tol=1e-8, m=20)
     r1 = be.run()
     r2 = be.run()
+    be.set_profiling(True)              # timed kinds: launched kernel by kernel, same results
+    r4 = be.run()
+    kt = be.kernel_times()
+    be.set_profiling(False)
+    assert kt["apply"][1] == kt["project"][1] == 20 * be.rounds() and kt["control"][1] == be.rounds()
+    assert kt["combine"][1] == be.rounds() + 1 and all(v[0] > 0 for v in kt.values())
     be.set_graph(False)
     r3 = be.run()
-    for a, b, c in zip(r1, r2, r3):
-        for x in (b, c):
+    for a, b, c, d in zip(r1, r2, r3, r4):
+        for x in (b, c, d):
             assert np.array_equal(a["state"].view(np.uint64), x["state"].view(np.uint64))
             assert np.array_equal(a["observables"].view(np.uint64), x["observables"].view(np.uint64))
             assert a["n_steps"] == x["n_steps"] and a["err"] == x["err"]

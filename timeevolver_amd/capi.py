@@ -94,6 +94,8 @@ _SIGS = {
     "te_batch_run": (_c.c_int, [_p, _dp, _c.POINTER(_dp), _c.POINTER(te_result)]),
     "te_batch_info": (_c.c_int, [_p, _szp, _szp, _c.POINTER(_c.c_longlong)]),
     "te_batch_set_graph": (_c.c_int, [_p, _c.c_int]),
+    "te_batch_set_profiling": (_c.c_int, [_p, _c.c_int]),
+    "te_batch_kernel_time": (_c.c_int, [_p, _c.c_int, _dp, _szp]),
     "te_adjoint_create": (_c.c_int, [_p, _p, _c.c_int, _pp, _p, _c.POINTER(te_evolve_params), _c.c_int, _pp]),
     "te_adjoint_destroy": (None, [_p]),
     "te_adjoint_run": (_c.c_int, [_p, _dp, _dp, _dp, _c.POINTER(te_result), _c.POINTER(te_adjoint_info)]),
@@ -447,6 +449,28 @@ class BatchEvolver:
 
     def set_graph(self, on):
         self.ctx.check(self.ctx.L.te_batch_set_graph(self.h, int(on)), "te_batch_set_graph")
+
+    KINDS = ("apply", "project", "control", "combine")
+
+    def set_profiling(self, kinds=KINDS):
+        """Time the launches of the given kinds (names from KINDS) with HIP events in the next
+        runs (rounds are then launched kernel by kernel); False/None/() turns timing off, True
+        times every kind."""
+        if kinds is True:
+            kinds = self.KINDS
+        mask = 0
+        for k in (kinds or ()):
+            mask |= 1 << self.KINDS.index(k)
+        self.ctx.check(self.ctx.L.te_batch_set_profiling(self.h, mask), "te_batch_set_profiling")
+
+    def kernel_times(self):
+        """{kind: (total_ms, launches)} of the last run."""
+        out = {}
+        for k, name in enumerate(self.KINDS):
+            ms, n = _c.c_double(), _c.c_size_t()
+            self.ctx.check(self.ctx.L.te_batch_kernel_time(self.h, k, _c.byref(ms), _c.byref(n)), "kernel_time")
+            out[name] = (ms.value, n.value)
+        return out
 
     def rounds(self):
         r = _c.c_longlong()

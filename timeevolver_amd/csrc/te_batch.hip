@@ -455,6 +455,13 @@ struct te_batch {
     hipGraphExec_t exec = nullptr;
     long long rounds = 0;
     double round_ms = 0;
+    // optional timing by launch kind (HIP event pairs on the context stream); while it is on,
+    // rounds are launched kernel by kernel (no graph replay; the results are the same)
+    int prof_kinds = 0;
+    std::vector<hipEvent_t> prof_ev[4];
+    size_t prof_used[4] = {0, 0, 0, 0};
+    double prof_ms[4] = {0, 0, 0, 0};
+    size_t prof_n[4] = {0, 0, 0, 0};
 };
 
 namespace te {
@@ -462,6 +469,8 @@ namespace te {
 static void batch_free(te_batch* b) {
     if (!b) return;
     hipSetDevice(b->ctx->device);
+    for (auto& v : b->prof_ev)
+        for (hipEvent_t e : v) hipEventDestroy(e);
     if (b->exec) hipGraphExecDestroy(b->exec);
     if (b->graph) hipGraphDestroy(b->graph);
     void* ptrs[] = {b->inst, b->sunit,   b->vunit,  b->U,  b->psi0, b->sampled, b->wcoef, b->Y, b->tmp,
@@ -480,27 +489,104 @@ static int balloc(te_ctx* ctx, T** p, size_t count) {
     return TE_OK;
 }
 
-// Enqueues one round on the context's stream.
+constexpr size_t kBatchProfPairs = 8192;   // event pairs per launch kind
+
+// Launch kinds of the timing: 0 apply (with its finish launch), 1 projection (with its finish
+// launch), 2 step controller, 3 combine (with the observable sums and the live count).
+static void prof_begin(te_batch* b, int kind, hipStream_t s) {
+    if (((b->prof_kinds >> kind) & 1) && b->prof_used[kind] + 2 <= b->prof_ev[kind].size())
+        hipEventRecord(b->prof_ev[kind][b->prof_used[kind]], s);
+}
+static void prof_end(te_batch* b, int kind, hipStream_t s) {
+    if (((b->prof_kinds >> kind) & 1) && b->prof_used[kind] + 2 <= b->prof_ev[kind].size()) {
+        hipEventRecord(b->prof_ev[kind][b->prof_used[kind] + 1], s);
+        b->prof_used[kind] += 2;
+    }
+}
+// After the stream is synchronised: sums the recorded pairs.
+static void prof_collect(te_batch* b) {
+    for (int k = 0; k < 4; k++) {
+        b->prof_ms[k] = 0.0;
+        b->prof_n[k] = b->prof_used[k] / 2;
+        for (size_t i = 0; i + 1 < b->prof_used[k]; i += 2) {
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, b->prof_ev[k][i], b->prof_ev[k][i + 1]);
+            b->prof_ms[k] += ms;
+        }
+    }
+}
+
+// Enqueues one round on the context's stream (events only when timing is on, which also
+// keeps the round out of a graph).
 static void batch_round(te_batch* b, hipStream_t s) {
     const BatchDev& d = b->dev;
     for (int j = 0; j < b->m_max; j++) {
+        prof_begin(b, 0, s);
         if (b->sell) hipLaunchKernelGGL(batch_apply_sell_kernel, dim3(b->grid_s), dim3(kBlock), 0, s, d, j);
         else hipLaunchKernelGGL(batch_apply_kernel, dim3(b->grid_s), dim3(kBlock), 0, s, d, j);
         hipLaunchKernelGGL(batch_finish_kernel<false>, dim3(b->n_inst), dim3(kBlock), 0, s, d, j);
+        prof_end(b, 0, s);
+        prof_begin(b, 1, s);
         hipLaunchKernelGGL(batch_project_kernel, dim3(b->grid_v), dim3(kBlock), 0, s, d, j);
         hipLaunchKernelGGL(batch_finish_kernel<true>, dim3(b->n_inst), dim3(kBlock), 0, s, d, j);
+        prof_end(b, 1, s);
     }
+    prof_begin(b, 2, s);
     launch_batch_step_control(b->consts, b->st, b->ks, b->bc, b->n_inst, b->n_samples, s);
+    prof_end(b, 2, s);
 }
 
 static void batch_combine(te_batch* b, hipStream_t s) {
     const BatchDev& d = b->dev;
+    prof_begin(b, 3, s);
     if (b->nobs == 0) hipLaunchKernelGGL(batch_combine_kernel<0>, dim3(b->grid_v), dim3(kBlock), 0, s, d);
     else if (b->nobs == 1) hipLaunchKernelGGL(batch_combine_kernel<1>, dim3(b->grid_v), dim3(kBlock), 0, s, d);
     else if (b->nobs == 2) hipLaunchKernelGGL(batch_combine_kernel<2>, dim3(b->grid_v), dim3(kBlock), 0, s, d);
     else hipLaunchKernelGGL(batch_combine_kernel<4>, dim3(b->grid_v), dim3(kBlock), 0, s, d);
     hipLaunchKernelGGL(batch_obs_finish_kernel, dim3(b->n_inst), dim3(kBlock), 0, s, d);
     hipLaunchKernelGGL(batch_count_kernel, dim3(1), dim3(kBlock), 0, s, d);
+    prof_end(b, 3, s);
+}
+
+// Drops the round graph (a failed capture or instantiation included): ends a capture still
+// open on the stream, destroys what exists, clears the runtime's last error and leaves the
+// batch launching kernel by kernel.
+static void drop_graph(te_batch* b, hipStream_t s) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+        hipGraph_t g = nullptr;
+        hipStreamEndCapture(s, &g);
+        if (g) hipGraphDestroy(g);
+    }
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->use_graph = false;
+    (void)hipGetLastError();
+}
+
+// Captures one round (Lanczos iterations, controller, combine) as a graph of kernel nodes
+// and instantiates it. On any failure the batch falls back to direct launches (same results)
+// and the run goes on; the failure is logged.
+static void capture_round(te_batch* b, hipStream_t s) {
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        batch_round(b, s);
+        batch_combine(b, s);
+        e = hipGetLastError();
+        hipGraph_t g = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(s, &g);
+        if (e == hipSuccess) e = e2;
+        b->graph = g;
+        if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+        if (e == hipSuccess && !b->exec) e = hipErrorUnknown;
+    }
+    if (e != hipSuccess) {
+        log_msg(2, "batched evolution: the round could not be captured as a HIP graph (%s); launching its "
+                   "kernels one by one", hipGetErrorString(e));
+        drop_graph(b, s);
+    }
 }
 
 }  // namespace te
@@ -775,6 +861,26 @@ extern "C" int te_batch_set_graph(te_batch* b, int enable) {
     return TE_OK;
 }
 
+extern "C" int te_batch_set_profiling(te_batch* b, int kinds) {
+    if (!b || kinds < 0 || kinds > 15) return TE_ERR_ARG;
+    b->prof_kinds = kinds;
+    if (kinds && b->prof_ev[0].empty()) {
+        hipSetDevice(b->ctx->device);
+        for (auto& v : b->prof_ev) {
+            v.resize(2 * te::kBatchProfPairs);
+            for (hipEvent_t& e : v) TE_HIP(b->ctx, hipEventCreate(&e));
+        }
+    }
+    return TE_OK;
+}
+
+extern "C" int te_batch_kernel_time(const te_batch* b, int kind, double* total_ms, size_t* launches) {
+    if (!b || kind < 0 || kind > 3 || !total_ms || !launches) return TE_ERR_ARG;
+    *total_ms = b->prof_ms[kind];
+    *launches = b->prof_n[kind];
+    return TE_OK;
+}
+
 #undef TE_TRY
 #undef TE_TRYH
 
@@ -787,6 +893,7 @@ extern "C" int te_batch_run(te_batch* b, double* obs_values, double* const* stat
     const int n_inst = b->n_inst;
     const size_t ns = (size_t)b->n_samples;
 
+    for (int k = 0; k < 4; k++) b->prof_used[k] = 0;
     // initial state: U[:,0] = psi0, the sample at t = 0 (:305-313) through the combine pass
     std::vector<StepState> hst(n_inst);
     std::vector<BatchCtl> hbc(n_inst);
@@ -825,22 +932,15 @@ extern "C" int te_batch_run(te_batch* b, double* obs_values, double* const* stat
                    "recommend recomputing with accurate integration to establish the validity of the result.");
 
     // rounds until no instance is live
-    if (b->use_graph && !b->exec) {
-        TE_HIP(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        batch_round(b, s);
-        batch_combine(b, s);
-        hipGraph_t g = nullptr;
-        TE_HIP(ctx, hipStreamEndCapture(s, &g));
-        b->graph = g;
-        TE_HIP(ctx, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-    }
+    const bool replay = b->use_graph && !b->prof_kinds;
+    if (replay && !b->exec) capture_round(b, s);
     b->rounds = 0;
     TE_HIP(ctx, hipMemcpyAsync(b->n_live_host, b->n_live, sizeof(int), hipMemcpyDeviceToHost, s));
     TE_HIP(ctx, hipStreamSynchronize(s));
     const long long max_rounds = 1LL << 24;
     while (*b->n_live_host > 0) {
         if (b->rounds >= max_rounds) return fail(ctx, TE_ERR_FATAL, "batched evolution makes no progress");
-        if (b->use_graph) {
+        if (replay && b->exec) {
             TE_HIP(ctx, hipGraphLaunch(b->exec, s));
         } else {
             batch_round(b, s);
@@ -871,6 +971,7 @@ extern "C" int te_batch_run(te_batch* b, double* obs_values, double* const* stat
                 TE_HIP(ctx, hipMemcpyAsync(states[i], src, sizeof(double2) * b->dim[i], hipMemcpyDeviceToHost, s));
             }
     TE_HIP(ctx, hipStreamSynchronize(s));
+    if (b->prof_kinds) prof_collect(b);
     int first_fatal = -1, fatal_code = 0;
     for (int i = 0; i < n_inst; i++) {
         const StepState& h = hst[i];
