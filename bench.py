@@ -229,7 +229,9 @@ def bench_batched(np, capi, ctx, args):
     log("batched: %d instances built in %.2f s, sum D = %d, sum nnz = %d" %
         (len(Hs), build_s, sum(dims), sum(H.nnz for H in Hs)))
     be = capi.BatchEvolver(ctx, Hs, psis, T, step, obs, tol=tol, m=m)
-    rb = be.run(want_states=False)                       # warmup (captures the round graph)
+    rb = be.run(want_states=False)                       # warmup (tunes the shape, captures the round graph)
+    shape = be.shape()
+    log("batched: launch shape %s" % shape)
     times = []
     for _ in range(max(args.steps, 1)):
         ctx.synchronize()
@@ -277,6 +279,7 @@ def bench_batched(np, capi, ctx, args):
             "batched_s": round(t_batch, 4), "plain_s": round(t_plain, 4), "krylov_steps": steps_batch,
             "rounds": rounds, "same_steps_and_status_as_plain": same, "max_obs_diff_vs_plain": dobs,
             "build_s": round(build_s, 2),
+            "launch_shape": shape,
             "kernel_ms_per_run": {k: round(v[0], 3) for k, v in kt.items()},
             "kernel_launches_per_run": {k: v[1] for k, v in kt.items()},
             "profiled_run_s": round(t_prof, 4),

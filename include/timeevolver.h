@@ -206,6 +206,21 @@ int te_batch_info(const te_batch* b, size_t* packed_rows, size_t* n_samples, lon
  * The results are identical. A round that cannot be captured or instantiated is launched kernel
  * by kernel (logged as a warning). */
 int te_batch_set_graph(te_batch* b, int enable);
+/* Launch shape of the batched kernels (no reference counterpart): unit_slices sliced-ELL slices
+ * per SpMV unit (a multiple of 4; 0 on the CSR row-block path, whose units are fixed), vec_rows
+ * rows per vector unit (projection, combine; a multiple of 512), blocks per CU of the SpMV and of
+ * the vector launches. By default the first te_batch_run measures a few candidates (one Lanczos
+ * iteration of every instance each: unit_slices in {16, 32, 64, 128}, vec_rows in {1024, 2048,
+ * 4096, 8192}, blocks per CU in {4, 8, 16}), keeps the fastest and caches it for the process
+ * under the instances' sorted (dimension, nnz) pairs; TE_BATCH_TUNE=0 in the environment keeps
+ * the fixed defaults instead. The shape changes no result bit: every per-instance sum adds
+ * partials of 4 slices and of 512 rows in index order. te_batch_set_shape fixes a shape (tuner off); all four
+ * arguments 0 turn the tuner back on for the next run. te_batch_shape reports the shape in use,
+ * source (0 defaults, 1 measured, 2 the process cache, 3 set by the caller) and the time the
+ * measurement took (tune_ms, 0 unless measured); any pointer may be NULL. */
+int te_batch_set_shape(te_batch* b, int unit_slices, int vec_rows, int spmv_blocks_per_cu, int vec_blocks_per_cu);
+int te_batch_shape(const te_batch* b, int* unit_slices, int* vec_rows, int* spmv_blocks_per_cu,
+                   int* vec_blocks_per_cu, int* source, double* tune_ms);
 /* Kernel timing of te_batch_run, for measurement harnesses (no reference counterpart): HIP
  * events recorded around the first 8192 launches of each kind on the context's stream, for the
  * kinds whose bit is set in kinds (0 turns timing off). While timing is on, rounds are launched

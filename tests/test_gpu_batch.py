@@ -155,3 +155,64 @@ def test_batch_sell_and_csr_paths(gpu, monkeypatch):
             assert r["n_steps"] == rp["n_steps"] and r["statusCode"] == rp["statusCode"]
             assert np.linalg.norm(r["state"] - rp["state"]) <= r["err"] + rp["err"] + 1e-10
             assert np.abs(r["observables"] - rp["observables"]).max() <= tol
+
+
+def test_batch_tuned_shape_is_bitwise_the_default(gpu, monkeypatch):
+    """The launch-shape tuner (first run: candidate unit sizes and blocks per CU timed on one
+    Lanczos iteration) changes no result bit: the tuned batch, the batch at the fixed defaults
+    (TE_BATCH_TUNE=0), hand-set extreme shapes and the CSR row-block path's shapes give the same
+    states, observables, errors and step counts; a second batch on the same instance sizes takes
+    the tuned shape from the process cache."""
+    specs = _seeded_specs(404, 24, (6, 7, 8, 9))
+    inst = _instances(gpu, specs, (0, 1))
+    T, step, tol, m = 1.5, 0.25, 1e-8, 30
+    args = ([x[1] for x in inst], [x[4] for x in inst], T, step, [x[2] for x in inst])
+    bt = capi.BatchEvolver(gpu, *args, tol=tol, m=m)
+    ref = bt.run()
+    sh = bt.shape()
+    print("tuned shape", sh)
+    assert sh["source"] == "tuned" and sh["tune_ms"] > 0
+    assert sh["unit_slices"] in (16, 32, 64, 128) and sh["vec_rows"] in (1024, 2048, 4096, 8192)
+    runs = [bt.run()]
+    again = capi.BatchEvolver(gpu, *args, tol=tol, m=m)
+    runs.append(again.run())
+    sh2 = again.shape()
+    assert sh2["source"] == "tuned (cached)"
+    assert {k: sh2[k] for k in ("unit_slices", "vec_rows")} == {k: sh[k] for k in ("unit_slices", "vec_rows")}
+    monkeypatch.setenv("TE_BATCH_TUNE", "0")
+    bd = capi.BatchEvolver(gpu, *args, tol=tol, m=m)
+    monkeypatch.delenv("TE_BATCH_TUNE")
+    runs.append(bd.run())
+    assert bd.shape()["source"] == "default"
+    for shape in ((4, 512, 1, 1), (128, 8192, 16, 16), (8, 1024, 2, 8)):
+        bs = capi.BatchEvolver(gpu, *args, tol=tol, m=m)
+        bs.set_shape(*shape)
+        runs.append(bs.run())
+        assert bs.shape()["source"] == "set"
+    with pytest.raises(capi.TEError) as e:
+        bs.set_shape(4, 768, 8, 8)                      # not a multiple of 512 rows
+    with pytest.raises(capi.TEError):
+        bs.set_shape(6, 512, 8, 8)                      # not a multiple of 4 slices
+    assert e.value.code == capi.TE_ERR_ARG
+    for rr in runs:
+        for a, c in zip(ref, rr):
+            assert a["n_steps"] == c["n_steps"] and a["statusCode"] == c["statusCode"]
+            assert np.float64(a["err"]).view(np.uint64) == np.float64(c["err"]).view(np.uint64)
+            assert np.array_equal(a["state"].view(np.uint64), c["state"].view(np.uint64))
+            assert np.array_equal(a["observables"].view(np.uint64), c["observables"].view(np.uint64))
+    # the CSR row-block path: its SpMV units are fixed, vector units and grids follow the shape
+    monkeypatch.setenv("TE_BATCH_SELL", "0")
+    csr = []
+    for shape in (None, (0, 512, 2, 2), (0, 8192, 16, 4)):
+        bc = capi.BatchEvolver(gpu, *args, tol=tol, m=m)
+        if shape:
+            bc.set_shape(max(shape[0], 4), *shape[1:])
+        csr.append(bc.run())
+    monkeypatch.delenv("TE_BATCH_SELL")
+    for rr in csr[1:]:
+        for a, c in zip(csr[0], rr):
+            assert np.array_equal(a["state"].view(np.uint64), c["state"].view(np.uint64))
+            assert np.array_equal(a["observables"].view(np.uint64), c["observables"].view(np.uint64))
+    for a, c in zip(ref, csr[0]):
+        assert a["n_steps"] == c["n_steps"] and a["statusCode"] == c["statusCode"]
+        assert np.linalg.norm(a["state"] - c["state"]) <= a["err"] + c["err"] + 1e-10

@@ -95,6 +95,8 @@ _SIGS = {
     "te_batch_info": (_c.c_int, [_p, _szp, _szp, _c.POINTER(_c.c_longlong)]),
     "te_batch_set_graph": (_c.c_int, [_p, _c.c_int]),
     "te_batch_set_profiling": (_c.c_int, [_p, _c.c_int]),
+    "te_batch_set_shape": (_c.c_int, [_p, _c.c_int, _c.c_int, _c.c_int, _c.c_int]),
+    "te_batch_shape": (_c.c_int, [_p, _ip, _ip, _ip, _ip, _ip, _dp]),
     "te_batch_kernel_time": (_c.c_int, [_p, _c.c_int, _dp, _szp]),
     "te_adjoint_create": (_c.c_int, [_p, _p, _c.c_int, _pp, _p, _c.POINTER(te_evolve_params), _c.c_int, _pp]),
     "te_adjoint_destroy": (None, [_p]),
@@ -449,6 +451,20 @@ class BatchEvolver:
 
     def set_graph(self, on):
         self.ctx.check(self.ctx.L.te_batch_set_graph(self.h, int(on)), "te_batch_set_graph")
+
+    SHAPE_SOURCES = ("default", "tuned", "tuned (cached)", "set")
+
+    def set_shape(self, unit_slices=0, vec_rows=0, spmv_blocks_per_cu=0, vec_blocks_per_cu=0):
+        """Fix the launch shape (tuner off); all zero: tune again on the next run."""
+        self.ctx.check(self.ctx.L.te_batch_set_shape(self.h, unit_slices, vec_rows, spmv_blocks_per_cu,
+                                                     vec_blocks_per_cu), "te_batch_set_shape")
+
+    def shape(self):
+        v = [_c.c_int() for _ in range(5)]
+        ms = _c.c_double()
+        self.ctx.check(self.ctx.L.te_batch_shape(self.h, *[_c.byref(x) for x in v], _c.byref(ms)), "te_batch_shape")
+        return {"unit_slices": v[0].value, "vec_rows": v[1].value, "spmv_blocks_per_cu": v[2].value,
+                "vec_blocks_per_cu": v[3].value, "source": self.SHAPE_SOURCES[v[4].value], "tune_ms": ms.value}
 
     KINDS = ("apply", "project", "control", "combine")
 

@@ -14,18 +14,25 @@
 //    the project/combine passes walk vector units (up to vrows rows of one instance); each unit
 //    first reads its instance's flags and skips the instance when it is finished, broken down,
 //    or past its Krylov dimension;
-//  * reductions are per instance and deterministic: a unit writes its block partial and
-//    batch_finish_kernel (one block per instance, a launch of its own) sums the instance's
-//    partials in unit order;
+//  * reductions are per instance, deterministic and independent of the launch shape: a
+//    wavefront writes one partial per sliced-ELL slice (apply) or per kBatchChunk-row chunk
+//    (projection, observable sums), and batch_finish_kernel / batch_obs_finish_kernel (one
+//    block per instance, launches of their own) sum the instance's partials in index order, so
+//    unit sizes and grid sizes (which the tuner picks) leave every result bit unchanged;
+//  * the unit sizes and blocks per CU are measured on the first run (a few candidates, one
+//    Lanczos iteration each) unless the caller fixes them, and cached per instance-size key;
 //  * the step controller runs one block per instance (launch_batch_step_control);
 //  * one round (m_max Lanczos iterations, controller, combine, observable sums, live count) is
 //    captured once as a HIP graph and replayed until no instance is live. The graph holds
 //    kernel nodes only.
 #include <algorithm>
 #include <cfloat>
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "te_internal.h"
@@ -36,16 +43,21 @@
 namespace te {
 
 constexpr int kBatchRB = 4;          // row blocks (kStreamChunk entries + rows each) per CSR SpMV unit
-// Slices (kSlice rows each) per sliced-ELL SpMV unit: the batch's slices per block of the grid,
-// clamped to [16, 32]. Each unit ends with a block reduction and may restage its instance's
-// value table; with the instance sums in batch_finish_kernel small units balance best: on 512
-// ring instances with L in {6..10} (181k slices, 4096-row vector units) the batched run took
+// Default slices (kSlice rows each) per sliced-ELL SpMV unit: the batch's slices per block of the
+// grid, clamped to [16, 32]. A unit may restage its instance's value table; on 512 ring
+// instances with L in {6..10} (181k slices, 4096-row vector units) the batched run took
 // 0.1473 / 0.1462 / 0.1470 / 0.1489 s with 16 / 24 / 32 / 48-slice units, 0.1601 s with 88.
 constexpr int kBatchSlicesMin = 16, kBatchSlicesMax = 32;
-// Rows per vector unit (projection, combine), grown when the observable partials would be large:
-// 2048 / 4096 rows gave 0.1467 / 0.1470 s with 32-slice units; 4096 / 8192 / 16384 rows
-// 0.1601 / 0.1628 / 0.1667 s with 88-slice units.
+// Default rows per vector unit (projection, combine): 2048 / 4096 rows gave 0.1467 / 0.1470 s
+// with 32-slice units; 4096 / 8192 / 16384 rows 0.1601 / 0.1628 / 0.1667 s with 88-slice units.
 constexpr int kBatchVecRows = 4096;
+constexpr int kBatchBlocksPerCU = 8; // default blocks per CU of the streaming launches
+// Rows per reduction chunk of the vector passes: one wavefront's share (8 rows a lane), whose
+// partial is one entry of the instance's partials. Vector units are whole chunks.
+constexpr int kBatchChunk = 8 * 64;
+// Slices per reduction group of the sliced-ELL apply: a wavefront takes the group's slices one
+// after the other and writes the group's <u_j, w> as one partial. SpMV units are whole groups.
+constexpr int kBatchGroup = 4;
 constexpr int kBatchSC = 8;          // samples per pass of the combine kernel
 constexpr int kBatchMaxObs = 4;      // diagonal observables per instance
 constexpr int kBatchTraceCap = 4096; // recorded step sizes per instance
@@ -53,8 +65,10 @@ constexpr int kBatchTraceCap = 4096; // recorded step sizes per instance
 struct BatchInst {
     int64_t off;            // first row of the instance in the packed vectors
     int n;                  // dimension
-    int nsu, su0;           // SpMV units [su0, su0 + nsu)
-    int nvu, vu0;           // vector units [vu0, vu0 + nvu)
+    int nsu, su0;           // CSR SpMV units [su0, su0 + nsu) (row-block path; fixed size)
+    int nsl, sl0;           // apply partials [sl0, sl0 + nsl): one per group of kBatchGroup slices
+                            // (sliced ELL) or per CSR unit
+    int nch, ch0;           // kBatchChunk-row chunks [ch0, ch0 + nch): projection and observable partials
     const int* ptr;         // the instance's CSR (rows and columns local to the instance)
     const int* col;
     const double2* val;
@@ -66,8 +80,8 @@ struct BatchDev {
     const BatchInst* inst;
     const int4* sunit;      // (instance, first row block, end row block, 0), or slices for sliced ELL
     int n_sunits;
-    const int4* vunit;      // (instance, first row, end row, 0), rows local to the instance
-    int n_vunits;
+    const int4* vunit;      // (instance, first row, end row, 0), rows local to the instance; rows
+    int n_vunits;           // start on multiples of kBatchChunk
     double2* U;             // packed Krylov bases, column-major, ld rows per column
     int64_t ld;
     double2 alpha;          // -i expFactor
@@ -75,14 +89,14 @@ struct BatchDev {
     KrylovScalars* ks;      // per instance
     StepState* st;
     BatchCtl* bc;
-    double2* spart;         // per SpMV unit
-    double2* vpart;         // per vector unit
+    double2* spart;         // per slice (sliced ELL) or per CSR SpMV unit
+    double2* vpart;         // per chunk
     const double2* wcoef;   // per instance, kMaxKrylov
     const double2* Y;       // per instance, n_samples x kMaxKrylov
     long long n_samples;
     const double* diag;     // nobs x ld: Re of the diagonal observables, packed like the vectors
     int nobs;
-    double* opart;          // n_vunits x nobs x n_samples
+    double* opart;          // chunks x nobs x n_samples
     double* obs;            // n_inst x nobs x n_samples
     double2* sampled;       // packed last sampled states
     double* nrm;            // per instance |sampled|^2 (final pass)
@@ -141,17 +155,18 @@ __global__ __launch_bounds__(kBlock) void batch_apply_kernel(BatchDev a, int j) 
         for (int b = su.y; b < su.z; b++)
             stream_row_block(I.rb[b], I.rb[b + 1], I.ptr, I.col, I.val, epi.uj, prod, epi, nullptr);
         const double2 bs = block_sum2<kBlock>(epi.dot, sh);
-        if (threadIdx.x == 0) a.spart[u] = bs;
+        if (threadIdx.x == 0) a.spart[I.sl0 + (u - I.su0)] = bs;      // the unit's partial
     }
 }
 
 // The same for instances held as sliced-ELL value-dictionary copies (vectors in each instance's
 // locality order): a unit is a run of slices of one instance, its wavefronts take the
 // slices in turn, and the instance's value table is staged in LDS when the unit's instance
-// differs from the block's previous one.
+// differs from the block's previous one. A wavefront takes a group of kBatchGroup consecutive
+// slices; the group's <u_j, w> goes to the group's partial (a wavefront sum), whatever unit or
+// block computed it.
 __global__ __launch_bounds__(kBlock) void batch_apply_sell_kernel(BatchDev a, int j) {
     __shared__ double2 vt[kVdMaxValues];
-    __shared__ double2 sh[kBlock / 64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     int staged = -1;
     int u0, u1, us;
@@ -177,25 +192,31 @@ __global__ __launch_bounds__(kBlock) void batch_apply_sell_kernel(BatchDev a, in
         epi.w = a.U + (size_t)(j + 1) * a.ld + I.off;
         epi.dot = make_double2(0.0, 0.0);
         const double2* x = a.U + (size_t)j * a.ld + I.off;
-        for (int sl = su.y + wv; sl < su.z; sl += kBlock / 64)
-            sell_slice(m, x, vt, __builtin_amdgcn_readfirstlane(sl), lane, epi);
-        const double2 bs = block_sum2<kBlock>(epi.dot, sh);
-        if (threadIdx.x == 0) a.spart[u] = bs;
+        const int g1 = (su.z + kBatchGroup - 1) / kBatchGroup;
+        for (int g = su.y / kBatchGroup + wv; g < g1; g += kBlock / 64) {
+            const int gw = __builtin_amdgcn_readfirstlane(g);
+            const int se = min((gw + 1) * kBatchGroup, su.z);
+            epi.dot = make_double2(0.0, 0.0);
+            for (int sl = gw * kBatchGroup; sl < se; sl++) sell_slice(m, x, vt, sl, lane, epi);
+            const double dx = wave_sum(epi.dot.x), dy = wave_sum(epi.dot.y);
+            if (lane == 0) a.spart[I.sl0 + gw] = make_double2(dx, dy);   // the group's partial
+        }
     }
 }
 
 // Lanczos project for iteration j: w' = w - (a_j s_j) u_j, beta_j = |w'|, stored in place
 // unless j + 1 is the instance's Krylov dimension; breakdown when beta_j < tolRate
-// (core/krylovTimeEvolver.cpp:551-567).
+// (core/krylovTimeEvolver.cpp:551-567). A wavefront takes a chunk of kBatchChunk rows (lane l:
+// rows l, l + 64, ..., in order) and writes the chunk's |w'|^2 as its partial.
 __global__ __launch_bounds__(kBlock) void batch_project_kernel(BatchDev a, int j) {
-    __shared__ double2 sh[kBlock / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     int u0, u1, us;
     xcd_sweep(a.n_vunits, u0, u1, us);
     for (int u = u0; u < u1; u += us) {
         const int4 vu = a.vunit[u];
         const int i = vu.x;
         if (!lanczos_active(a, i, j)) continue;
-        const BatchInst I = a.inst[i];
+        const BatchInst& I = a.inst[i];
         KrylovScalars* ks = a.ks + i;
         const int m = a.st[i].m;
         const double sj = ks->s[j];
@@ -204,40 +225,42 @@ __global__ __launch_bounds__(kBlock) void batch_project_kernel(BatchDev a, int j
         const double2* __restrict__ uj = a.U + (size_t)j * a.ld + I.off;
         double2* __restrict__ w = a.U + (size_t)(j + 1) * a.ld + I.off;
         const bool store = j + 1 < m;
-        double nrm = 0.0;
-        int row = vu.y + threadIdx.x;
-        for (; row + 3 * kBlock < vu.z; row += 4 * kBlock) {
-            double2 uu[4], wr[4];
+        const int cb = vu.y / kBatchChunk, ce = (vu.z + kBatchChunk - 1) / kBatchChunk;
+        for (int c = cb + wv; c < ce; c += kBlock / 64) {
+            constexpr int kR = kBatchChunk / 64;
+            const int r0 = c * kBatchChunk + lane;
+            double2 uu[kR], wr[kR];
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                uu[q] = uj[row + q * kBlock];
-                wr[q] = w[row + q * kBlock];
+            for (int q = 0; q < kR; q++) {
+                const int r = r0 + 64 * q;
+                uu[q] = make_double2(0.0, 0.0);
+                wr[q] = make_double2(0.0, 0.0);
+                if (r < vu.z) {
+                    uu[q] = uj[r];
+                    wr[q] = w[r];
+                }
             }
+            double nrm = 0.0;
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
+            for (int q = 0; q < kR; q++) {
+                const int r = r0 + 64 * q;
                 wr[q].x -= c3.x * uu[q].x - c3.y * uu[q].y;
                 wr[q].y -= c3.x * uu[q].y + c3.y * uu[q].x;
-                if (store) w[row + q * kBlock] = wr[q];
-                nrm += wr[q].x * wr[q].x + wr[q].y * wr[q].y;
+                if (r < vu.z) {
+                    if (store) w[r] = wr[q];
+                    nrm += wr[q].x * wr[q].x + wr[q].y * wr[q].y;
+                }
             }
+            nrm = wave_sum(nrm);
+            if (lane == 0) a.vpart[I.ch0 + c] = make_double2(nrm, 0.0);
         }
-        for (; row < vu.z; row += kBlock) {
-            const double2 uu = uj[row];
-            double2 wr = w[row];
-            wr.x -= c3.x * uu.x - c3.y * uu.y;
-            wr.y -= c3.x * uu.y + c3.y * uu.x;
-            if (store) w[row] = wr;
-            nrm += wr.x * wr.x + wr.y * wr.y;
-        }
-        const double2 bs = block_sum2<kBlock>(make_double2(nrm, 0.0), sh);
-        if (threadIdx.x == 0) a.vpart[u] = bs;
     }
 }
 
 // One block per instance after an apply (PROJECT false: alpha_j = s_j <u_j, w>) or a projection
 // (PROJECT true: beta_j = |w'|, stored with s_{j+1} = 1 / beta_j, or h when j + 1 is the Krylov
 // dimension; breakdown when beta_j < tolRate, core/krylovTimeEvolver.cpp:551-567) of iteration j:
-// the instance's unit partials summed in unit order. In a launch of its own: a reduction by the
+// the instance's slice (or CSR unit) or chunk partials summed in index order. In a launch of its own: a reduction by the
 // instance's last unit inside the apply or projection costs every unit an agent-scope release,
 // which on this multi-XCD part writes back the XCD's L2 (per-unit tickets made the projection
 // of 512 instances about 1.7x slower in a probe of its shape).
@@ -248,8 +271,8 @@ __global__ __launch_bounds__(kBlock) void batch_finish_kernel(BatchDev a, int j)
     if (i >= a.n_inst || !lanczos_active(a, i, j)) return;
     const BatchInst& I = a.inst[i];
     KrylovScalars* ks = a.ks + i;
-    const double2 tot = PROJECT ? reduce_partials<kBlock>(a.vpart + I.vu0, I.nvu, 1, sh)
-                                : reduce_partials<kBlock>(a.spart + I.su0, I.nsu, 1, sh);
+    const double2 tot = PROJECT ? reduce_partials<kBlock>(a.vpart + I.ch0, I.nch, 1, sh)
+                                : reduce_partials<kBlock>(a.spart + I.sl0, I.nsl, 1, sh);
     if (threadIdx.x != 0) return;
     if (!PROJECT) {
         const double sj = ks->s[j];
@@ -272,14 +295,15 @@ __global__ __launch_bounds__(kBlock) void batch_finish_kernel(BatchDev a, int j)
 
 // After the controller: for every instance that stepped, currentVec = U w written over column
 // 0 when the step asks for it (:382), the step's samples U y_s (:409-421), the last of them
-// kept as the sampled state, and per-unit partial sums of the diagonal observables
-// sum_r Re(O_rr) |psi_s(r)|^2. Samples go kBatchSC per pass over U; the pass that covers the
-// last sample is the one that overwrites column 0, after every pass has read it.
+// kept as the sampled state, and per-chunk partial sums of the diagonal observables
+// sum_r Re(O_rr) |psi_s(r)|^2 (a wavefront per kBatchChunk-row chunk, lane l: rows l, l + 64,
+// ...). Samples go kBatchSC per pass over U; the pass that covers the last sample is the one
+// that overwrites column 0, after every pass has read it.
 template <int NOB>
 __global__ __launch_bounds__(kBlock) void batch_combine_kernel(BatchDev a) {
     constexpr int kBatchBurst = 8;
     __shared__ double2 cf[(1 + kBatchSC) * kMaxKrylov];
-    __shared__ double2 sh[kBlock / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     int u0, u1, us;
     xcd_sweep(a.n_vunits, u0, u1, us);
     for (int u = u0; u < u1; u += us) {
@@ -291,9 +315,11 @@ __global__ __launch_bounds__(kBlock) void batch_combine_kernel(BatchDev a) {
         const int S = st->S;
         const long long y0 = st->sample_base;
         const int write_w = st->write_w;
-        const int64_t off = a.inst[i].off;
+        const BatchInst& I = a.inst[i];
+        const int64_t off = I.off;
         const double2* Yi = a.Y + (size_t)i * a.n_samples * kMaxKrylov;
         const double2* wc = a.wcoef + (size_t)i * kMaxKrylov;
+        const int cb = vu.y / kBatchChunk, ce = (vu.z + kBatchChunk - 1) / kBatchChunk;
         for (int c0 = 0; c0 < S || (c0 == 0 && write_w); c0 += kBatchSC) {
             const int sc = min(kBatchSC, S - c0);      // samples of this pass (may be <= 0)
             const bool lastpass = c0 + kBatchSC >= S;
@@ -311,66 +337,69 @@ __global__ __launch_bounds__(kBlock) void batch_combine_kernel(BatchDev a) {
                 cf[q] = v;
             }
             __syncthreads();
-            double acc[NOB > 0 ? NOB * kBatchSC : 1];
+            for (int ch = cb + wv; ch < ce; ch += kBlock / 64) {
+                double acc[NOB > 0 ? NOB * kBatchSC : 1];
 #pragma unroll
-            for (int q = 0; q < (NOB > 0 ? NOB * kBatchSC : 1); q++) acc[q] = 0.0;
-            for (int row = vu.y + threadIdx.x; row < vu.z; row += kBlock) {
-                const size_t gr = (size_t)off + row;
-                double2 aw = make_double2(0.0, 0.0);
-                double2 as[kBatchSC];
+                for (int q = 0; q < (NOB > 0 ? NOB * kBatchSC : 1); q++) acc[q] = 0.0;
+                for (int qr = 0; qr < kBatchChunk / 64; qr++) {
+                    const int row = ch * kBatchChunk + lane + 64 * qr;
+                    if (row >= vu.z) break;
+                    const size_t gr = (size_t)off + row;
+                    double2 aw = make_double2(0.0, 0.0);
+                    double2 as[kBatchSC];
 #pragma unroll
-                for (int s = 0; s < kBatchSC; s++) as[s] = make_double2(0.0, 0.0);
-                auto add = [&](int jj, double2 uv) {
-                    if (ww) {
-                        const double2 t = cmul(uv, cf[jj]);
-                        aw.x += t.x;
-                        aw.y += t.y;
+                    for (int s = 0; s < kBatchSC; s++) as[s] = make_double2(0.0, 0.0);
+                    auto add = [&](int jj, double2 uv) {
+                        if (ww) {
+                            const double2 t = cmul(uv, cf[jj]);
+                            aw.x += t.x;
+                            aw.y += t.y;
+                        }
+#pragma unroll
+                        for (int s = 0; s < kBatchSC; s++) {
+                            if (s < sc) {
+                                const double2 t = cmul(uv, cf[(1 + s) * kMaxKrylov + jj]);
+                                as[s].x += t.x;
+                                as[s].y += t.y;
+                            }
+                        }
+                    };
+                    // kBatchBurst basis columns in flight per lane (sums still in order of jj)
+                    int jj = 0;
+                    for (; jj + kBatchBurst <= m; jj += kBatchBurst) {
+                        double2 uv[kBatchBurst];
+#pragma unroll
+                        for (int q = 0; q < kBatchBurst; q++) uv[q] = a.U[(size_t)(jj + q) * a.ld + gr];
+#pragma unroll
+                        for (int q = 0; q < kBatchBurst; q++) add(jj + q, uv[q]);
                     }
+                    for (; jj < m; jj++) add(jj, a.U[(size_t)jj * a.ld + gr]);
+                    if (ww) a.U[gr] = aw;
+                    if (lastpass && sc > 0) {
+                        double2 v = as[0];
+#pragma unroll
+                        for (int s = 1; s < kBatchSC; s++)
+                            if (s == sc - 1) v = as[s];
+                        a.sampled[gr] = v;
+                    }
+                    double dg[NOB > 0 ? NOB : 1];              // each observable's diagonal read once a row
+#pragma unroll
+                    for (int o = 0; o < NOB; o++) dg[o] = o < a.nobs ? a.diag[(size_t)o * a.ld + gr] : 0.0;
 #pragma unroll
                     for (int s = 0; s < kBatchSC; s++) {
-                        if (s < sc) {
-                            const double2 t = cmul(uv, cf[(1 + s) * kMaxKrylov + jj]);
-                            as[s].x += t.x;
-                            as[s].y += t.y;
-                        }
+                        const double pr = as[s].x * as[s].x + as[s].y * as[s].y;
+#pragma unroll
+                        for (int o = 0; o < NOB; o++)
+                            if (o < a.nobs && s < sc) acc[o * kBatchSC + s] += dg[o] * pr;
                     }
-                };
-                // kBatchBurst basis columns in flight per thread (sums still in order of jj)
-                int jj = 0;
-                for (; jj + kBatchBurst <= m; jj += kBatchBurst) {
-                    double2 uv[kBatchBurst];
-#pragma unroll
-                    for (int q = 0; q < kBatchBurst; q++) uv[q] = a.U[(size_t)(jj + q) * a.ld + gr];
-#pragma unroll
-                    for (int q = 0; q < kBatchBurst; q++) add(jj + q, uv[q]);
                 }
-                for (; jj < m; jj++) add(jj, a.U[(size_t)jj * a.ld + gr]);
-                if (ww) a.U[gr] = aw;
-                if (lastpass && sc > 0) {
-                    double2 v = as[0];
 #pragma unroll
-                    for (int s = 1; s < kBatchSC; s++)
-                        if (s == sc - 1) v = as[s];
-                    a.sampled[gr] = v;
-                }
-                double dg[NOB > 0 ? NOB : 1];              // each observable's diagonal read once a row
-#pragma unroll
-                for (int o = 0; o < NOB; o++) dg[o] = o < a.nobs ? a.diag[(size_t)o * a.ld + gr] : 0.0;
-#pragma unroll
-                for (int s = 0; s < kBatchSC; s++) {
-                    const double pr = as[s].x * as[s].x + as[s].y * as[s].y;
-#pragma unroll
-                    for (int o = 0; o < NOB; o++)
-                        if (o < a.nobs && s < sc) acc[o * kBatchSC + s] += dg[o] * pr;
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < NOB * kBatchSC; q++) {
-                const int o = q / kBatchSC, s = q % kBatchSC;
-                if (o < a.nobs && s < sc) {          // uniform in the block
-                    const double2 t = block_sum2<kBlock>(make_double2(acc[q], 0.0), sh);
-                    if (threadIdx.x == 0)
-                        a.opart[((size_t)u * a.nobs + o) * a.n_samples + y0 + c0 + s] = t.x;
+                for (int q = 0; q < NOB * kBatchSC; q++) {
+                    const int o = q / kBatchSC, s = q % kBatchSC;
+                    if (o < a.nobs && s < sc) {          // uniform in the wavefront
+                        const double t = wave_sum(acc[q]);
+                        if (lane == 0) a.opart[((size_t)(I.ch0 + ch) * a.nobs + o) * a.n_samples + y0 + c0 + s] = t;
+                    }
                 }
             }
             __syncthreads();                     // cf is reloaded by the next pass
@@ -379,7 +408,7 @@ __global__ __launch_bounds__(kBlock) void batch_combine_kernel(BatchDev a) {
 }
 
 // One block per instance that stepped: its observable values for the step's samples, summed
-// over its vector units in unit order.
+// over its chunks in index order.
 __global__ __launch_bounds__(kBlock) void batch_obs_finish_kernel(BatchDev a) {
     const int i = blockIdx.x;
     if (!a.bc[i].stepped || a.nobs == 0) return;
@@ -389,7 +418,7 @@ __global__ __launch_bounds__(kBlock) void batch_obs_finish_kernel(BatchDev a) {
     for (int p = threadIdx.x; p < a.nobs * S; p += kBlock) {
         const int o = p / S, s = p % S;
         double t = 0.0;
-        for (int q = 0; q < I.nvu; q++) t += a.opart[((size_t)(I.vu0 + q) * a.nobs + o) * a.n_samples + y0 + s];
+        for (int q = 0; q < I.nch; q++) t += a.opart[((size_t)(I.ch0 + q) * a.nobs + o) * a.n_samples + y0 + s];
         a.obs[((size_t)i * a.nobs + o) * a.n_samples + y0 + s] = t;
     }
 }
@@ -434,6 +463,15 @@ struct te_batch {
     BatchDev dev{};
     int grid_s = 8, grid_v = 8;
     bool sell = false;                 // instances held as sliced-ELL copies (locality order)
+    std::vector<int> n_slices, n_rblocks;   // per instance: sliced-ELL slices, CSR row blocks
+    int64_t total_slices = 0, total_groups = 0, total_chunks = 0;
+    // launch shape: slices per SpMV unit, rows per vector unit, blocks per CU of the SpMV and
+    // the vector launches; source 0 default, 1 measured by the tuner, 2 the tuner's cache, 3 set
+    // by the caller. tune: measure on the next run.
+    int unit_slices = 0, vec_rows = 0, spmv_bpc = 0, vec_bpc = 0, shape_source = 0;
+    bool tune = true;
+    double tune_ms = 0.0;
+    uint64_t shape_key = 0;
     std::vector<const int*> perm, iperm;
     double2* tmp = nullptr;            // packed staging vector (reordering)
     // device buffers
@@ -589,6 +627,153 @@ static void capture_round(te_batch* b, hipStream_t s) {
     }
 }
 
+// Unit tables of a launch shape: SpMV units of unit_slices slices of one instance (sliced ELL;
+// the CSR path keeps its fixed kBatchRB-row-block units), vector units of vec_rows rows (a
+// multiple of kBatchChunk) of one instance.
+static void shape_units(const te_batch* b, int unit_slices, int vec_rows, std::vector<int4>& su,
+                        std::vector<int4>& vu) {
+    su.clear();
+    vu.clear();
+    for (int i = 0; i < b->n_inst; i++) {
+        if (b->sell)
+            for (int sl = 0; sl < b->n_slices[i]; sl += unit_slices)
+                su.push_back(make_int4(i, sl, std::min(sl + unit_slices, b->n_slices[i]), 0));
+        for (int r = 0; r < b->dim[i]; r += vec_rows) vu.push_back(make_int4(i, r, std::min(r + vec_rows, b->dim[i]), 0));
+    }
+}
+
+static int shape_grid(const te_batch* b, int items, int blocks_per_cu) {
+    const int g = std::min(items, b->ctx->num_cus * blocks_per_cu);
+    return std::max(8, (g + 7) / 8 * 8);
+}
+
+// Uploads the unit tables of a shape (asynchronously on the context's stream) and points the
+// launch arguments at them. Any captured round graph is dropped: it holds the old arguments.
+static int apply_shape(te_batch* b, int unit_slices, int vec_rows, int spmv_bpc, int vec_bpc) {
+    te_ctx* ctx = b->ctx;
+    hipStream_t s = ctx->stream;
+    std::vector<int4> su, vu;
+    shape_units(b, unit_slices, vec_rows, su, vu);
+    if (b->sell) TE_HIP(ctx, hipMemcpyAsync(b->sunit, su.data(), sizeof(int4) * su.size(), hipMemcpyHostToDevice, s));
+    TE_HIP(ctx, hipMemcpyAsync(b->vunit, vu.data(), sizeof(int4) * vu.size(), hipMemcpyHostToDevice, s));
+    // the host vectors must outlive the copies
+    TE_HIP(ctx, hipStreamSynchronize(s));
+    if (b->sell) b->dev.n_sunits = (int)su.size();
+    b->dev.n_vunits = (int)vu.size();
+    b->unit_slices = b->sell ? unit_slices : 0;
+    b->vec_rows = vec_rows;
+    b->spmv_bpc = spmv_bpc;
+    b->vec_bpc = vec_bpc;
+    b->grid_s = shape_grid(b, b->dev.n_sunits, spmv_bpc);
+    b->grid_v = shape_grid(b, b->dev.n_vunits, vec_bpc);
+    if (b->exec || b->graph) {
+        if (b->exec) hipGraphExecDestroy(b->exec);
+        if (b->graph) hipGraphDestroy(b->graph);
+        b->exec = nullptr;
+        b->graph = nullptr;
+    }
+    return TE_OK;
+}
+
+struct BatchShape {
+    int unit_slices, vec_rows, spmv_bpc, vec_bpc;
+};
+
+// Tuned shapes of this process, by instance-size key.
+static std::mutex g_shape_mu;
+static std::map<uint64_t, BatchShape> g_shape_cache;
+
+// The tuner: times Lanczos iteration 0 of every live instance (the apply with its finish
+// launch, then the projection with its finish launch) for a few candidate shapes and keeps the
+// fastest of each kind (median of 3 timed launches after one untimed). Unit sizes and grids
+// change no result bit (per-slice and per-chunk partials), so the choice only moves time. Runs
+// with the run's initial state in place (U[:, 0] = psi0, scalars and flags uploaded); the
+// caller uploads the Krylov scalars again afterwards (the timed iterations write them).
+static int tune_shape(te_batch* b) {
+    te_ctx* ctx = b->ctx;
+    hipStream_t s = ctx->stream;
+    const auto t0 = std::chrono::steady_clock::now();
+    const int reps = 3;
+    hipEvent_t ev[2 * (reps + 1)];
+    for (auto& e : ev) TE_HIP(ctx, hipEventCreate(&e));
+    auto time_kind = [&](bool spmv, float* med) -> int {
+        for (int r = 0; r <= reps; r++) {
+            TE_HIP(ctx, hipEventRecord(ev[2 * r], s));
+            if (spmv) {
+                if (b->sell)
+                    hipLaunchKernelGGL(batch_apply_sell_kernel, dim3(b->grid_s), dim3(kBlock), 0, s, b->dev, 0);
+                else
+                    hipLaunchKernelGGL(batch_apply_kernel, dim3(b->grid_s), dim3(kBlock), 0, s, b->dev, 0);
+                hipLaunchKernelGGL(batch_finish_kernel<false>, dim3(b->n_inst), dim3(kBlock), 0, s, b->dev, 0);
+            } else {
+                hipLaunchKernelGGL(batch_project_kernel, dim3(b->grid_v), dim3(kBlock), 0, s, b->dev, 0);
+                hipLaunchKernelGGL(batch_finish_kernel<true>, dim3(b->n_inst), dim3(kBlock), 0, s, b->dev, 0);
+            }
+            TE_HIP(ctx, hipEventRecord(ev[2 * r + 1], s));
+        }
+        TE_HIP(ctx, hipGetLastError());
+        TE_HIP(ctx, hipStreamSynchronize(s));
+        float t[reps];
+        for (int r = 1; r <= reps; r++) TE_HIP(ctx, hipEventElapsedTime(&t[r - 1], ev[2 * r], ev[2 * r + 1]));
+        std::sort(t, t + reps);
+        *med = t[reps / 2];
+        return TE_OK;
+    };
+    int rc = TE_OK;
+    BatchShape best{b->unit_slices, b->vec_rows, b->spmv_bpc, b->vec_bpc};
+    // the apply needs its own w; the projection reads it: SpMV candidates first, then the
+    // vector candidates on the w the last apply left
+    if (b->sell) {
+        float best_t = 1e30f;
+        for (int us : {16, 32, 64, 128})
+            for (int bpc : {4, 8, 16}) {
+                if ((rc = apply_shape(b, us, best.vec_rows, bpc, best.vec_bpc)) != TE_OK) goto done;
+                float t = 0.f;
+                if ((rc = time_kind(true, &t)) != TE_OK) goto done;
+                if (t < best_t) {
+                    best_t = t;
+                    best.unit_slices = us;
+                    best.spmv_bpc = bpc;
+                }
+            }
+    } else {
+        float best_t = 1e30f;
+        for (int bpc : {4, 8, 16}) {
+            if ((rc = apply_shape(b, 0, best.vec_rows, bpc, best.vec_bpc)) != TE_OK) goto done;
+            float t = 0.f;
+            if ((rc = time_kind(true, &t)) != TE_OK) goto done;
+            if (t < best_t) {
+                best_t = t;
+                best.spmv_bpc = bpc;
+            }
+        }
+    }
+    {
+        float best_t = 1e30f;
+        for (int vr : {1024, 2048, 4096, 8192})
+            for (int bpc : {4, 8, 16}) {
+                if ((rc = apply_shape(b, best.unit_slices, vr, best.spmv_bpc, bpc)) != TE_OK) goto done;
+                float t = 0.f;          // (each projection moves w further; the data do not matter here)
+                if ((rc = time_kind(false, &t)) != TE_OK) goto done;
+                if (t < best_t) {
+                    best_t = t;
+                    best.vec_rows = vr;
+                    best.vec_bpc = bpc;
+                }
+            }
+    }
+    rc = apply_shape(b, best.unit_slices, best.vec_rows, best.spmv_bpc, best.vec_bpc);
+done:
+    for (auto& e : ev) hipEventDestroy(e);
+    b->tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (rc == TE_OK) {
+        b->shape_source = 1;
+        std::lock_guard<std::mutex> lk(g_shape_mu);
+        g_shape_cache[b->shape_key] = best;
+    }
+    return rc;
+}
+
 }  // namespace te
 
 #undef TE_TRY
@@ -635,24 +820,22 @@ extern "C" int te_batch_create(te_ctx* ctx, int n_inst, te_matrix* const* H, con
             all = H[i] && H[i]->use_vd && H[i]->vd.sell && H[i]->vd.n_values <= kVdMaxValues;
         b->sell = all;
     }
-    // launch shape: blocks per CU of the streaming kernels (8: 0.247 s against 0.278 s at 4)
-    const char* gs_env = std::getenv("TE_BATCH_BLOCKS_PER_CU");     // launch-shape sweeps
-    const int per_cu = gs_env ? std::max(1, std::atoi(gs_env)) : 8;
-    const int grid_cap = std::max(8, ctx->num_cus * per_cu);
-    int unit_slices = kBatchSlicesMin;
-    if (b->sell) {
-        int64_t total_slices = 0;
-        for (int i = 0; i < n_inst; i++) total_slices += H[i]->vd.n_slices;
-        unit_slices = (int)std::max<int64_t>(kBatchSlicesMin, std::min<int64_t>(kBatchSlicesMax, total_slices / grid_cap));
+    // default launch shape (the tuner may replace it on the first run; TE_BATCH_TUNE=0 keeps it).
+    // Blocks per CU of the streaming kernels: 8 (0.247 s against 0.278 s at 4).
+    b->spmv_bpc = b->vec_bpc = kBatchBlocksPerCU;
+    b->vec_rows = kBatchVecRows;
+    {
+        const char* e = std::getenv("TE_BATCH_TUNE");
+        b->tune = !(e && e[0] == '0');
     }
-    const char* us_env = std::getenv("TE_BATCH_UNIT_SLICES");       // launch-shape sweeps
-    if (us_env) unit_slices = std::max(1, std::atoi(us_env));
     b->perm.assign(n_inst, nullptr);
     b->iperm.assign(n_inst, nullptr);
-    // instances: dimensions, Krylov dimensions, unit tables
+    // instances: dimensions, Krylov dimensions, partial ranges
     std::vector<BatchInst> hinst(n_inst);
-    std::vector<int4> hsu, hvu;
+    std::vector<int4> hsu;
     int64_t total = 0;
+    b->n_slices.assign(n_inst, 0);
+    b->n_rblocks.assign(n_inst, 0);
     for (int i = 0; i < n_inst; i++) {
         te_matrix* A = H[i];
         if (!A || !psi0[i]) {
@@ -710,53 +893,83 @@ extern "C" int te_batch_create(te_ctx* ctx, int n_inst, te_matrix* const* H, con
         I.val = A->val;
         I.rb = A->row_block;
         I.su0 = (int)hsu.size();
+        b->n_rblocks[i] = A->n_row_blocks;
         if (b->sell) {
             I.sell = sell_view_of(A);
             b->perm[i] = A->vd.perm;
             b->iperm[i] = A->vd.iperm;
-            for (int sl = 0; sl < I.sell.n_slices; sl += unit_slices)
-                hsu.push_back(make_int4(i, sl, std::min(sl + unit_slices, I.sell.n_slices), 0));
+            b->n_slices[i] = I.sell.n_slices;
+            I.nsu = 0;
+            I.sl0 = (int)b->total_groups;
+            I.nsl = (I.sell.n_slices + kBatchGroup - 1) / kBatchGroup;
+            b->total_slices += I.sell.n_slices;
+            b->total_groups += I.nsl;
         } else {
             I.sell = SellView{};
             for (int r = 0; r < A->n_row_blocks; r += kBatchRB)
                 hsu.push_back(make_int4(i, r, std::min(r + kBatchRB, A->n_row_blocks), 0));
+            I.nsu = (int)hsu.size() - I.su0;
+            I.sl0 = I.su0;
+            I.nsl = I.nsu;
         }
-        I.nsu = (int)hsu.size() - I.su0;
+        I.ch0 = (int)b->total_chunks;
+        I.nch = (n + kBatchChunk - 1) / kBatchChunk;
+        b->total_chunks += I.nch;
     }
     b->ld = total;
-    // vector units: kBatchVecRows rows, grown so the observable partials stay under 256 MiB
-    const char* vr_env = std::getenv("TE_BATCH_VEC_ROWS");          // launch-shape sweeps
-    int vrows = vr_env ? std::max(256, std::atoi(vr_env)) : kBatchVecRows;
-    while (true) {
-        int64_t nvu = 0;
-        for (int i = 0; i < n_inst; i++) nvu += (b->dim[i] + vrows - 1) / vrows;
-        if ((double)nvu * n_obs * (double)b->n_samples * 8.0 <= 256.0 * 1048576.0 || vrows >= (1 << 28)) break;
-        vrows *= 2;
+    if (b->sell) {
+        int64_t per = b->total_slices / std::max(8, ctx->num_cus * b->spmv_bpc);
+        b->unit_slices = (int)std::max<int64_t>(kBatchSlicesMin, std::min<int64_t>(kBatchSlicesMax, per)) /
+                         kBatchGroup * kBatchGroup;
+    } else {
+        b->unit_slices = 0;                 // CSR units are kBatchRB row blocks
     }
-    for (int i = 0; i < n_inst; i++) {
-        hinst[i].vu0 = (int)hvu.size();
-        for (int r = 0; r < b->dim[i]; r += vrows) hvu.push_back(make_int4(i, r, std::min(r + vrows, b->dim[i]), 0));
-        hinst[i].nvu = (int)hvu.size() - hinst[i].vu0;
+    if (b->total_chunks >= (1LL << 31) || (int64_t)hsu.size() >= (1LL << 31)) {
+        batch_free(b);
+        return fail(ctx, TE_ERR_UNSUPPORTED, "batched evolution: too many rows in all");
     }
-
+    // instance-size key of the tuner's cache: the sorted (dimension, nnz) pairs, the storage and
+    // the device
+    {
+        std::vector<std::pair<int64_t, int64_t>> sizes(n_inst);
+        for (int i = 0; i < n_inst; i++) sizes[i] = {b->dim[i], H[i]->nnz};
+        std::sort(sizes.begin(), sizes.end());
+        uint64_t h = 1469598103934665603ull;
+        auto mix = [&](uint64_t v) {
+            for (int q = 0; q < 8; q++) {
+                h ^= (v >> (8 * q)) & 0xffu;
+                h *= 1099511628211ull;
+            }
+        };
+        for (auto& p : sizes) {
+            mix((uint64_t)p.first);
+            mix((uint64_t)p.second);
+        }
+        mix(b->sell ? 1u : 0u);
+        mix((uint64_t)ctx->device);
+        mix((uint64_t)ctx->num_cus);
+        b->shape_key = h;
+    }
     const size_t ns = (size_t)b->n_samples;
     TE_TRY(balloc(ctx, &b->inst, n_inst));
-    TE_TRY(balloc(ctx, &b->sunit, hsu.size()));
-    TE_TRY(balloc(ctx, &b->vunit, hvu.size()));
+    // unit tables sized for the smallest units: one group of slices (or kBatchRB row blocks) and
+    // one chunk
+    TE_TRY(balloc(ctx, &b->sunit, b->sell ? (size_t)b->total_groups : hsu.size()));
+    TE_TRY(balloc(ctx, &b->vunit, (size_t)b->total_chunks));
     TE_TRY(balloc(ctx, &b->U, (size_t)total * (b->m_max + 1)));
     TE_TRY(balloc(ctx, &b->psi0, (size_t)total));
     TE_TRY(balloc(ctx, &b->sampled, (size_t)total));
     TE_TRY(balloc(ctx, &b->tmp, (size_t)total));
     TE_TRY(balloc(ctx, &b->wcoef, (size_t)n_inst * kMaxKrylov));
     TE_TRY(balloc(ctx, &b->Y, (size_t)n_inst * ns * kMaxKrylov));
-    TE_TRY(balloc(ctx, &b->spart, hsu.size()));
-    TE_TRY(balloc(ctx, &b->vpart, hvu.size()));
+    TE_TRY(balloc(ctx, &b->spart, b->sell ? (size_t)b->total_groups : hsu.size()));
+    TE_TRY(balloc(ctx, &b->vpart, (size_t)b->total_chunks));
     TE_TRY(balloc(ctx, &b->ks, n_inst));
     TE_TRY(balloc(ctx, &b->st, n_inst));
     TE_TRY(balloc(ctx, &b->bc, n_inst));
     TE_TRY(balloc(ctx, &b->consts, n_inst));
     TE_TRY(balloc(ctx, &b->diag, (size_t)std::max(n_obs, 1) * total));
-    TE_TRY(balloc(ctx, &b->opart, hvu.size() * std::max(n_obs, 1) * ns));
+    TE_TRY(balloc(ctx, &b->opart, (size_t)b->total_chunks * std::max(n_obs, 1) * ns));
     TE_TRY(balloc(ctx, &b->obs, (size_t)n_inst * std::max(n_obs, 1) * ns));
     TE_TRY(balloc(ctx, &b->nrm, n_inst));
     TE_TRY(balloc(ctx, &b->trace, (size_t)n_inst * kBatchTraceCap));
@@ -766,8 +979,8 @@ extern "C" int te_batch_create(te_ctx* ctx, int n_inst, te_matrix* const* H, con
 
     hipStream_t s = ctx->stream;
     TE_TRYH(hipMemcpyAsync(b->inst, hinst.data(), sizeof(BatchInst) * n_inst, hipMemcpyHostToDevice, s));
-    TE_TRYH(hipMemcpyAsync(b->sunit, hsu.data(), sizeof(int4) * hsu.size(), hipMemcpyHostToDevice, s));
-    TE_TRYH(hipMemcpyAsync(b->vunit, hvu.data(), sizeof(int4) * hvu.size(), hipMemcpyHostToDevice, s));
+    if (!b->sell)
+        TE_TRYH(hipMemcpyAsync(b->sunit, hsu.data(), sizeof(int4) * hsu.size(), hipMemcpyHostToDevice, s));
     TE_TRYH(hipMemsetAsync(b->psi0, 0, sizeof(double2) * total, s));
     TE_TRYH(hipMemsetAsync(b->diag, 0, sizeof(double) * std::max(n_obs, 1) * total, s));
     std::vector<StepConsts> hc(n_inst);
@@ -814,7 +1027,6 @@ extern "C" int te_batch_create(te_ctx* ctx, int n_inst, te_matrix* const* H, con
     d.sunit = b->sunit;
     d.n_sunits = (int)hsu.size();
     d.vunit = b->vunit;
-    d.n_vunits = (int)hvu.size();
     d.U = b->U;
     d.ld = total;
     d.alpha = make_double2(0.0, -1.0 * params->exp_factor);                               // :31
@@ -835,12 +1047,8 @@ extern "C" int te_batch_create(te_ctx* ctx, int n_inst, te_matrix* const* H, con
     d.nrm = b->nrm;
     d.n_inst = n_inst;
     d.n_live = b->n_live;
-    auto grid = [&](int items, int blocks_per_cu) {
-        int g = std::min(items, ctx->num_cus * blocks_per_cu);
-        return std::max(8, (g + 7) / 8 * 8);
-    };
-    b->grid_s = grid(d.n_sunits, per_cu);
-    b->grid_v = grid(d.n_vunits, per_cu);
+    TE_TRY(apply_shape(b, b->unit_slices, b->vec_rows, b->spmv_bpc, b->vec_bpc));
+    TE_TRYH(hipStreamSynchronize(s));
     *out = b;
     return TE_OK;
 }
@@ -858,6 +1066,40 @@ extern "C" int te_batch_info(const te_batch* b, size_t* packed_rows, size_t* n_s
 extern "C" int te_batch_set_graph(te_batch* b, int enable) {
     if (!b) return TE_ERR_ARG;
     b->use_graph = enable != 0;
+    return TE_OK;
+}
+
+extern "C" int te_batch_set_shape(te_batch* b, int unit_slices, int vec_rows, int spmv_blocks_per_cu,
+                                  int vec_blocks_per_cu) {
+    using namespace te;
+    if (!b) return TE_ERR_ARG;
+    const int all = unit_slices | vec_rows | spmv_blocks_per_cu | vec_blocks_per_cu;
+    if (all == 0) {                      // back to the tuner
+        b->tune = true;
+        return TE_OK;
+    }
+    if (unit_slices < kBatchGroup || unit_slices % kBatchGroup != 0 || vec_rows < kBatchChunk ||
+        vec_rows % kBatchChunk != 0 || spmv_blocks_per_cu < 1 || vec_blocks_per_cu < 1 || spmv_blocks_per_cu > 64 ||
+        vec_blocks_per_cu > 64)
+        return fail(b->ctx, TE_ERR_ARG, "batched launch shape: unit_slices a positive multiple of %d, vec_rows a "
+                                        "positive multiple of %d, blocks per CU in [1, 64]", kBatchGroup, kBatchChunk);
+    hipSetDevice(b->ctx->device);
+    const int rc = apply_shape(b, unit_slices, vec_rows, spmv_blocks_per_cu, vec_blocks_per_cu);
+    if (rc != TE_OK) return rc;
+    b->tune = false;
+    b->shape_source = 3;
+    return TE_OK;
+}
+
+extern "C" int te_batch_shape(const te_batch* b, int* unit_slices, int* vec_rows, int* spmv_blocks_per_cu,
+                              int* vec_blocks_per_cu, int* source, double* tune_ms) {
+    if (!b) return TE_ERR_ARG;
+    if (unit_slices) *unit_slices = b->unit_slices;
+    if (vec_rows) *vec_rows = b->vec_rows;
+    if (spmv_blocks_per_cu) *spmv_blocks_per_cu = b->spmv_bpc;
+    if (vec_blocks_per_cu) *vec_blocks_per_cu = b->vec_bpc;
+    if (source) *source = b->shape_source;
+    if (tune_ms) *tune_ms = b->tune_ms;
     return TE_OK;
 }
 
@@ -924,6 +1166,29 @@ extern "C" int te_batch_run(te_batch* b, double* obs_values, double* const* stat
         TE_HIP(ctx, hipMemcpyAsync(b->Y + (size_t)i * ns * kMaxKrylov, e1.data() + (size_t)i * kMaxKrylov,
                                    sizeof(double2) * kMaxKrylov, hipMemcpyHostToDevice, s));
     TE_HIP(ctx, hipMemcpyAsync(b->U, b->psi0, sizeof(double2) * b->ld, hipMemcpyDeviceToDevice, s));
+    if (b->tune) {
+        // launch shape: the tuner's cached choice for these instance sizes, else measured now
+        bool cached = false;
+        BatchShape sh{};
+        {
+            std::lock_guard<std::mutex> lk(g_shape_mu);
+            auto it = g_shape_cache.find(b->shape_key);
+            if (it != g_shape_cache.end()) {
+                cached = true;
+                sh = it->second;
+            }
+        }
+        int rc = cached ? apply_shape(b, sh.unit_slices, sh.vec_rows, sh.spmv_bpc, sh.vec_bpc) : tune_shape(b);
+        if (rc != TE_OK) return rc;
+        if (cached) {
+            b->shape_source = 2;
+            b->tune_ms = 0.0;
+        }
+        b->tune = false;
+        // the tuner's iterations wrote the Krylov scalars: back to the run's start
+        TE_HIP(ctx, hipMemcpyAsync(b->ks, hks.data(), sizeof(KrylovScalars) * n_inst, hipMemcpyHostToDevice, s));
+        TE_HIP(ctx, hipStreamSynchronize(s));
+    }
     TE_HIP(ctx, hipMemsetAsync(b->obs, 0, sizeof(double) * n_inst * std::max(b->nobs, 1) * ns, s));
     batch_combine(b, s);
     TE_HIP(ctx, hipGetLastError());
