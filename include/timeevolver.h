@@ -79,6 +79,14 @@ int te_spmv(te_matrix* A, double alpha_re, double alpha_im, const double* in, do
 /* Same on device pointers (n_cols and n_rows complex128 each), asynchronous on the
  * context's stream. */
 int te_spmv_device(te_matrix* A, double alpha_re, double alpha_im, const void* in_dev, void* out_dev);
+/* No counterpart in the reference: te_spmv for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). Same arguments, kernels
+ * and results as te_spmv, but it allocates nothing: in and out are staged through a fixed
+ * device array of the library (2 x 2,097,152 complex128, 64 MiB, held on every device the
+ * library runs on until the process ends). Matrices with more than 2,097,152 rows or columns
+ * return TE_ERR_UNSUPPORTED. Every call on a device shares that device's array: calls that
+ * overlap on one device (two contexts or threads) race. */
+int te_spmv_noalloc(te_matrix* A, double alpha_re, double alpha_im, const double* in, double* out);
 
 /* ---- basis: replaces basis / tensorBasis (helper/Basis.h:277-357) -------------------- */
 /* basis(N, K, nbQudits, capacity, quant) (helper/Basis.cpp:37-120), built on the device. */

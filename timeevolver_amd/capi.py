@@ -70,6 +70,7 @@ _SIGS = {
     "te_matrix_to_csr": (_c.c_int, [_p, _dp, _i64p, _ip]),
     "te_spmv": (_c.c_int, [_p, _c.c_double, _c.c_double, _dp, _dp]),
     "te_spmv_device": (_c.c_int, [_p, _c.c_double, _c.c_double, _p, _p]),
+    "te_spmv_noalloc": (_c.c_int, [_p, _c.c_double, _c.c_double, _dp, _dp]),
     "te_basis_create": (_c.c_int, [_p, _c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.c_int, _pp]),
     "te_tensor_basis_create": (_c.c_int, [_p, _c.c_int, _c.c_int, _c.c_int, _c.c_int, _c.c_int, _pp]),
     "te_basis_destroy": (None, [_p]),
@@ -227,6 +228,16 @@ class Matrix:
         alpha = complex(alpha)
         self.ctx.check(self.ctx.L.te_spmv(self.h, alpha.real, alpha.imag, _ptr(xx.view(np.float64), _c.c_double),
                                           _ptr(y.view(np.float64), _c.c_double)), "te_spmv")
+        return y
+
+    def spmv_noalloc(self, x, alpha=1.0):
+        """te_spmv_noalloc: the product staged through the library's fixed device array."""
+        xx = np.ascontiguousarray(x, dtype=np.complex128)
+        y = np.zeros(self.n_rows, dtype=np.complex128)
+        alpha = complex(alpha)
+        self.ctx.check(self.ctx.L.te_spmv_noalloc(self.h, alpha.real, alpha.imag,
+                                                  _ptr(xx.view(np.float64), _c.c_double),
+                                                  _ptr(y.view(np.float64), _c.c_double)), "te_spmv_noalloc")
         return y
 
     def close(self):

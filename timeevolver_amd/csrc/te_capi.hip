@@ -11,7 +11,9 @@
 #include <cstdarg>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <numeric>
+#include <vector>
 
 #include "te_internal.h"
 #include "te_krylov.h"
@@ -253,6 +255,37 @@ extern "C" int te_version(void) { return 201; }
 
 extern "C" void te_set_log_level(int level) { te::g_log_level = level; }
 
+namespace te {
+// Staging of te_spmv_noalloc, which calls no allocator: x in the first kNoAllocRows entries, y in
+// the second (2 x 32 MiB). Every device the library runs on holds its own copy until the process
+// ends; calls on one device share it (and race when they overlap). Host code reaches it only
+// through the per-device address that noalloc_setup reads with hipGetSymbolAddress: the array's
+// name in host code is a stand-in, not a device address.
+constexpr int64_t kNoAllocRows = int64_t(1) << 21;
+static __device__ __align__(256) double2 scratch[2 * kNoAllocRows];
+static std::mutex g_scratch_mu;
+static std::vector<double2*> g_scratch_dev;     // per device: scratch's address there (null: not set up)
+
+// Once per device, with that device current (te_ctx_create): records scratch's address.
+static int noalloc_setup(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch_dev.size() <= device) g_scratch_dev.resize(device + 1, nullptr);
+    if (g_scratch_dev[device]) return TE_OK;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+        (void)hipGetLastError();
+        return TE_ERR_HIP;
+    }
+    g_scratch_dev[device] = static_cast<double2*>(p);
+    return TE_OK;
+}
+
+static double2* noalloc_scratch(int device) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return device < (int)g_scratch_dev.size() ? g_scratch_dev[device] : nullptr;
+}
+}  // namespace te
+
 extern "C" int te_ctx_create(int device, te_ctx** out) {
     if (!out) return TE_ERR_ARG;
     *out = nullptr;
@@ -274,6 +307,9 @@ extern "C" int te_ctx_create(int device, te_ctx** out) {
         delete c;
         return TE_ERR_HIP;
     }
+    // te_spmv_noalloc's staging on this device; without it that entry point reports an error
+    if (te::noalloc_setup(device) != TE_OK)
+        te::log_msg(2, "device %d: the staging array of te_spmv_noalloc has no device address", device);
     *out = c;
     return TE_OK;
 }
@@ -499,6 +535,26 @@ extern "C" int te_spmv(te_matrix* A, double alpha_re, double alpha_im, const dou
     TE_HIP(ctx, hipStreamSynchronize(ctx->stream));
     hipFree(x);
     hipFree(y);
+    return TE_OK;
+}
+
+extern "C" int te_spmv_noalloc(te_matrix* A, double alpha_re, double alpha_im, const double* in, double* out) {
+    using namespace te;
+    if (!A || !in || !out) return TE_ERR_ARG;
+    te_ctx* ctx = A->ctx;
+    if (A->n_rows > kNoAllocRows || A->n_cols > kNoAllocRows)
+        return fail(ctx, TE_ERR_UNSUPPORTED,
+                    "te_spmv_noalloc: %lld x %lld matrix exceeds the %lld rows and columns of its staging; use "
+                    "te_spmv", (long long)A->n_rows, (long long)A->n_cols, (long long)kNoAllocRows);
+    double2* x = noalloc_scratch(ctx->device);
+    if (!x) return fail(ctx, TE_ERR_HIP, "te_spmv_noalloc: no staging address on device %d", ctx->device);
+    double2* y = x + kNoAllocRows;
+    hipSetDevice(ctx->device);
+    TE_HIP(ctx, hipMemcpyAsync(x, in, sizeof(double2) * A->n_cols, hipMemcpyHostToDevice, ctx->stream));
+    launch_spmv(A, make_double2(alpha_re, alpha_im), x, y, ctx->stream);
+    TE_HIP(ctx, hipGetLastError());
+    TE_HIP(ctx, hipMemcpyAsync(out, y, sizeof(double2) * A->n_rows, hipMemcpyDeviceToHost, ctx->stream));
+    TE_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return TE_OK;
 }
 
