@@ -171,6 +171,12 @@ int te_evolver_num_samples(const te_evolver* ev, size_t* n_samples);
  * sampled state (krylovReturn::evolvedState, :665; may be NULL); result may not be NULL.
  * Returns TE_OK also for soft failures reported through result->status_code. */
 int te_evolver_run(te_evolver* ev, double* obs_values, double* evolved_state, te_result* result);
+/* krylovBasicObservable::sampleIndex (core/krylovObservables.h:60) of each observable after the
+ * last run (counts: n_obs entries): the run's numSamples, plus one for the observables listed
+ * before a host observable that requested a stop, which sample() had already evaluated at the
+ * stop sample (core/krylovTimeEvolver.cpp:120-126). Samples an observable did not record read 0
+ * in te_evolver_run's obs_values. */
+int te_evolver_obs_recorded(const te_evolver* ev, size_t* counts);
 /* Device pointer to the last sampled state (valid until the next run/destroy). */
 int te_evolver_state_device(te_evolver* ev, const void** state_dev);
 /* The Krylov step sizes of the last run (up to cap entries; returns the count in *n). */

@@ -71,7 +71,7 @@ def lib():
             "or_gauss_poly": (_d, [_i, _i, _dp, _d, _d]),
             "or_tanh_sinh_rows": (_i, [_i]),
             "or_evolve_csr": (_i, [_l, _lp, _ip, _dp, _dp, _d, _d, _d, _d, _i, _i, _l, _i, _dp, _dp, _dp, _dp]),
-            "or_evolve": (_i, [_l, _l, _dp, _lp, _lp, _dp, _d, _d, _d, _d, _i, _i, _l, _l, _i, _ip, _lp,
+            "or_evolve": (_i, [_l, _l, _dp, _lp, _lp, _dp, _d, _d, _d, _d, _i, _i, _l, _l, _l, _i, _ip, _lp,
                                _dp, _lp, _lp, _dp, _dp, _dp, _dp, _dp, _l]),
             "or_mkl_enable": (_i, [ctypes.c_char_p, _i]),
             "or_mkl_version": (None, [ctypes.c_char_p, _i]),
@@ -267,9 +267,11 @@ def gauss_poly(n, coeffs, a, b):
 
 
 def evolve(n, H, psi0, t, samplingStep, observables=(), expFactor=1.0, tol=1e-6, m=40, fastIntegration=False,
-           max_steps=0, stop_at_sample=-1, trace_cap=100000):
+           max_steps=0, stop_at_sample=-1, stop_obs=0, trace_cap=100000):
     """krylovTimeEvolver(t, psi0, samplingStep, observables, H, expFactor, tol, m, fastIntegration)
-    followed by timeEvolve().
+    followed by timeEvolve(). stop_at_sample >= 0: a host observable placed before observable
+    `stop_obs` of the list throws requestStopException at that sample (the observables before it
+    record the sample, the run returns with status 3).
 
     H: (values complex128, cols, rows) COO. observables: list of ("sparse", (vals, cols, rows)) or
     ("vector", ndarray). Returns dict with state, observables (n_obs x n_samples), err, n_steps,
@@ -306,7 +308,8 @@ def evolve(n, H, psi0, t, samplingStep, observables=(), expFactor=1.0, tol=1e-6,
     trace = np.zeros(trace_cap)
     rc = lib().or_evolve(n, len(hc), _ptr(hv, ctypes.c_double), _ptr(hc, ctypes.c_long), _ptr(hr, ctypes.c_long),
                          _ptr(psi.view(np.float64), ctypes.c_double), t, samplingStep, expFactor, tol, m,
-                         int(fastIntegration), max_steps, stop_at_sample, len(kinds), _ptr(kinds_, ctypes.c_int),
+                         int(fastIntegration), max_steps, stop_at_sample, stop_obs, len(kinds),
+                         _ptr(kinds_, ctypes.c_int),
                          _ptr(nnzs_, ctypes.c_long), _ptr(ov_, ctypes.c_double), _ptr(oc_, ctypes.c_long),
                          _ptr(or_, ctypes.c_long), _ptr(vecs_, ctypes.c_double), _ptr(obs_out, ctypes.c_double),
                          _ptr(state.view(np.float64), ctypes.c_double), _ptr(result, ctypes.c_double),

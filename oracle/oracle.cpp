@@ -778,6 +778,8 @@ struct Params {
     int m, fastIntegration;
     long max_steps;        // 0 = unlimited (CPU-baseline timing hook)
     long stop_at_sample;   // -1 = never; emulates an observable throwing requestStopException
+    long stop_obs;         // ... placed before observable stop_obs of the list (the ones before it
+                           // record the stop sample, sample() evaluates them in order)
 };
 
 struct Result {
@@ -806,8 +808,14 @@ struct Evolver {
     // krylovSpMatrixObservable::expectation (krylovObservables.cpp:173-194) stores Re<v,Ov>;
     // krylovVectorObservable::expectation (:216-236) stores |<v,o>|^2.
     void sample() {
-        if (P.stop_at_sample >= 0 && index_samples == P.stop_at_sample) { stop_requested = true; return; }
-        for (auto& o : *obs) {
+        const bool stop_here = P.stop_at_sample >= 0 && index_samples == P.stop_at_sample;
+        for (size_t i = 0; i <= obs->size(); i++) {
+            if (stop_here && (long)i == std::min<long>(P.stop_obs, (long)obs->size())) {
+                stop_requested = true;
+                return;
+            }
+            if (i == obs->size()) break;
+            Observable& o = (*obs)[i];
             if (o.kind == 0) {
                 spmv(o.A, cd(1.0, 0.0), sampledState.data(), tmp.data());
                 cd v = zdotc(Hsize, sampledState.data(), tmp.data());
@@ -1162,7 +1170,9 @@ static Csr diag_csr(long n, const double* d) {
     return A;
 }
 
-// Full time evolution. H and sparse observables given as COO (int64 indices).
+// Full time evolution. H and sparse observables given as COO (int64 indices). stop_at_sample
+// >= 0 emulates a host observable throwing requestStopException at that sample, placed before
+// observable stop_obs of the list.
 // obs_kind[i]: 0 sparse (obs_nnz[i] entries taken in turn from obs_vals/cols/rows),
 // 1 vector (D complex entries taken in turn from obs_vecs).
 // obs_out: n_obs x n_samples. result: [err, evolvedTime, numSamples, n_steps, dim,
@@ -1170,12 +1180,12 @@ static Csr diag_csr(long n, const double* d) {
 // Krylov step, up to trace_cap entries. Returns 0, or -1 where the reference would exit(1).
 int or_evolve(long n, long nnz, const double* hv, const long* hc, const long* hr, const double* psi0,
               double t, double samplingStep, double expFactor, double tol, int m, int fastIntegration,
-              long max_steps, long stop_at_sample, int n_obs, const int* obs_kind, const long* obs_nnz,
+              long max_steps, long stop_at_sample, long stop_obs, int n_obs, const int* obs_kind, const long* obs_nnz,
               const double* obs_vals, const long* obs_cols, const long* obs_rows, const double* obs_vecs,
               double* obs_out, double* state_out, double* result, double* trace, long trace_cap) {
     Csr H = coo_to_csr(n, n, nnz, hv, hc, hr);
     Evolver ev;
-    ev.P = Params{t, samplingStep, expFactor, tol, m, fastIntegration, max_steps, stop_at_sample};
+    ev.P = Params{t, samplingStep, expFactor, tol, m, fastIntegration, max_steps, stop_at_sample, stop_obs};
     ev.H = &H;
     ev.Hsize = n;
     ev.matrixNorm = norm1_coo(n, nnz, hv, hc);
@@ -1242,7 +1252,7 @@ int or_evolve_csr(long n, const long* ptr, const int* col, const double* vals, c
     std::vector<double> colVal(n, 0.0);
     for (long k = 0; k < nnz; k++) colVal[H.col[k]] += std::abs(H.val[k]);
     Evolver ev;
-    ev.P = Params{t, samplingStep, expFactor, tol, m, fastIntegration, max_steps, -1};
+    ev.P = Params{t, samplingStep, expFactor, tol, m, fastIntegration, max_steps, -1, 0};
     ev.H = &H;
     ev.Hsize = n;
     ev.matrixNorm = *std::max_element(colVal.begin(), colVal.end());

@@ -620,6 +620,43 @@ def test_stop_request(gpu, oracle_lib):
     assert np.linalg.norm(r["state"] - ro["state"]) <= r["err"] + ro["err"] + 1e-10
 
 
+@pytest.mark.parametrize("at", [0, 5])
+def test_stop_request_records_earlier_observables(gpu, oracle_lib, at):
+    """sample() evaluates the observables in list order (core/krylovTimeEvolver.cpp:120-126), so
+    at the stop sample the ones listed before the stopping observable have recorded a value and
+    the later ones have not: a diagonal and a vector observable before a host stopper, a diagonal
+    and a vector observable after it. Values and per-observable sample counts (sampleIndex)
+    against the oracle with the stopper at the same place (at sample 0 and sample 5)."""
+    L = N = 5
+    ob = oracle_lib.Basis.basis(N, L)
+    Hc = oracle_lib.build_operator(ob, as_tuples(_ring_terms(L)))
+    n0 = oracle_lib.build_operator(ob, as_tuples([createNumberOperator(0, 1)]))
+    n2 = oracle_lib.build_operator(ob, as_tuples([createNumberOperator(2, 1)]))
+    p1, p2 = random_state(ob.D, 31), random_state(ob.D, 32)
+    psi = random_state(ob.D, 33)
+    H = capi.Matrix.from_coo(gpu, *Hc, ob.D)
+    N0, N2 = capi.Matrix.from_coo(gpu, *n0, ob.D), capi.Matrix.from_coo(gpu, *n2, ob.D)
+    calls = {"n": 0}
+
+    def stopper(st):
+        calls["n"] += 1
+        if calls["n"] == at + 1:
+            raise StopIteration
+        return 0.0
+
+    r = capi.Evolver(gpu, H, psi, 2.0, 0.1, [N0, ("vector", p1), ("host", stopper), N2, ("vector", p2)],
+                     tol=1e-8).run()
+    ro = oracle_lib.evolve(ob.D, Hc, psi, 2.0, 0.1, [("sparse", n0), ("vector", p1), ("sparse", n2), ("vector", p2)],
+                           tol=1e-8, stop_at_sample=at, stop_obs=2)
+    assert r["statusCode"] == ro["statusCode"] == (3 if at else 0)
+    assert r["numSamples"] == ro["numSamples"] == at
+    assert r["recorded"] == [at + 1, at + 1, at, at, at]
+    for k, ko in ((0, 0), (1, 1), (3, 2), (4, 3)):
+        assert np.abs(r["observables"][k] - ro["observables"][ko]).max() <= 1e-8
+    assert np.all(r["observables"][3:, at:] == 0) and np.all(r["observables"][:2, at + 1:] == 0)
+    assert np.linalg.norm(r["state"] - ro["state"]) <= r["err"] + ro["err"] + 1e-10
+
+
 def test_fatal_unnormalised(gpu, oracle_lib):
     L = N = 3
     ob = oracle_lib.Basis.basis(N, L)

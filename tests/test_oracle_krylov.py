@@ -142,6 +142,16 @@ def test_status_stop_request_and_fatal(oracle_lib):
     assert r0["statusCode"] == 0 and r0["numSamples"] == 0 and r0["n_steps"] == 0
     with pytest.raises(RuntimeError):      # unnormalised initial vector: FATAL + exit(1)
         oracle_lib.evolve(b.D, H, 2 * fock_state(b.D), 1.0, 0.1, [], tol=1e-8)
+    # sample() evaluates the observables in list order (core/krylovTimeEvolver.cpp:120-126): with
+    # the stopping observable between n_0 and n_1, n_0 records the stop sample and n_1 does not
+    obs = [("sparse", oracle_lib.build_operator(b, as_tuples([createNumberOperator(k, 1)]))) for k in (0, 1)]
+    full = oracle_lib.evolve(b.D, H, fock_state(b.D), 2.0, 0.1, obs, tol=1e-8)
+    for at in (0, 5):
+        r = oracle_lib.evolve(b.D, H, fock_state(b.D), 2.0, 0.1, obs, tol=1e-8, stop_at_sample=at, stop_obs=1)
+        assert r["numSamples"] == at and r["statusCode"] == (3 if at else 0)
+        assert np.array_equal(r["observables"][0, :at + 1], full["observables"][0, :at + 1])
+        assert np.array_equal(r["observables"][1, :at], full["observables"][1, :at])
+        assert np.all(r["observables"][1, at:] == 0) and np.all(r["observables"][0, at + 1:] == 0)
 
 
 def test_sampling_arithmetic(oracle_lib):

@@ -86,6 +86,7 @@ _SIGS = {
     "te_evolver_num_samples": (_c.c_int, [_p, _szp]),
     "te_evolver_run": (_c.c_int, [_p, _dp, _dp, _c.POINTER(te_result)]),
     "te_evolver_state_device": (_c.c_int, [_p, _pp]),
+    "te_evolver_obs_recorded": (_c.c_int, [_p, _szp]),
     "te_evolver_step_trace": (_c.c_int, [_p, _dp, _c.c_size_t, _szp]),
     "te_evolver_set_profiling": (_c.c_int, [_p, _c.c_int]),
     "te_evolver_kernel_time": (_c.c_int, [_p, _c.c_int, _dp, _szp]),
@@ -379,9 +380,12 @@ class Evolver:
         self.ctx.L.te_evolver_step_trace(self.h, None, 0, _c.byref(nt))
         trace = np.zeros(max(nt.value, 1))
         self.ctx.L.te_evolver_step_trace(self.h, _ptr(trace, _c.c_double), nt.value, _c.byref(nt))
+        rec = (_c.c_size_t * max(self.n_obs, 1))()
+        self.ctx.L.te_evolver_obs_recorded(self.h, rec)
         return {
             "state": state,
             "observables": obs[:self.n_obs * self.n_samples].reshape(self.n_obs, self.n_samples),
+            "recorded": [rec[i] for i in range(self.n_obs)],
             "err": R.err, "evolvedTime": R.evolved_time, "numSamples": R.num_samples, "n_steps": R.n_steps,
             "dim": R.dim, "krylovDim": R.krylov_dim, "statusCode": R.status_code, "n_samples": self.n_samples,
             "trace": trace[:nt.value],

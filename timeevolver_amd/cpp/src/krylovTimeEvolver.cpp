@@ -100,13 +100,17 @@ krylovReturn* krylovTimeEvolver::timeEvolve() {
     krylovReturn* ret = new krylovReturn((unsigned int)Hsize, 0);
     te_result R;
     rc = te_evolver_run(ev, values.data(), reinterpret_cast<double*>(ret->evolvedState), &R);
+    std::vector<size_t> recorded(obs.size(), 0);
+    if (rc == TE_OK && !obs.empty()) rc = te_evolver_obs_recorded(ev, recorded.data());
     te_evolver_destroy(ev);
     tecpp::check(rc, "timeEvolve");
+    // values and sampleIndex of the device-evaluated observables: the samples each recorded
+    // (observables listed before one that requested a stop also hold the stop sample)
     for (size_t i = 0; i < obsVector.size(); i++) {
         if (!on_device[i]) continue;
         krylovBasicObservable* o = obsVector[i].get();
-        for (size_t s = 0; s < R.num_samples && s < o->numSamples; s++) o->expectationValues[s] = values[i * n_samples + s];
-        o->sampleIndex = R.num_samples;
+        for (size_t s = 0; s < recorded[i] && s < o->numSamples; s++) o->expectationValues[s] = values[i * n_samples + s];
+        o->sampleIndex = recorded[i];
     }
     if (progressBar) std::printf("[%s] 100 %%\n", std::string(70, '|').c_str());
     ret->statusCode = R.status_code;

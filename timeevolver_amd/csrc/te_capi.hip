@@ -584,6 +584,8 @@ struct te_evolver {
     double* diag_vals = nullptr;    // |diag_obs| x n_samples
     std::vector<double2*> vec_dev;  // per observable (vector kind)
     std::vector<double> host_vals;  // n_obs x n_samples, host-callback observables
+    int stop_obs = -1;              // observable whose host callback requested the stop (last run)
+    std::vector<size_t> recorded;   // per observable: samples recorded in the last run
     std::vector<double2> host_state;
     DevQuad quad;
     ReduceScratch rs;
@@ -845,7 +847,10 @@ static int eval_generic(te_evolver* ev, long long sample) {
                 host_copied = true;
             }
             double v = 0.0;
-            if (o.callback(o.user, (const double*)ev->host_state.data(), (size_t)ev->n, &v) != 0) return TE_ERR_STOP;
+            if (o.callback(o.user, (const double*)ev->host_state.data(), (size_t)ev->n, &v) != 0) {
+                ev->stop_obs = i;
+                return TE_ERR_STOP;
+            }
             ev->host_vals[(size_t)i * ev->n_samples + sample] = v;
         }
     }
@@ -961,6 +966,7 @@ extern "C" int te_evolver_run(te_evolver* ev, double* obs_values, double* evolve
     const int n_obs = (int)ev->obs.size();
     const double2 alpha = make_double2(0.0, -1.0 * ev->P.exp_factor);                   // :31
     std::fill(ev->host_vals.begin(), ev->host_vals.end(), 0.0);
+    ev->stop_obs = -1;
     ev->trace_host.clear();
     for (int k = 0; k < 4; k++) ev->prof_used[k] = 0;
 
@@ -989,8 +995,11 @@ extern "C" int te_evolver_run(te_evolver* ev, double* obs_values, double* evolve
         TE_HIP(ctx, hipMemcpyAsync(ev->tmp, ev->psi0, sizeof(double2) * n, hipMemcpyDeviceToDevice, s));
         int rc = eval_generic(ev, 0);
         if (rc == TE_ERR_STOP) {
-            stopped = true;                 // status stays 0, no sample recorded (:309-313)
+            // status stays 0 and no sample is counted (:309-313); the observables listed before
+            // the one that stopped have recorded sample 0, as sample() evaluates them in order (:120-126)
+            stopped = true;
             R.num_samples = 0;
+            combine_step(ev, 1, 0, 1, false, false);
             TE_HIP(ctx, hipMemcpyAsync(ev->sampled, ev->psi0, sizeof(double2) * n, hipMemcpyDeviceToDevice, s));
         } else if (rc != TE_OK) {
             return rc;
@@ -1056,7 +1065,9 @@ extern "C" int te_evolver_run(te_evolver* ev, double* obs_values, double* evolve
             materialise(ev, m, sample);
             const int rc = eval_generic(ev, sample);
             if (rc == TE_ERR_STOP) {                                                    // :425-436
-                combine_step(ev, m, h.sample_base, si, false, false);
+                // the diagonal observables of the step's samples up to and including this one (those
+                // listed after the stopping observable drop this sample below)
+                combine_step(ev, m, h.sample_base, si + 1, false, false);
                 TE_HIP(ctx, hipMemcpyAsync(ev->sampled, ev->tmp, sizeof(double2) * n, hipMemcpyDeviceToDevice, s));
                 stopped = true;
                 statusCode = 3;
@@ -1091,6 +1102,11 @@ extern "C" int te_evolver_run(te_evolver* ev, double* obs_values, double* evolve
     }
     R.status_code = statusCode;
 
+    // samples each observable recorded: all of the run's, and after a stop request the stop
+    // sample as well for the observables listed before the one that requested it
+    ev->recorded.assign((size_t)n_obs, R.num_samples);
+    if (stopped && ev->stop_obs >= 0)
+        for (int i = 0; i < ev->stop_obs; i++) ev->recorded[i] = R.num_samples + 1;
     // trace and outputs
     const size_t nt = (size_t)std::min<long long>(h.n_steps, ev->trace_cap);
     ev->trace_host.resize(nt);
@@ -1117,7 +1133,16 @@ extern "C" int te_evolver_run(te_evolver* ev, double* obs_values, double* evolve
             if (ev->obs[i].kind == TE_OBS_HOST)
                 std::memcpy(obs_values + (size_t)i * ev->n_samples, ev->host_vals.data() + (size_t)i * ev->n_samples,
                             sizeof(double) * ev->n_samples);
+        // a sample an observable did not record reads 0
+        for (int i = 0; i < n_obs; i++)
+            for (size_t k = ev->recorded[i]; k < (size_t)ev->n_samples; k++) obs_values[(size_t)i * ev->n_samples + k] = 0.0;
     *result = R;
+    return TE_OK;
+}
+
+extern "C" int te_evolver_obs_recorded(const te_evolver* ev, size_t* counts) {
+    if (!ev || !counts) return TE_ERR_ARG;
+    for (size_t i = 0; i < ev->obs.size(); i++) counts[i] = i < ev->recorded.size() ? ev->recorded[i] : 0;
     return TE_OK;
 }
 
