@@ -226,6 +226,7 @@ def bench_batched(np, capi, ctx, args):
     ctx.synchronize()
     build_s = time.perf_counter() - t0
     dims = [H.n_rows for H in Hs]
+    mat_bytes = [H.storage()[2] for H in Hs]          # matrix stream of one product, per instance
     log("batched: %d instances built in %.2f s, sum D = %d, sum nnz = %d" %
         (len(Hs), build_s, sum(dims), sum(H.nnz for H in Hs)))
     be = capi.BatchEvolver(ctx, Hs, psis, T, step, obs, tol=tol, m=m)
@@ -265,6 +266,10 @@ def bench_batched(np, capi, ctx, args):
     steps_plain = sum(r["n_steps"] for r in rp)
     log("plain, one instance after another: %.3f s, %d Krylov steps" % (t_plain, steps_plain))
     same = all(a["n_steps"] == b["n_steps"] and a["statusCode"] == b["statusCode"] for a, b in zip(rb, rp))
+    # algorithmic bytes of a run's Lanczos passes: per instance and Krylov step, krylovDim applies
+    # (matrix stream + 48 D) and projections (48 D), as in the single-instance kernels
+    b_apply = sum(r["n_steps"] * r["krylovDim"] * (mb + 48 * d) for r, mb, d in zip(rb, mat_bytes, dims))
+    b_proj = sum(r["n_steps"] * r["krylovDim"] * 48 * d for r, d in zip(rb, dims))
     dobs = max(float(np.abs(a["observables"] - b["observables"]).max()) for a, b in zip(rb, rp))
     for ev in evs:
         ev.close()
@@ -280,6 +285,11 @@ def bench_batched(np, capi, ctx, args):
             "rounds": rounds, "same_steps_and_status_as_plain": same, "max_obs_diff_vs_plain": dobs,
             "build_s": round(build_s, 2),
             "launch_shape": shape,
+            "apply_GBps": round(b_apply / (kt["apply"][0] / 1e3) / 1e9, 1),
+            "project_GBps": round(b_proj / (kt["project"][0] / 1e3) / 1e9, 1),
+            "pass_bytes_note": "algorithmic bytes of the run's applies (matrix stream + 48 D per instance and "
+                               "iteration) and projections (48 D) over their event-timed totals (finish launches "
+                               "included); an instance that broke down moved fewer",
             "kernel_ms_per_run": {k: round(v[0], 3) for k, v in kt.items()},
             "kernel_launches_per_run": {k: v[1] for k, v in kt.items()},
             "profiled_run_s": round(t_prof, 4),
