@@ -146,9 +146,10 @@ def main():
     # of a step, which stores nothing) and combine pass (the basis columns read, currentVec and
     # the sampled state written, the observables' diagonals) over the wall time
     n_steps_run = r["n_steps"]
+    diag_row = ev.diag_bytes_per_row()              # 14 one-byte observable indices (16 B a row) or 8 L
     b_run = (kernel_launches.get("apply", 0) * b_apply
              + kernel_launches.get("project", 0) * 48 * D - n_steps_run * 16 * D
-             + kernel_launches.get("combine", 0) * (16 * args.m * D + 32 * D + 8 * L * D))
+             + kernel_launches.get("combine", 0) * (16 * args.m * D + 32 * D + diag_row * D))
 
     line = {
         "metric": METRIC,
@@ -186,7 +187,8 @@ def main():
         "run_roofline": {"bytes": int(b_run), "achieved_GBps": round(b_run / time_to_T / 1e9, 1),
                          "frac": round(b_run / time_to_T / 1e9 / PEAK_HBM_GBS, 4),
                          "basis": "algorithmic bytes of a run's applies, projections and combine passes "
-                                  "over time-to-T"},
+                                  "over time-to-T",
+                         "observable_bytes_per_row": diag_row},
         "roofline": {"kernel": kname + " (fused SpMV + axpy + dot)", "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": measured_traffic(kname, D, b_apply),

@@ -177,6 +177,12 @@ int te_evolver_run(te_evolver* ev, double* obs_values, double* evolved_state, te
  * stop sample (core/krylovTimeEvolver.cpp:120-126). Samples an observable did not record read 0
  * in te_evolver_run's obs_values. */
 int te_evolver_obs_recorded(const te_evolver* ev, size_t* counts);
+/* For measurement harnesses (no reference counterpart): bytes per row of diagonal-observable
+ * data each combine pass reads. Diagonal observables with at most 256 distinct values (all of
+ * them, else none) are held as one-byte indices into tables of those values, 16 bytes per row
+ * per 16 observables (TE_DIAG_BYTES=0 in the environment keeps 8-byte diagonals); the values
+ * used are the same doubles. */
+int te_evolver_diag_bytes(const te_evolver* ev, size_t* bytes_per_row);
 /* Device pointer to the last sampled state (valid until the next run/destroy). */
 int te_evolver_state_device(te_evolver* ev, const void** state_dev);
 /* The Krylov step sizes of the last run (up to cap entries; returns the count in *n). */

@@ -657,6 +657,41 @@ def test_stop_request_records_earlier_observables(gpu, oracle_lib, at):
     assert np.linalg.norm(r["state"] - ro["state"]) <= r["err"] + ro["err"] + 1e-10
 
 
+@pytest.mark.parametrize("n_distinct", [2, 256, 257, 5000])
+def test_diagonal_observables_byte_tables(gpu, oracle_lib, monkeypatch, n_distinct):
+    """Diagonal observables with at most 256 distinct values are read by the combine as one-byte
+    indices into a table of those values (the same doubles): the observables are bit for bit
+    those of the 8-byte diagonals (TE_DIAG_BYTES=0), with 256 values and with more (257, 5000:
+    every diagonal stays 8 bytes when one has too many), next to the n_k observables; and within
+    tol of the oracle."""
+    L = N = 8
+    ob = oracle_lib.Basis.basis(N, L)
+    Hc = oracle_lib.build_operator(ob, as_tuples(_ring_terms(L)))
+    rng = np.random.default_rng(n_distinct)
+    vals = rng.normal(size=n_distinct)
+    d = vals[rng.integers(0, n_distinct, ob.D)]
+    d[:n_distinct] = vals                                   # every value present
+    idx = np.arange(ob.D, dtype=np.int64)
+    dcoo = (d.astype(np.complex128), idx, idx)
+    nks = [oracle_lib.build_operator(ob, as_tuples([createNumberOperator(k, 1)])) for k in range(L)]
+    psi = random_state(ob.D, 41)
+    res = {}
+    for mode in ("bytes", "doubles"):
+        if mode == "doubles":
+            monkeypatch.setenv("TE_DIAG_BYTES", "0")
+        gb = capi.Basis.basis(gpu, N, L)
+        H = capi.operator_build(gpu, gb, as_tuples(_ring_terms(L)))
+        obs = [capi.operator_build(gpu, gb, [createNumberOperator(k, 1)]) for k in range(L)]
+        obs.append(capi.Matrix.from_coo(gpu, *dcoo, ob.D))
+        res[mode] = capi.Evolver(gpu, H, psi, 2.0, 0.25, obs, tol=1e-8).run()
+        monkeypatch.delenv("TE_DIAG_BYTES", raising=False)
+    a, b = res["bytes"], res["doubles"]
+    assert np.array_equal(a["observables"].view(np.uint64), b["observables"].view(np.uint64))
+    assert np.array_equal(a["state"].view(np.uint64), b["state"].view(np.uint64))
+    ro = oracle_lib.evolve(ob.D, Hc, psi, 2.0, 0.25, [("sparse", o) for o in nks] + [("sparse", dcoo)], tol=1e-8)
+    assert np.abs(a["observables"] - ro["observables"]).max() <= 1e-8
+
+
 def test_fatal_unnormalised(gpu, oracle_lib):
     L = N = 3
     ob = oracle_lib.Basis.basis(N, L)

@@ -87,6 +87,7 @@ _SIGS = {
     "te_evolver_run": (_c.c_int, [_p, _dp, _dp, _c.POINTER(te_result)]),
     "te_evolver_state_device": (_c.c_int, [_p, _pp]),
     "te_evolver_obs_recorded": (_c.c_int, [_p, _szp]),
+    "te_evolver_diag_bytes": (_c.c_int, [_p, _szp]),
     "te_evolver_step_trace": (_c.c_int, [_p, _dp, _c.c_size_t, _szp]),
     "te_evolver_set_profiling": (_c.c_int, [_p, _c.c_int]),
     "te_evolver_kernel_time": (_c.c_int, [_p, _c.c_int, _dp, _szp]),
@@ -412,6 +413,12 @@ class Evolver:
             self.ctx.check(self.ctx.L.te_evolver_kernel_time(self.h, k, _c.byref(ms), _c.byref(n)), "kernel_time")
             out[name] = (ms.value, n.value)
         return out
+
+    def diag_bytes_per_row(self):
+        """Bytes of diagonal-observable data a combine pass reads per row."""
+        b = _c.c_size_t()
+        self.ctx.check(self.ctx.L.te_evolver_diag_bytes(self.h, _c.byref(b)), "te_evolver_diag_bytes")
+        return b.value
 
     def state_device_ptr(self):
         p = _p()

@@ -580,7 +580,10 @@ struct te_evolver {
     double* obs_dev = nullptr;      // n_obs x n_samples
     std::vector<te_observable> obs;
     std::vector<int> diag_obs, generic_obs;
-    double* diag_all = nullptr;     // |diag_obs| x n
+    double* diag_all = nullptr;     // |diag_obs| x n (freed once the byte form below exists)
+    unsigned char* diag_idx = nullptr;  // n x idx_stride: row r's indices into diag_tab
+    int idx_stride = 0;             // bytes per row: |diag_obs| rounded up to 16
+    double* diag_tab = nullptr;     // |diag_obs| x kDiagTab distinct values
     double* diag_vals = nullptr;    // |diag_obs| x n_samples
     std::vector<double2*> vec_dev;  // per observable (vector kind)
     std::vector<double> host_vals;  // n_obs x n_samples, host-callback observables
@@ -623,6 +626,8 @@ static void evolver_free(te_evolver* ev) {
     if (ev->trace) hipFree(ev->trace);
     if (ev->obs_dev) hipFree(ev->obs_dev);
     if (ev->diag_all) hipFree(ev->diag_all);
+    if (ev->diag_idx) hipFree(ev->diag_idx);
+    if (ev->diag_tab) hipFree(ev->diag_tab);
     if (ev->diag_vals) hipFree(ev->diag_vals);
     free_quadrature(ev->quad);
     for (double2* p : ev->vec_dev)
@@ -670,6 +675,137 @@ static int dmalloc(te_ctx* ctx, T** p, size_t count) {
             return rc_;              \
         }                            \
     } while (0)
+
+// ---- diagonal observables as one-byte indices into tables of their distinct values ----
+// The combine reads every diagonal observable once a row on every pass over the basis; most
+// of them (occupation numbers, projectors, small-integer functions of the state) take few
+// distinct values, so a byte per row and a table of at most kDiagTab values replace the 8-byte
+// diagonal. The values the combine multiplies are the same doubles, bit for bit.
+constexpr int kDiagSlots = 1024;                 // open-addressing slots per observable
+constexpr unsigned long long kDiagEmpty = ~0ull;  // slot key of an empty slot
+
+__device__ __forceinline__ unsigned diag_hash(unsigned long long k) {
+    return (unsigned)((k * 0x9E3779B97F4A7C15ull) >> 54) & (kDiagSlots - 1);
+}
+
+// Inserts every value's bit pattern into its observable's slots; count[o] = distinct values,
+// flag[0] = 1 when a table overflows (or a value has the empty slot's bit pattern). A slot is
+// read before any atomic: after the first few rows every value is found without one.
+__global__ void diag_insert_kernel(const double* __restrict__ d, int nd, int64_t n, unsigned long long* keys,
+                                   int* count, int* flag) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (int64_t)nd * n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int o = (int)(i / n);
+        const unsigned long long k = (unsigned long long)__double_as_longlong(d[i]);
+        if (k == kDiagEmpty) {
+            *flag = 1;
+            continue;
+        }
+        unsigned long long* T = keys + (size_t)o * kDiagSlots;
+        unsigned h = diag_hash(k);
+        for (int probe = 0;; probe++) {
+            if (probe == kDiagSlots) {
+                *flag = 1;
+                break;
+            }
+            unsigned long long cur = __atomic_load_n(&T[h], __ATOMIC_RELAXED);
+            if (cur == k) break;
+            if (cur == kDiagEmpty) {
+                cur = atomicCAS(&T[h], kDiagEmpty, k);
+                if (cur == kDiagEmpty) {
+                    if (atomicAdd(&count[o], 1) >= kDiagTab) *flag = 1;
+                    break;
+                }
+                if (cur == k) break;
+            }
+            h = (h + 1) & (kDiagSlots - 1);
+        }
+    }
+}
+
+// idx[r * stride + o] = the table index of value d[o * n + r] (slot_index: the index of each
+// occupied slot); the padding bytes of a row stay 0.
+__global__ void diag_index_kernel(const double* __restrict__ d, int nd, int64_t n, const unsigned long long* keys,
+                                  const unsigned char* __restrict__ slot_index, unsigned char* idx, int stride) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (int64_t)nd * n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int o = (int)(i / n);
+        const unsigned long long k = (unsigned long long)__double_as_longlong(d[i]);
+        const unsigned long long* T = keys + (size_t)o * kDiagSlots;
+        unsigned h = diag_hash(k);
+        // present (diag_insert_kernel inserted every value without overflow); the probe count
+        // is bounded all the same
+        for (int probe = 0; probe < kDiagSlots && T[h] != k; probe++) h = (h + 1) & (kDiagSlots - 1);
+        idx[(size_t)(i - (int64_t)o * n) * stride + o] = slot_index[(size_t)o * kDiagSlots + h];
+    }
+}
+
+// Builds ev->diag_idx / ev->diag_tab from ev->diag_all when every diagonal observable has at
+// most kDiagTab distinct values, and frees ev->diag_all; otherwise leaves the doubles in place.
+static int compress_diagonals(te_evolver* ev) {
+    te_ctx* ctx = ev->ctx;
+    hipStream_t s = ctx->stream;
+    const int nd = (int)ev->diag_obs.size();
+    const int64_t n = ev->n;
+    if (nd == 0 || !ev->diag_all) return TE_OK;
+    const char* e = std::getenv("TE_DIAG_BYTES");                   // 0: keep the 8-byte diagonals
+    if (e && e[0] == '0') return TE_OK;
+    unsigned long long* keys = nullptr;
+    int* cnt = nullptr;                  // nd counts, then the overflow flag
+    unsigned char* slot_index = nullptr;
+    TE_HIP(ctx, hipMalloc(&keys, sizeof(unsigned long long) * kDiagSlots * nd));
+    auto done = [&](int r) {
+        if (keys) hipFree(keys);
+        if (cnt) hipFree(cnt);
+        if (slot_index) hipFree(slot_index);
+        return r;
+    };
+    if (hipMalloc(&cnt, sizeof(int) * (nd + 1)) != hipSuccess ||
+        hipMalloc(&slot_index, (size_t)kDiagSlots * nd) != hipSuccess)
+        return done(fail(ctx, TE_ERR_ALLOC, "diagonal observable tables"));
+    if (hipMemsetAsync(keys, 0xff, sizeof(unsigned long long) * kDiagSlots * nd, s) != hipSuccess ||
+        hipMemsetAsync(cnt, 0, sizeof(int) * (nd + 1), s) != hipSuccess)
+        return done(fail(ctx, TE_ERR_HIP, "diagonal observable tables"));
+    const unsigned grid = (unsigned)std::min<int64_t>(((int64_t)nd * n + 255) / 256, (int64_t)ctx->num_cus * 8);
+    hipLaunchKernelGGL(diag_insert_kernel, dim3(grid), dim3(256), 0, s, ev->diag_all, nd, n, keys, cnt, cnt + nd);
+    std::vector<unsigned long long> hk((size_t)kDiagSlots * nd);
+    std::vector<int> hc(nd + 1);
+    if (hipMemcpyAsync(hk.data(), keys, sizeof(unsigned long long) * hk.size(), hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipMemcpyAsync(hc.data(), cnt, sizeof(int) * hc.size(), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return done(fail(ctx, TE_ERR_HIP, "diagonal observable tables"));
+    if (hc[nd] != 0) return done(TE_OK);                         // too many values: keep the doubles
+    // table order: ascending bit patterns (deterministic)
+    std::vector<unsigned char> hs((size_t)kDiagSlots * nd, 0);
+    std::vector<double> tab((size_t)kDiagTab * nd, 0.0);
+    for (int o = 0; o < nd; o++) {
+        std::vector<std::pair<unsigned long long, int>> used;
+        for (int h = 0; h < kDiagSlots; h++)
+            if (hk[(size_t)o * kDiagSlots + h] != kDiagEmpty) used.push_back({hk[(size_t)o * kDiagSlots + h], h});
+        std::sort(used.begin(), used.end());
+        for (size_t q = 0; q < used.size(); q++) {
+            hs[(size_t)o * kDiagSlots + used[q].second] = (unsigned char)q;
+            std::memcpy(&tab[(size_t)o * kDiagTab + q], &used[q].first, sizeof(double));
+        }
+    }
+    ev->idx_stride = (nd + 15) / 16 * 16;
+    if (hipMalloc(&ev->diag_idx, (size_t)ev->idx_stride * n) != hipSuccess ||
+        hipMalloc(&ev->diag_tab, sizeof(double) * kDiagTab * nd) != hipSuccess)
+        return done(fail(ctx, TE_ERR_ALLOC, "diagonal observable tables"));
+    if (hipMemcpyAsync(slot_index, hs.data(), hs.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(ev->diag_tab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, s) != hipSuccess)
+        return done(fail(ctx, TE_ERR_HIP, "diagonal observable tables"));
+    if (hipMemsetAsync(ev->diag_idx, 0, (size_t)ev->idx_stride * n, s) != hipSuccess)
+        return done(fail(ctx, TE_ERR_HIP, "diagonal observable tables"));
+    hipLaunchKernelGGL(diag_index_kernel, dim3(grid), dim3(256), 0, s, ev->diag_all, nd, n, keys, slot_index,
+                       ev->diag_idx, ev->idx_stride);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+        return done(fail(ctx, TE_ERR_HIP, "diagonal observable tables"));
+    hipFree(ev->diag_all);
+    ev->diag_all = nullptr;
+    return done(TE_OK);
+}
 
 static int evolver_create(te_ctx* ctx, te_matrix* H, const void* psi0, bool psi0_on_device,
                           const te_evolve_params* params, const te_observable* obs, int n_obs, te_evolver** out) {
@@ -801,6 +937,7 @@ static int evolver_create(te_ctx* ctx, te_matrix* H, const void* psi0, bool psi0
                            : TE_ERR_HIP);
         }
         TE_TRY(hipStreamSynchronize(ctx->stream) == hipSuccess ? TE_OK : TE_ERR_HIP);
+        TE_TRY(compress_diagonals(ev));
     }
     if (!ev->generic_obs.empty()) {
         TE_TRY(dmalloc(ctx, &ev->tmp, n));
@@ -914,6 +1051,9 @@ static void combine_step(te_evolver* ev, int m, long long base, int S, bool writ
         a.nobs = L[i].no;
         a.obs_base = L[i].o0;
         a.diag = ev->diag_all ? ev->diag_all + (size_t)L[i].o0 * ev->n : nullptr;
+        a.diag_idx = ev->diag_idx ? ev->diag_idx + L[i].o0 : nullptr;      // launches start on multiples of 16
+        a.idx_stride = ev->idx_stride;
+        a.diag_tab = ev->diag_tab ? ev->diag_tab + (size_t)L[i].o0 * kDiagTab : nullptr;
         a.write_w = (last && write_w) ? 1 : 0;
         a.write_last = (last && write_last && S > 0) ? 1 : 0;
         a.last_out = ev->sampled;
@@ -1137,6 +1277,12 @@ extern "C" int te_evolver_run(te_evolver* ev, double* obs_values, double* evolve
         for (int i = 0; i < n_obs; i++)
             for (size_t k = ev->recorded[i]; k < (size_t)ev->n_samples; k++) obs_values[(size_t)i * ev->n_samples + k] = 0.0;
     *result = R;
+    return TE_OK;
+}
+
+extern "C" int te_evolver_diag_bytes(const te_evolver* ev, size_t* bytes_per_row) {
+    if (!ev || !bytes_per_row) return TE_ERR_ARG;
+    *bytes_per_row = ev->diag_idx ? (size_t)ev->idx_stride : sizeof(double) * ev->diag_obs.size();
     return TE_OK;
 }
 

@@ -73,6 +73,9 @@ struct StepConsts {
     long long trace_cap;
 };
 
+// Distinct values a diagonal observable may have to be held as one-byte indices.
+constexpr int kDiagTab = 256;
+
 struct CombineArgs {
     int64_t n;
     const double2* U;
@@ -86,7 +89,10 @@ struct CombineArgs {
     int S;
     double2* last_out;                  // state of sample y_base + S - 1 when write_last
     int write_last;
-    const double* diag;                 // nobs x n, Re of diagonal observables
+    const double* diag;                 // nobs x n, Re of diagonal observables (or null: diag_idx)
+    const unsigned char* diag_idx;      // n x idx_stride bytes: row r's indices into diag_tab of
+    int idx_stride;                     // the launch's observables (when diag is null)
+    const double* diag_tab;             // nobs x kDiagTab distinct values
     int nobs;
     int obs_base;                       // index of the first of these observables in obs_values
     double* obs_values;                 // total_obs x n_samples

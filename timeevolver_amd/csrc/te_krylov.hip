@@ -929,8 +929,11 @@ template <int SC, int NOB>
 __global__ __launch_bounds__(kBlock) void combine_kernel(CombineArgs a) {
     __shared__ double2 cf[(1 + SC) * kMaxKrylov];
     __shared__ double2 sh[kBlock / 64];
+    __shared__ double tabs[NOB > 0 ? NOB * kDiagTab : 1];   // value tables of byte-indexed diagonals
     const int tid = threadIdx.x;
     const int m = a.m, S = a.S;
+    if (NOB > 0 && !a.diag && a.diag_tab)
+        for (int i = tid; i < a.nobs * kDiagTab; i += kBlock) tabs[i] = a.diag_tab[i];
     for (int i = tid; i < (1 + SC) * kMaxKrylov; i += kBlock) {
         const int c = i / kMaxKrylov, j = i % kMaxKrylov;
         double2 v = make_double2(0.0, 0.0);
@@ -987,8 +990,23 @@ __global__ __launch_bounds__(kBlock) void combine_kernel(CombineArgs a) {
         }
         if (NOB > 0) {
             double dg[NOB > 0 ? NOB : 1];                 // each observable's diagonal read once a row
+            if (a.diag) {
 #pragma unroll
-            for (int o = 0; o < (NOB > 0 ? NOB : 1); o++) dg[o] = o < a.nobs ? a.diag[(size_t)o * a.n + row] : 0.0;
+                for (int o = 0; o < (NOB > 0 ? NOB : 1); o++) dg[o] = o < a.nobs ? a.diag[(size_t)o * a.n + row] : 0.0;
+            } else {                                      // one-byte indices into the value tables
+                typedef unsigned u4v __attribute__((ext_vector_type(4)));
+                constexpr int NW = (NOB + 15) / 16;       // 16-byte words of indices per row
+                u4v wv[NW > 0 ? NW : 1];
+#pragma unroll
+                for (int q = 0; q < NW; q++)
+                    wv[q] = __builtin_nontemporal_load(
+                        reinterpret_cast<const u4v*>(a.diag_idx + (size_t)row * a.idx_stride) + q);
+#pragma unroll
+                for (int o = 0; o < (NOB > 0 ? NOB : 1); o++) {
+                    const unsigned word = wv[o / 16][(o % 16) / 4];
+                    dg[o] = o < a.nobs ? tabs[o * kDiagTab + ((word >> (8 * (o % 4))) & 0xffu)] : 0.0;
+                }
+            }
 #pragma unroll
             for (int s = 0; s < SC; s++) {
                 const double pr = as[s].x * as[s].x + as[s].y * as[s].y;
