@@ -363,10 +363,12 @@ def bench_adjoint(np, capi, ctx, args):
         out["K"][str(K)] = {"total_s": round(tk, 4), "ratio_vs_forward": round(tk / t_fwd, 3),
                             "device_forward_ms": round(info["forward_ms"], 2),
                             "device_backward_ms": round(info["backward_ms"], 2), "nodes": info["nodes"],
+                            "node_states": info["node_states"],
                             "max_phase": round(info["max_phase"], 1), "lambda_err": info["backward_err"],
                             "psi_err": info["psi_err"],
                             "grad_rel_diff_vs_K0": float(np.linalg.norm(grad - grad0) / np.linalg.norm(grad0))}
-        log("adjoint K=%d: %.3f s (forward %.3f s) ratio %.2f nodes %d" % (K, tk, t_fwd, tk / t_fwd, info["nodes"]))
+        log("adjoint K=%d: %.3f s (forward %.3f s) ratio %.2f nodes %d, node states %d"
+            % (K, tk, t_fwd, tk / t_fwd, info["nodes"], info["node_states"]))
         if K != 0:
             pk.adj.close()
     # central differences of the plain evolver's loss, tight tolerance

@@ -262,6 +262,8 @@ typedef struct {
     int capped;              /* steps whose rule was capped at the node limit (0 expected) */
     double backward_err;     /* sum of the a-posteriori Krylov error integrals of lambda */
     double psi_err;          /* the same for back-evolved psi (checkpoint interval >= 1) */
+    long long node_states;   /* node states formed per basis over all steps: each step's node
+                                sum in low-rank form (at most its quadrature node count) */
 } te_adjoint_info;
 /* Loss L = <psi(t)|O|psi(t)> for psi(t) = exp(-i e H t) psi0 (e = params->exp_factor, evolved by
  * the plain method with params' t, tol, m and fast_integration; the sampling step is ignored)

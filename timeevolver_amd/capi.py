@@ -51,7 +51,7 @@ class te_result(_c.Structure):
 class te_adjoint_info(_c.Structure):
     _fields_ = [("forward_ms", _c.c_double), ("backward_ms", _c.c_double), ("n_steps", _c.c_int),
                 ("nodes", _c.c_longlong), ("max_phase", _c.c_double), ("capped", _c.c_int),
-                ("backward_err", _c.c_double), ("psi_err", _c.c_double)]
+                ("backward_err", _c.c_double), ("psi_err", _c.c_double), ("node_states", _c.c_longlong)]
 
 
 _SIGS = {

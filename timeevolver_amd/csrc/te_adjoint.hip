@@ -37,6 +37,7 @@ constexpr int kNodeGroup = 8;       // nodes per node-interleaved block
 constexpr int kMatNodes = 64;       // nodes per materialisation pass (8 groups)
 constexpr int kAdjMaxLabels = 64;   // generators (LDS accumulators of the term-wise pass)
 constexpr int kAdjMaxSteps = 1 << 16;
+constexpr double kAdjRankTol = 1e-14;   // relative Frobenius residual of a step's rank form
 
 __global__ void diag_mul_kernel(const double* __restrict__ d, const double2* __restrict__ x, double2* __restrict__ y,
                                 int64_t n) {
@@ -929,6 +930,7 @@ static int backward(te_adjoint* a) {
     a->info.backward_err = 0;
     a->info.psi_err = 0;
     a->info.nodes = 0;
+    a->info.node_states = 0;
     a->info.max_phase = 0;
     a->info.capped = 0;
     // launch shapes of the node passes (blocks per CU; launch-shape sweeps: TE_ADJ_MAT_BLOCKS_PER_CU,
@@ -954,6 +956,11 @@ static int backward(te_adjoint* a) {
         const int p = v ? std::atoi(v) : kMatNodes;
         return p <= 16 ? 16 : (p <= 32 ? 32 : 64);
     }();
+    // The node sums in low-rank form, exact to kAdjRankTol of their norm (adjoint_rank_kernel);
+    // TE_ADJ_RANK_TOL sets another tolerance, 0 keeps the quadrature nodes themselves (read at
+    // every run).
+    double rank_tol = kAdjRankTol;
+    if (const char* v = std::getenv("TE_ADJ_RANK_TOL")) rank_tol = std::max(0.0, std::atof(v));
     for (int k = nsteps - 1; k >= 0; k--) {
         const double hstep = a->t_steps[k];
         // lambda: Lanczos of +i e H from lambda(t_{k+1}); with K >= 1 also psi's, in lockstep
@@ -1005,11 +1012,12 @@ static int backward(te_adjoint* a) {
         ca.coef_end[1] = a->cend1;
         ca.w = a->w;
         ca.out = a->step;
-        launch_adjoint_coef(ca, s);
+        launch_adjoint_coef(ca, rank_tol, s);
         TE_HIP(ctx, hipMemcpyAsync(a->step_host, a->step, sizeof(AdjStep), hipMemcpyDeviceToHost, s));
         TE_HIP(ctx, hipStreamSynchronize(s));
         const AdjStep st = *a->step_host;
-        a->info.nodes += st.n_nodes;
+        a->info.nodes += st.n_quad;
+        a->info.node_states += st.n_nodes;
         a->info.max_phase = std::max(a->info.max_phase, st.phase);
         a->info.backward_err += st.err[1];
         if (a->K >= 1) a->info.psi_err += st.err[0];

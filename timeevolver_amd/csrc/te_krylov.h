@@ -125,7 +125,8 @@ constexpr int kAdjMaxNodes = 256;   // quadrature nodes per Krylov step
 
 // What the host reads back after each backward step.
 struct AdjStep {
-    int n_nodes;         // quadrature nodes of the step (order x sub-intervals)
+    int n_nodes;         // node states the gradient passes form (n_quad, or fewer: the rank form)
+    int n_quad;          // quadrature nodes of the step (order x sub-intervals)
     int order, n_sub;
     int capped;          // the phase needed more than kAdjMaxNodes nodes (accuracy not assured)
     double phase;        // |omega|_max h: largest phase of the integrand over the step
@@ -146,7 +147,9 @@ struct AdjCoefArgs {
     double* w;                      // kAdjMaxNodes quadrature weights (on [t_k, t_{k+1}])
     AdjStep* out;
 };
-void launch_adjoint_coef(const AdjCoefArgs& a, hipStream_t s);
+// The step's rule and node coefficients; with rank_tol > 0 the node sum is then replaced by a
+// low-rank form of at most as many terms (adjoint_rank_kernel), exact to rank_tol relative.
+void launch_adjoint_coef(const AdjCoefArgs& a, double rank_tol, hipStream_t s);
 
 // Per-instance control flags of the batched driver, updated by the batched step controller.
 struct BatchCtl {
