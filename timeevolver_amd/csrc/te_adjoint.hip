@@ -53,18 +53,20 @@ __global__ void start_norm_kernel(const double2* res, KrylovScalars* ks, double*
     *scale = nv;
 }
 
-// Node states of one basis: out[g][r][q'] = sum_j U[j][r] coef[node0 + 8 g + q'][j] for the
-// nodes of this pass (zero past the step's node count; out holds the pass's blocks); end[r] =
-// sum_j U[j][r] cend[j] when end is given. A wavefront forms one 8-node block for a tile of 256
-// rows, kMatRows rows a lane: every coefficient is an LDS broadcast (one address for the whole
-// wavefront) that feeds 4 kMatRows FP64 FMAs, and the basis rows load two columns ahead. The
-// blocks of one tile go to neighbouring wavefronts, which read the same basis rows together
-// (the pass re-reads the basis once per block, from L2: splitting a block over two wavefronts
-// doubled that and was slower). A lane writes one 128-byte block per row.
+// Node states of one basis for the pass of nodes [node0, node0 + 64) of the step (the node count
+// and the basis' Krylov dimension are read from the step record, written by the coefficient
+// kernels on the device): out[g][r][q'] = sum_j U[j][r] coef[node0 + 8 g + q'][j] (zero past the
+// step's node count; out holds the pass's blocks); end[r] = sum_j U[j][r] cend[j] when end is
+// given. A wavefront forms one 8-node block for a tile of 256 rows, kMatRows rows a lane: every
+// coefficient is an LDS broadcast (one address for the whole wavefront) that feeds 4 kMatRows
+// FP64 FMAs, and the basis rows load two columns ahead. Work items are (tile, block) pairs, the
+// blocks of one tile on neighbouring wavefronts, which read the same basis rows together (the
+// pass re-reads the basis once per block, from L2: splitting a block over two wavefronts doubled
+// that and was slower). A lane writes one 128-byte block per row.
 struct MatArgs {
     const double2* U;
     int64_t ld, n;
-    int m;
+    int basis;                // 0: psi, 1: lambda (its Krylov dimension: step->m[basis])
     const double2* coef;      // kAdjMaxNodes x kMaxKrylov
     const double2* cend;      // kMaxKrylov (with end)
     const AdjStep* step;
@@ -76,28 +78,29 @@ struct MatArgs {
 constexpr int kMatRows = 4;        // rows formed by one lane
 constexpr int kMatTile = 64 * kMatRows;
 
-template <int NG>                  // node groups in the pass (8 NG nodes)
 __global__ __launch_bounds__(256) void materialise_nodes_kernel(MatArgs a) {
-    constexpr int NQ = NG * kNodeGroup;
-    __shared__ double2 cs[kMaxKrylov * (NQ + 1)];        // cs[j * (NQ + 1) + q], q = NQ: cend
+    __shared__ double2 cs[kMaxKrylov * (kMatNodes + 1)];      // cs[j * (nq + 1) + q], q = nq: cend
     const int nn = a.step->n_nodes;
-    const int nloc = min(NQ, nn - a.node0);
-    for (int i = threadIdx.x; i < a.m * (NQ + 1); i += blockDim.x) {
-        const int j = i / (NQ + 1), q = i % (NQ + 1);
+    const int m = a.step->m[a.basis];
+    const int nloc = max(0, min(kMatNodes, nn - a.node0));
+    const int ng = (nloc + kNodeGroup - 1) / kNodeGroup;     // blocks written by this pass
+    const int nq = ng * kNodeGroup;
+    const int ngi = max(ng, 1);                               // blocks per tile in the work items
+    for (int i = threadIdx.x; i < m * (nq + 1); i += blockDim.x) {
+        const int j = i / (nq + 1), q = i % (nq + 1);
         double2 c = make_double2(0.0, 0.0);
         if (q < nloc) c = a.coef[(size_t)(a.node0 + q) * kMaxKrylov + j];
-        else if (q == NQ && a.end) c = a.cend[j];
+        else if (q == nq && a.end) c = a.cend[j];
         cs[i] = c;
     }
     __syncthreads();
-    const int ng = (nloc + kNodeGroup - 1) / kNodeGroup;        // blocks written by this pass
     const int lane = threadIdx.x & 63;
     const int waves = blockDim.x >> 6;
-    const int64_t items = (a.n + kMatTile - 1) / kMatTile * NG;   // (tile, block) pairs
+    const int64_t items = (a.n + kMatTile - 1) / kMatTile * ngi;   // (tile, block) pairs
     for (int64_t it = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * waves + (threadIdx.x >> 6))); it < items;
          it += (int64_t)gridDim.x * waves) {
-        const int64_t tile = it / NG;
-        const int g = (int)(it % NG);
+        const int64_t tile = it / ngi;
+        const int g = (int)(it % ngi);
         const bool with_end = a.end && g == 0;
         if (g >= ng && !with_end) continue;
         int r[kMatRows];                                   // rows < 2^31 (te_adjoint_create)
@@ -113,15 +116,16 @@ __global__ __launch_bounds__(256) void materialise_nodes_kernel(MatArgs a) {
         auto load_u = [&](int j, double2* u) {
 #pragma unroll
             for (int i = 0; i < kMatRows; i++)
-                u[i] = (j < a.m && r[i] < a.n) ? a.U[(size_t)j * a.ld + r[i]] : make_double2(0.0, 0.0);
+                u[i] = (j < m && r[i] < a.n) ? a.U[(size_t)j * a.ld + r[i]] : make_double2(0.0, 0.0);
         };
         double2 u0[kMatRows], u1[kMatRows];
         load_u(0, u0);
         load_u(1, u1);
-        for (int j = 0; j < a.m; j++) {
+        const int gq = g < ng ? g * kNodeGroup : 0;
+        for (int j = 0; j < m; j++) {
             double2 u2[kMatRows];
             load_u(j + 2, u2);
-            const double2* cj = cs + j * (NQ + 1) + g * kNodeGroup;
+            const double2* cj = cs + j * (nq + 1) + gq;
 #pragma unroll
             for (int q = 0; q < kNodeGroup; q++) {
                 const double2 c = cj[q];
@@ -132,7 +136,7 @@ __global__ __launch_bounds__(256) void materialise_nodes_kernel(MatArgs a) {
                 }
             }
             if (with_end) {
-                const double2 c = cs[j * (NQ + 1) + NQ];
+                const double2 c = cs[j * (nq + 1) + nq];
 #pragma unroll
                 for (int i = 0; i < kMatRows; i++) {
                     ae[i].x += u0[i].x * c.x - u0[i].y * c.y;
@@ -158,22 +162,17 @@ __global__ __launch_bounds__(256) void materialise_nodes_kernel(MatArgs a) {
     }
 }
 
-// A pass of `nodes` nodes (at most kMatNodes): 2, 4 or 8 groups of 8.
-static void launch_materialise(const MatArgs& m, int nodes, int grid, hipStream_t s) {
-    if (nodes <= 2 * kNodeGroup) hipLaunchKernelGGL(materialise_nodes_kernel<2>, dim3(grid), dim3(256), 0, s, m);
-    else if (nodes <= 4 * kNodeGroup) hipLaunchKernelGGL(materialise_nodes_kernel<4>, dim3(grid), dim3(256), 0, s, m);
-    else hipLaunchKernelGGL(materialise_nodes_kernel<8>, dim3(grid), dim3(256), 0, s, m);
-}
-
-// One group of 8 nodes: for every row r, L_q = w_q conj(lambda_q(r)),
+// The node blocks of one materialisation pass (the step's nodes [node0, node0 + 64), count from
+// the step record): for every row r and node q, L_q = w_q conj(lambda_q(r)),
 //   Z(r) += sum_q Im(L_q psi_q(r))                              (diagonal generators)
 //   acc[label] += Im(t sum_q L_q psi_q(c))  per entry (r, c, label, t)   (off-diagonal ones)
 // The labelled entries are held as sliced ELL (64-row slices, slot-major, padded to the slice's
 // longest row, row lengths predicate the padding) with values through a table of the distinct
-// values in LDS (or stored when there are too many); a wavefront takes a slice, lane = row,
-// loading the next slot's column, label and value index before the current slot's gather.
-// Per-thread label accumulators live in LDS; block sums per label go to partials and the last
-// block adds their sum (in block order) to out[label].
+// values in LDS (or stored when there are too many); a wavefront takes a slice, lane = row, and
+// goes through the pass's 8-node blocks GP at a time (GP blocks gathered per entry), loading the
+// next slot's column, label and value index before the current slot's gathers. Per-thread label
+// accumulators live in LDS; block sums per label go to partials, which termwise_finish_kernel
+// adds (in block order) to out[label].
 constexpr int kTermMaxValues = 1024;
 
 struct TermArgs {
@@ -188,86 +187,96 @@ struct TermArgs {
     const double2* vtab;
     int n_values;
     int n_labels;               // compact labels
-    const double2* psi;         // node block of the group, n x 8
+    const double2* psi;         // the pass's node blocks, n x 8 each
     const double2* lam;
     const double* w;            // kAdjMaxNodes weights
     const AdjStep* step;
-    int group;
+    int node0;
     double* zsum;
     double* partials;           // gridDim.x x n_labels
     double* out;                // n_labels running sums (termwise_finish_kernel adds to them)
 };
 
-template <int LCAP, bool TABLE>
+template <int LCAP, bool TABLE, int GP>
 __global__ __launch_bounds__(256) void termwise_kernel(TermArgs a) {
     __shared__ double acc[LCAP * 256];
     __shared__ double2 vt[TABLE ? kTermMaxValues : 1];
-    __shared__ double wq[kNodeGroup];
+    __shared__ double wq[kMatNodes];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int nloc = max(0, min(kMatNodes, a.step->n_nodes - a.node0));
+    const int ng = (nloc + kNodeGroup - 1) / kNodeGroup;
     for (int l = 0; l < a.n_labels; l++) acc[l * 256 + tid] = 0.0;
     if (TABLE)
         for (int i = tid; i < a.n_values; i += 256) vt[i] = a.vtab[i];
-    if (tid < kNodeGroup) {
-        const int q = a.group * kNodeGroup + tid;
-        wq[tid] = q < a.step->n_nodes ? a.w[q] : 0.0;
-    }
+    if (tid < kMatNodes) wq[tid] = tid < nloc ? a.w[a.node0 + tid] : 0.0;
     __syncthreads();
     const int n_sl = (int)((a.n + 63) / 64);
+    const size_t gstride = (size_t)a.n * kNodeGroup;
     for (int sl = blockIdx.x * 4 + wv; sl < n_sl; sl += gridDim.x * 4) {
         const int64_t r = (int64_t)sl * 64 + lane;
-        double2 L[kNodeGroup];
-        int len = 0;
-        if (r < a.n) {
-            const double2* lr = a.lam + (size_t)r * kNodeGroup;
-            const double2* pr = a.psi + (size_t)r * kNodeGroup;
-            double z = 0.0;
+        const int len = (r < a.n && a.n_slices) ? a.slen[r] : 0;
+        const int o = a.n_slices ? a.soff[sl] : 0;
+        const int ls = a.n_slices ? (a.soff[sl + 1] - o) >> 6 : 0;
+        double z = 0.0;
+        for (int g0 = 0; g0 < ng; g0 += GP) {
+            double2 L[GP][kNodeGroup];
 #pragma unroll
-            for (int q = 0; q < kNodeGroup; q++) {
-                const double2 l = lr[q];
-                L[q] = make_double2(wq[q] * l.x, -wq[q] * l.y);
-                const double2 p = pr[q];
-                z += L[q].x * p.y + L[q].y * p.x;
+            for (int h = 0; h < GP; h++) {
+                const int g = g0 + h;
+#pragma unroll
+                for (int q = 0; q < kNodeGroup; q++) L[h][q] = make_double2(0.0, 0.0);
+                if (r < a.n && g < ng) {
+                    const double2* lr = a.lam + g * gstride + (size_t)r * kNodeGroup;
+                    const double2* pr = a.psi + g * gstride + (size_t)r * kNodeGroup;
+#pragma unroll
+                    for (int q = 0; q < kNodeGroup; q++) {
+                        const double2 l = lr[q];
+                        const double wv_ = wq[g * kNodeGroup + q];
+                        L[h][q] = make_double2(wv_ * l.x, -wv_ * l.y);
+                        const double2 p = pr[q];
+                        z += L[h][q].x * p.y + L[h][q].y * p.x;
+                    }
+                }
             }
-            a.zsum[r] += z;
-            if (a.n_slices) len = a.slen[r];
-        } else {
-#pragma unroll
-            for (int q = 0; q < kNodeGroup; q++) L[q] = make_double2(0.0, 0.0);
-        }
-        if (!a.n_slices) continue;
-        const int o = a.soff[sl];
-        const int ls = (a.soff[sl + 1] - o) >> 6;
-        int c = 0, lb = 0, vi = 0;
-        double2 tv = make_double2(0.0, 0.0);
-        if (ls > 0) {
-            const size_t k = (size_t)o + lane;
-            c = a.scol[k];
-            lb = a.slab[k];
-            if (TABLE) vi = a.svid[k];
-            else tv = a.sval[k];
-        }
-        for (int j = 0; j < ls; j++) {
-            const int cc = c, lbl = lb;
-            const double2 t = TABLE ? vt[vi] : tv;
-            if (j + 1 < ls) {                          // next slot's indices in flight
-                const size_t k = (size_t)o + (size_t)(j + 1) * 64 + lane;
+            if (ls == 0) continue;
+            const bool two = GP > 1 && g0 + 1 < ng;
+            int c = 0, lb = 0, vi = 0;
+            double2 tv = make_double2(0.0, 0.0);
+            {
+                const size_t k = (size_t)o + lane;
                 c = a.scol[k];
                 lb = a.slab[k];
                 if (TABLE) vi = a.svid[k];
                 else tv = a.sval[k];
             }
-            if (j < len) {
-                const double2* pc = a.psi + (size_t)cc * kNodeGroup;
-                double sr = 0.0, si = 0.0;
-#pragma unroll
-                for (int q = 0; q < kNodeGroup; q++) {
-                    const double2 p = pc[q];
-                    sr += L[q].x * p.x - L[q].y * p.y;
-                    si += L[q].x * p.y + L[q].y * p.x;
+            for (int j = 0; j < ls; j++) {
+                const int cc = c, lbl = lb;
+                const double2 t = TABLE ? vt[vi] : tv;
+                if (j + 1 < ls) {                          // next slot's indices in flight
+                    const size_t k = (size_t)o + (size_t)(j + 1) * 64 + lane;
+                    c = a.scol[k];
+                    lb = a.slab[k];
+                    if (TABLE) vi = a.svid[k];
+                    else tv = a.sval[k];
                 }
-                acc[lbl * 256 + tid] += t.x * si + t.y * sr;
+                if (j < len) {
+                    double sr = 0.0, si = 0.0;
+#pragma unroll
+                    for (int h = 0; h < GP; h++) {
+                        if (h > 0 && !two) break;
+                        const double2* pc = a.psi + (g0 + h) * gstride + (size_t)cc * kNodeGroup;
+#pragma unroll
+                        for (int q = 0; q < kNodeGroup; q++) {
+                            const double2 p = pc[q];
+                            sr += L[h][q].x * p.x - L[h][q].y * p.y;
+                            si += L[h][q].x * p.y + L[h][q].y * p.x;
+                        }
+                    }
+                    acc[lbl * 256 + tid] += t.x * si + t.y * sr;
+                }
             }
         }
+        if (r < a.n && ng > 0) a.zsum[r] += z;
     }
     __shared__ double2 sh[4];
     for (int l = 0; l < a.n_labels; l++) {
@@ -316,17 +325,24 @@ __global__ __launch_bounds__(256) void diag_grad_kernel(const double* __restrict
     }
 }
 
-template <bool TABLE>
+template <bool TABLE, int GP>
 static void launch_termwise_t(const TermArgs& ta, int grid, hipStream_t s) {
-    if (ta.n_labels <= 8) hipLaunchKernelGGL((termwise_kernel<8, TABLE>), dim3(grid), dim3(256), 0, s, ta);
-    else if (ta.n_labels <= 16) hipLaunchKernelGGL((termwise_kernel<16, TABLE>), dim3(grid), dim3(256), 0, s, ta);
-    else if (ta.n_labels <= 32) hipLaunchKernelGGL((termwise_kernel<32, TABLE>), dim3(grid), dim3(256), 0, s, ta);
-    else hipLaunchKernelGGL((termwise_kernel<64, TABLE>), dim3(grid), dim3(256), 0, s, ta);
+    if (ta.n_labels <= 8) hipLaunchKernelGGL((termwise_kernel<8, TABLE, GP>), dim3(grid), dim3(256), 0, s, ta);
+    else if (ta.n_labels <= 16) hipLaunchKernelGGL((termwise_kernel<16, TABLE, GP>), dim3(grid), dim3(256), 0, s, ta);
+    else if (ta.n_labels <= 32) hipLaunchKernelGGL((termwise_kernel<32, TABLE, GP>), dim3(grid), dim3(256), 0, s, ta);
+    else hipLaunchKernelGGL((termwise_kernel<64, TABLE, GP>), dim3(grid), dim3(256), 0, s, ta);
 }
 
-static void launch_termwise(const TermArgs& ta, int grid, hipStream_t s) {
-    if (ta.vtab) launch_termwise_t<true>(ta, grid, s);
-    else launch_termwise_t<false>(ta, grid, s);
+// One term-wise pass over a materialisation pass's node blocks (gp: blocks gathered per entry,
+// 1 or 2), then its finish launch.
+static void launch_termwise(const TermArgs& ta, int grid, int gp, hipStream_t s) {
+    if (ta.vtab) {
+        if (gp > 1) launch_termwise_t<true, 2>(ta, grid, s);
+        else launch_termwise_t<true, 1>(ta, grid, s);
+    } else {
+        if (gp > 1) launch_termwise_t<false, 2>(ta, grid, s);
+        else launch_termwise_t<false, 1>(ta, grid, s);
+    }
     hipLaunchKernelGGL(termwise_finish_kernel, dim3(1), dim3(256), 0, s, ta.partials, grid, ta.n_labels, ta.out);
 }
 
@@ -389,9 +405,13 @@ struct te_adjoint {
     int* gl_off = nullptr;
     double2 *coef0 = nullptr, *coef1 = nullptr, *cend0 = nullptr, *cend1 = nullptr;
     double* w = nullptr;
-    AdjStep* step = nullptr;
+    hipStream_t s2 = nullptr;       // term-wise passes beside the next step's coefficient kernels
+    hipEvent_t ev_lanczos = nullptr, ev_term = nullptr;
+    AdjStep* steps = nullptr;       // one record per step of the last backward
+    int steps_cap = 0;
     AdjStep* step_host = nullptr;
     double2 *nodes_psi = nullptr, *nodes_lam = nullptr;
+    double2 *nodes_psi1 = nullptr, *nodes_lam1 = nullptr;   // odd steps in low-rank form
     double2 *psi_next = nullptr, *lam_next = nullptr;
     double* zsum = nullptr;
     double* offacc = nullptr;       // P
@@ -420,12 +440,16 @@ static void adj_free(te_adjoint* a) {
     void* ptrs[] = {a->soff, a->scol, a->slab, a->svid, a->sval, a->slen, a->vtab, a->gdiag,  a->odiag,  a->U,      a->psiT,
                     a->scratch, a->psi0, a->ks,   a->ks_hist, a->st,  a->wcoef, a->Y,      a->trace,
                     a->obs,  a->Vl,      a->Vp,   a->ksl,  a->ksp,    a->scale,  a->gl,     a->gl_off,
-                    a->coef0, a->coef1,  a->cend0, a->cend1, a->w,    a->step,   a->nodes_psi, a->nodes_lam,
+                    a->coef0, a->coef1,  a->cend0, a->cend1, a->w,    a->steps,  a->nodes_psi, a->nodes_lam,
+                    a->nodes_psi1, a->nodes_lam1,
                     a->psi_next, a->lam_next, a->zsum, a->offacc, a->dgrad, a->partials, a->ticket};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (a->st_host) hipHostFree(a->st_host);
     if (a->step_host) hipHostFree(a->step_host);
+    if (a->ev_lanczos) hipEventDestroy(a->ev_lanczos);
+    if (a->ev_term) hipEventDestroy(a->ev_term);
+    if (a->s2) hipStreamDestroy(a->s2);
     free_quadrature(a->quad);
     reduce_scratch_free(a->rs);
     delete a;
@@ -696,11 +720,17 @@ extern "C" int te_adjoint_create(te_ctx* ctx, te_matrix* H, int n_params, te_mat
     TE_TRY(aalloc(ctx, &a->coef1, (size_t)kAdjMaxNodes * kMaxKrylov));
     TE_TRY(aalloc(ctx, &a->cend0, kMaxKrylov));
     TE_TRY(aalloc(ctx, &a->cend1, kMaxKrylov));
-    TE_TRY(aalloc(ctx, &a->w, kAdjMaxNodes));
-    TE_TRY(aalloc(ctx, &a->step, 1));
+    TE_TRY(aalloc(ctx, &a->w, 2 * kAdjMaxNodes));   // two steps' weights (alternating)
+    TE_TRYH(hipStreamCreateWithFlags(&a->s2, hipStreamNonBlocking));
+    TE_TRYH(hipEventCreateWithFlags(&a->ev_lanczos, hipEventDisableTiming));
+    TE_TRYH(hipEventCreateWithFlags(&a->ev_term, hipEventDisableTiming));
     TE_TRYH(hipHostMalloc((void**)&a->step_host, sizeof(AdjStep)));
     TE_TRY(aalloc(ctx, &a->nodes_psi, (size_t)kMatNodes * n));
     TE_TRY(aalloc(ctx, &a->nodes_lam, (size_t)kMatNodes * n));
+    // in low-rank form a step has at most m node states: the odd steps' blocks need no more
+    const int m_nodes = std::min(kMatNodes, (a->m0 + kNodeGroup - 1) / kNodeGroup * kNodeGroup);
+    TE_TRY(aalloc(ctx, &a->nodes_psi1, (size_t)m_nodes * n));
+    TE_TRY(aalloc(ctx, &a->nodes_lam1, (size_t)m_nodes * n));
     TE_TRY(aalloc(ctx, &a->psi_next, n));
     TE_TRY(aalloc(ctx, &a->lam_next, n));
     TE_TRY(aalloc(ctx, &a->zsum, n));
@@ -950,19 +980,69 @@ static int backward(te_adjoint* a) {
                        ? std::min(a->grid, std::max(8, std::min<int>((int)((n + 255) / 256),
                                                                      ctx->num_cus * term_per_cu)) / 8 * 8)
                        : a->grid;
-    // nodes per materialisation pass (launch-shape sweeps: TE_ADJ_PASS_NODES = 16, 32 or 64)
-    static const int pass_nodes = [] {
-        const char* v = std::getenv("TE_ADJ_PASS_NODES");
-        const int p = v ? std::atoi(v) : kMatNodes;
-        return p <= 16 ? 16 : (p <= 32 ? 32 : 64);
+    // Term-wise passes gather the node blocks of one group per entry (TE_ADJ_TERM_GROUPS=2: two;
+    // on the L = N = 12 ring, K = 1: backward 34.0 ms with one, 34.7 ms with two).
+    static const int term_groups = [] {
+        const char* v = std::getenv("TE_ADJ_TERM_GROUPS");
+        return (v && std::atoi(v) == 2) ? 2 : 1;
     }();
     // The node sums in low-rank form, exact to kAdjRankTol of their norm (adjoint_rank_kernel);
     // TE_ADJ_RANK_TOL sets another tolerance, 0 keeps the quadrature nodes themselves (read at
     // every run).
     double rank_tol = kAdjRankTol;
     if (const char* v = std::getenv("TE_ADJ_RANK_TOL")) rank_tol = std::max(0.0, std::atof(v));
+    // One step record per step. In low-rank form a step has at most min(nodes, m) <= 64 node
+    // states, one materialisation pass and one term-wise pass, whose kernels read the counts and
+    // Krylov dimensions from the record: the backward is queued without a host round trip and the
+    // records are read once at the end. The quadrature nodes themselves (up to kAdjMaxNodes) need
+    // passes of 64 nodes, counted by the host after each step's coefficient kernel.
+    if (a->steps_cap < nsteps) {
+        if (a->steps) hipFree(a->steps);
+        a->steps = nullptr;
+        a->steps_cap = 0;
+        TE_HIP(ctx, hipMalloc(&a->steps, sizeof(AdjStep) * nsteps));
+        a->steps_cap = nsteps;
+    }
+    // In low-rank form, step k's term-wise pass (the last reader of its node states and weights)
+    // runs on the second stream once step k - 1's Lanczos recurrences are done, beside step
+    // k - 1's coefficient and rank kernels (one block each: the rest of the device would idle)
+    // and materialisation. Node states and weights alternate between two buffers by step parity,
+    // and step k - 2's coefficient kernels (the next writer of step k's buffers) wait for it.
+    const bool overlap = rank_tol > 0.0;
+    TermArgs pend{};
+    bool pending = false;          // step k + 1's pass, to be queued on the second stream
+    bool term_queued = false;      // a pass queued there that the main stream has not joined
+    auto term_args = [&](AdjStep* step, const double* w, int node0, const double2* psi_nodes,
+                         const double2* lam_nodes) {
+        TermArgs ta{};
+        ta.n = n;
+        ta.n_slices = a->n_slices;
+        ta.soff = a->soff;
+        ta.scol = a->scol;
+        ta.slab = a->slab;
+        ta.svid = a->svid;
+        ta.sval = a->sval;
+        ta.slen = a->slen;
+        ta.vtab = a->vtab;
+        ta.n_values = a->n_values;
+        ta.n_labels = a->n_off;
+        ta.psi = psi_nodes;
+        ta.lam = lam_nodes;
+        ta.w = w;
+        ta.step = step;
+        ta.node0 = node0;
+        ta.zsum = a->zsum;
+        ta.partials = a->partials;
+        ta.out = a->offacc;
+        return ta;
+    };
     for (int k = nsteps - 1; k >= 0; k--) {
         const double hstep = a->t_steps[k];
+        AdjStep* step = a->steps + k;
+        const bool odd = overlap && (k & 1);
+        double* wk = a->w + (size_t)(odd ? kAdjMaxNodes : 0);
+        double2* psi_nodes = odd ? a->nodes_psi1 : a->nodes_psi;
+        double2* lam_nodes = odd ? a->nodes_lam1 : a->nodes_lam;
         // lambda: Lanczos of +i e H from lambda(t_{k+1}); with K >= 1 also psi's, in lockstep
         start_norm(a, a->Vl, a->ksl, a->scale + 1);
         if (a->K == 0) {
@@ -976,9 +1056,22 @@ static int backward(te_adjoint* a) {
             start_norm(a, a->Vp, a->ksp, a->scale);
             lanczos2(a, a->Vl, a->ksl, a->Vp, a->ksp, a->m0, alpha_b);
         }
+        if (term_queued) {            // step k + 2's pass: its buffers are written next
+            TE_HIP(ctx, hipStreamWaitEvent(s, a->ev_term, 0));
+            term_queued = false;
+        }
+        if (pending) {
+            TE_HIP(ctx, hipEventRecord(a->ev_lanczos, s));
+            TE_HIP(ctx, hipStreamWaitEvent(a->s2, a->ev_lanczos, 0));
+            launch_termwise(pend, tg, term_groups, a->s2);
+            TE_HIP(ctx, hipEventRecord(a->ev_term, a->s2));
+            pending = false;
+            term_queued = true;
+        }
         AdjCoefArgs ca{};
         ca.ks[1] = a->ksl;
-        ca.m[1] = a->m0;          // replaced by m_hbd below when the recurrence broke down
+        ca.m[1] = a->m0;
+        ca.m_from_ks[1] = 1;          // m_hbd when lambda's recurrence broke down
         ca.from_end[1] = 1;
         ca.scale[1] = a->scale + 1;
         const double2* Upsi;
@@ -992,17 +1085,10 @@ static int backward(te_adjoint* a) {
             Upsi = a->Vp;
             ca.ks[0] = a->ksp;
             ca.m[0] = a->m0;
+            ca.m_from_ks[0] = 1;
             ca.from_end[0] = 1;
             ca.scale[0] = a->scale;
         }
-        // Krylov dimensions after a breakdown (read back with the step record)
-        KrylovScalars kl{};
-        TE_HIP(ctx, hipMemcpyAsync(&kl, a->ksl, sizeof(KrylovScalars), hipMemcpyDeviceToHost, s));
-        KrylovScalars kp{};
-        if (a->K >= 1) TE_HIP(ctx, hipMemcpyAsync(&kp, a->ksp, sizeof(KrylovScalars), hipMemcpyDeviceToHost, s));
-        TE_HIP(ctx, hipStreamSynchronize(s));
-        if (kl.broken) ca.m[1] = kl.m_hbd;
-        if (a->K >= 1 && kp.broken) ca.m[0] = kp.m_hbd;
         ca.h = hstep;
         ca.gl = a->gl;
         ca.gl_off = a->gl_off;
@@ -1010,51 +1096,31 @@ static int backward(te_adjoint* a) {
         ca.coef[1] = a->coef1;
         ca.coef_end[0] = a->cend0;
         ca.coef_end[1] = a->cend1;
-        ca.w = a->w;
-        ca.out = a->step;
+        ca.w = wk;
+        ca.out = step;
         launch_adjoint_coef(ca, rank_tol, s);
-        TE_HIP(ctx, hipMemcpyAsync(a->step_host, a->step, sizeof(AdjStep), hipMemcpyDeviceToHost, s));
-        TE_HIP(ctx, hipStreamSynchronize(s));
-        const AdjStep st = *a->step_host;
-        a->info.nodes += st.n_quad;
-        a->info.node_states += st.n_nodes;
-        a->info.max_phase = std::max(a->info.max_phase, st.phase);
-        a->info.backward_err += st.err[1];
-        if (a->K >= 1) a->info.psi_err += st.err[0];
-        if (st.capped) a->info.capped++;
-        if (st.capped == 2) return fail(ctx, TE_ERR_FATAL, "Internal error: eigen-solver did not converge");
+        int n_nodes = kMatNodes;      // in low-rank form: one pass covers every node state
+        if (rank_tol <= 0.0) {
+            AdjStep st{};
+            TE_HIP(ctx, hipMemcpyAsync(a->step_host, step, sizeof(AdjStep), hipMemcpyDeviceToHost, s));
+            TE_HIP(ctx, hipStreamSynchronize(s));
+            st = *a->step_host;
+            if (st.capped == 2) return fail(ctx, TE_ERR_FATAL, "Internal error: eigen-solver did not converge");
+            n_nodes = st.n_nodes;
+        }
         // node states, 64 per pass, and the states at t_k
-        for (int node0 = 0; node0 < st.n_nodes; node0 += pass_nodes) {
-            MatArgs mp{Upsi, n, n, ca.m[0], a->coef0, a->cend0, a->step, node0, a->nodes_psi,
+        for (int node0 = 0; node0 < n_nodes; node0 += kMatNodes) {
+            MatArgs mp{Upsi, n, n, 0, a->coef0, a->cend0, step, node0, psi_nodes,
                        (node0 == 0 && a->K >= 1) ? a->psi_next : nullptr};
-            MatArgs ml{a->Vl, n, n, ca.m[1], a->coef1, a->cend1, a->step, node0, a->nodes_lam,
+            MatArgs ml{a->Vl, n, n, 1, a->coef1, a->cend1, step, node0, lam_nodes,
                        node0 == 0 ? a->lam_next : nullptr};
-            const int pn = std::min(pass_nodes, st.n_nodes - node0);
-            const int ng = (pn + kNodeGroup - 1) / kNodeGroup;
-            launch_materialise(mp, pn, mg, s);
-            launch_materialise(ml, pn, mg, s);
-            for (int g = 0; g < ng; g++) {
-                TermArgs ta{};
-                ta.n = n;
-                ta.n_slices = a->n_slices;
-                ta.soff = a->soff;
-                ta.scol = a->scol;
-                ta.slab = a->slab;
-                ta.svid = a->svid;
-                ta.sval = a->sval;
-                ta.slen = a->slen;
-                ta.vtab = a->vtab;
-                ta.n_values = a->n_values;
-                ta.n_labels = a->n_off;
-                ta.psi = a->nodes_psi + (size_t)g * n * kNodeGroup;
-                ta.lam = a->nodes_lam + (size_t)g * n * kNodeGroup;
-                ta.w = a->w;
-                ta.step = a->step;
-                ta.group = node0 / kNodeGroup + g;
-                ta.zsum = a->zsum;
-                ta.partials = a->partials;
-                ta.out = a->offacc;
-                launch_termwise(ta, tg, s);
+            hipLaunchKernelGGL(materialise_nodes_kernel, dim3(mg), dim3(256), 0, s, mp);
+            hipLaunchKernelGGL(materialise_nodes_kernel, dim3(mg), dim3(256), 0, s, ml);
+            if (overlap) {
+                pend = term_args(step, wk, node0, psi_nodes, lam_nodes);
+                pending = true;
+            } else {
+                launch_termwise(term_args(step, wk, node0, psi_nodes, lam_nodes), tg, term_groups, s);
             }
         }
         TE_HIP(ctx, hipGetLastError());
@@ -1063,10 +1129,26 @@ static int backward(te_adjoint* a) {
         if (a->K >= 1)
             TE_HIP(ctx, hipMemcpyAsync(a->Vp, a->psi_next, sizeof(double2) * n, hipMemcpyDeviceToDevice, s));
     }
+    if (term_queued) TE_HIP(ctx, hipStreamWaitEvent(s, a->ev_term, 0));
+    if (pending) launch_termwise(pend, tg, term_groups, s);    // the last step's pass: nothing to overlap
     if (a->P > 0)
         hipLaunchKernelGGL(diag_grad_kernel, dim3(tg), dim3(256), 0, s, a->gdiag, a->zsum, n, a->P, a->partials,
                            a->ticket, a->dgrad);
     TE_HIP(ctx, hipGetLastError());
+    std::vector<AdjStep> steps(nsteps);
+    if (nsteps > 0) {
+        TE_HIP(ctx, hipMemcpyAsync(steps.data(), a->steps, sizeof(AdjStep) * nsteps, hipMemcpyDeviceToHost, s));
+        TE_HIP(ctx, hipStreamSynchronize(s));
+    }
+    for (const AdjStep& st : steps) {
+        a->info.nodes += st.n_quad;
+        a->info.node_states += st.n_nodes;
+        a->info.max_phase = std::max(a->info.max_phase, st.phase);
+        a->info.backward_err += st.err[1];
+        if (a->K >= 1) a->info.psi_err += st.err[0];
+        if (st.capped) a->info.capped++;
+        if (st.capped == 2) return fail(ctx, TE_ERR_FATAL, "Internal error: eigen-solver did not converge");
+    }
     return TE_OK;
 }
 

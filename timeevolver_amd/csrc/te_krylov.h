@@ -127,6 +127,7 @@ constexpr int kAdjMaxNodes = 256;   // quadrature nodes per Krylov step
 struct AdjStep {
     int n_nodes;         // node states the gradient passes form (n_quad, or fewer: the rank form)
     int n_quad;          // quadrature nodes of the step (order x sub-intervals)
+    int m[2];            // Krylov dimension of each basis over the step (after a breakdown: m_hbd)
     int order, n_sub;
     int capped;          // the phase needed more than kAdjMaxNodes nodes (accuracy not assured)
     double phase;        // |omega|_max h: largest phase of the integrand over the step
@@ -135,7 +136,8 @@ struct AdjStep {
 
 struct AdjCoefArgs {
     const KrylovScalars* ks[2];     // basis 0: psi, basis 1: lambda
-    int m[2];                       // Krylov dimension (m_hbd after a breakdown)
+    int m[2];                       // Krylov dimension
+    int m_from_ks[2];               // 1: m_hbd from ks instead when that recurrence broke down
     int from_end[2];                // 0: the basis starts at the step start t_k (coefficients
                                     //    at x = s - t_k); 1: it starts at t_{k+1} (at h - x)
     const double* scale[2];         // norm of the basis' start vector (null: 1)

@@ -754,6 +754,7 @@ struct AdjShared {
     double p[2][kMaxKrylov];
     double lo[2], hi[2];
     int info[2];
+    int mm[2];
     int order, n_sub;
     double phase;
 };
@@ -800,8 +801,14 @@ __device__ double gl_log_bound(int n, double phase) {
 __global__ __launch_bounds__(256) void adjoint_coef_kernel(AdjCoefArgs a) {
     __shared__ AdjShared S;
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
-    if (wv < 2 && lane < a.m[wv]) {
-        const int b = wv, m = a.m[b];
+    if (tid < 2) {
+        int m = a.m[tid];
+        if (a.m_from_ks[tid] && a.ks[tid]->broken) m = a.ks[tid]->m_hbd;
+        S.mm[tid] = m;
+    }
+    __syncthreads();
+    if (wv < 2 && lane < S.mm[wv]) {
+        const int b = wv, m = S.mm[b];
         const KrylovScalars* ks = a.ks[b];
         S.d[b][lane] = -ks->a[lane].y;
         S.e[b][lane] = (lane + 1 < m) ? ks->beta[lane] : 0.0;
@@ -809,12 +816,12 @@ __global__ __launch_bounds__(256) void adjoint_coef_kernel(AdjCoefArgs a) {
     }
     __syncthreads();
     if (wv < 2) {                     // each wavefront decomposes one basis (no barrier inside)
-        const int info = wave_tridiag_ql(a.m[wv], S.d[wv], S.e[wv], S.Q[wv]);
+        const int info = wave_tridiag_ql(S.mm[wv], S.d[wv], S.e[wv], S.Q[wv]);
         if (lane == 0) S.info[wv] = info;
     }
     __syncthreads();
-    if (wv < 2 && lane < a.m[wv]) {
-        const int b = wv, m = a.m[b];
+    if (wv < 2 && lane < S.mm[wv]) {
+        const int b = wv, m = S.mm[b];
         S.lam[b][lane] = S.d[b][lane];
         S.g[b][lane] = S.Q[b][lane * 64];
         S.p[b][lane] = S.Q[b][lane * 64 + m - 1] * S.Q[b][lane * 64];
@@ -823,7 +830,7 @@ __global__ __launch_bounds__(256) void adjoint_coef_kernel(AdjCoefArgs a) {
     if (tid < 2) {
         const int b = tid;
         double lo = 1e300, hi = -1e300;
-        for (int k = 0; k < a.m[b]; k++) {
+        for (int k = 0; k < S.mm[b]; k++) {
             const double v = (a.from_end[b] ? 1.0 : -1.0) * S.lam[b][k];
             lo = fmin(lo, v);
             hi = fmax(hi, v);
@@ -855,6 +862,8 @@ __global__ __launch_bounds__(256) void adjoint_coef_kernel(AdjCoefArgs a) {
         AdjStep o{};
         o.n_nodes = best_n * best_sub;
         o.n_quad = o.n_nodes;
+        o.m[0] = S.mm[0];
+        o.m[1] = S.mm[1];
         o.order = best_n;
         o.n_sub = best_sub;
         // 2: an eigen-decomposition failed; 1: the rule was capped at kAdjMaxNodes nodes
@@ -885,12 +894,12 @@ __global__ __launch_bounds__(256) void adjoint_coef_kernel(AdjCoefArgs a) {
             out = a.coef_end[b];
         }
         const double arg = a.from_end[b] ? a.h - x : x;
-        adj_coef_wave(S, b, a.m[b], arg, a.scale[b] ? *a.scale[b] : 1.0, out);
+        adj_coef_wave(S, b, S.mm[b], arg, a.scale[b] ? *a.scale[b] : 1.0, out);
     }
     // error integrals h_b int |[exp(y H_m) e1]_{m-1}| over the step, with the same rule
     __syncthreads();
     if (wv < 2) {
-        const int b = wv, m = a.m[b];
+        const int b = wv, m = S.mm[b];
         double acc = 0.0;
         for (int q = 0; q < nn; q++) {
             const double2 gw = a.gl[a.gl_off[order] + q % order];
@@ -925,27 +934,50 @@ __global__ __launch_bounds__(256) void adjoint_coef_kernel(AdjCoefArgs a) {
 // 28 nodes: 1e-14 of the largest at the 15th). The residual left out is exact, not
 // estimated: each column is updated in place and its norm recomputed.
 // One block; the Gram-Schmidt runs in wavefront 0, lane j owning column j of C (in LDS).
+constexpr int kRankChunk = 32;     // nodes staged in LDS at a time while C is formed
+
 __global__ __launch_bounds__(256) void adjoint_rank_kernel(AdjCoefArgs a, double rank_tol) {
     __shared__ double2 C[kMaxKrylov * kMaxKrylov];     // C[i * kMaxKrylov + j]
+    __shared__ double2 As[kRankChunk * kMaxKrylov], Bs[kRankChunk * kMaxKrylov];
     __shared__ double2 qv[kMaxKrylov];
     __shared__ int s_go, s_piv;
     __shared__ double s_inv;
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int nn = a.out->n_quad;
-    const int m0 = a.m[0], m1 = a.m[1];
+    const int m0 = a.out->m[0], m1 = a.out->m[1];
     const double2* A = a.coef[1];
     const double2* B = a.coef[0];
-    for (int e = tid; e < m1 * m0; e += blockDim.x) {
-        const int i = e / m0, j = e % m0;
-        double re = 0.0, im = 0.0;
-        for (int q = 0; q < nn; q++) {
-            const double2 x = A[(size_t)q * kMaxKrylov + i], y = B[(size_t)q * kMaxKrylov + j];
-            const double w = a.w[q];
-            re += w * (x.x * y.x + x.y * y.y);        // w conj(x) y
-            im += w * (x.x * y.y - x.y * y.x);
+    // C = sum_q w_q conj(a_q) b_q^T: thread tid accumulates the entries tid + 256 t (row i =
+    // e / 64, column j = e % 64) over chunks of nodes staged in LDS (w_q folded into a_q)
+    constexpr int kPer = kMaxKrylov * kMaxKrylov / 256;
+    double2 acc[kPer];
+#pragma unroll
+    for (int t = 0; t < kPer; t++) acc[t] = make_double2(0.0, 0.0);
+    for (int q0 = 0; q0 < nn; q0 += kRankChunk) {
+        const int nq = min(kRankChunk, nn - q0);
+        for (int e = tid; e < nq * kMaxKrylov; e += blockDim.x) {
+            const int q = e / kMaxKrylov, i = e % kMaxKrylov;
+            const double w = a.w[q0 + q];
+            const double2 x = i < m1 ? A[(size_t)(q0 + q) * kMaxKrylov + i] : make_double2(0.0, 0.0);
+            As[e] = make_double2(w * x.x, w * x.y);
+            Bs[e] = i < m0 ? B[(size_t)(q0 + q) * kMaxKrylov + i] : make_double2(0.0, 0.0);
         }
-        C[i * kMaxKrylov + j] = make_double2(re, im);
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < kPer; t++) {
+            const int e = tid + 256 * t, i = e / kMaxKrylov, j = e % kMaxKrylov;
+            double re = acc[t].x, im = acc[t].y;
+            for (int q = 0; q < nq; q++) {
+                const double2 x = As[q * kMaxKrylov + i], y = Bs[q * kMaxKrylov + j];
+                re += x.x * y.x + x.y * y.y;          // conj(x) y
+                im += x.x * y.y - x.y * y.x;
+            }
+            acc[t] = make_double2(re, im);
+        }
+        __syncthreads();
     }
+#pragma unroll
+    for (int t = 0; t < kPer; t++) C[tid + 256 * t] = acc[t];
     __syncthreads();
     const int rmax = min(nn, min(m0, m1));
     double thr = 0.0;
