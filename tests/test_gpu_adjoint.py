@@ -173,3 +173,25 @@ def test_adjoint_rank_form_matches_quadrature_nodes(gpu, K, monkeypatch):
     assert i0["node_states"] == i0["nodes"] == i1["nodes"]
     assert i1["node_states"] < i1["nodes"]
     assert rel <= 1e-11
+
+
+@pytest.mark.parametrize("K", [0, 1])
+def test_adjoint_eigen_solvers_agree(gpu, K, monkeypatch):
+    """The coefficient kernel's eigen-decompositions by bisection and twisted factorisations
+    (the default) against the QL iteration (TE_ADJ_EIG=ql, read at every run): both are
+    accurate to about 1e-13 of ||T|| on these Krylov matrices, so the gradients agree far below
+    the 1e-6 bar, and the loss (the forward pass) is the same bit for bit."""
+    L = N = 8
+    T = 2.0
+    theta = _theta(L, 31)
+    prob = ring_problem(gpu, L, N, 0.5, T, tol=1e-8, m=40, checkpoint_interval=K, theta0=theta)
+    psi0 = fock_state(prob.basis.D)
+    l1, g1, _, i1 = prob.loss_and_grad(theta, psi0)
+    monkeypatch.setenv("TE_ADJ_EIG", "ql")
+    l0, g0, _, i0 = prob.loss_and_grad(theta, psi0)
+    rel = np.linalg.norm(np.asarray(g1) - np.asarray(g0)) / np.linalg.norm(g0)
+    print("K=%d bisection/twisted vs QL: rel diff %.2e, node states %d / %d"
+          % (K, rel, i1["node_states"], i0["node_states"]))
+    assert l1 == l0
+    assert i1["nodes"] == i0["nodes"]
+    assert rel <= 1e-10

@@ -989,6 +989,10 @@ static int backward(te_adjoint* a) {
     // The node sums in low-rank form, exact to kAdjRankTol of their norm (adjoint_rank_kernel);
     // TE_ADJ_RANK_TOL sets another tolerance, 0 keeps the quadrature nodes themselves (read at
     // every run).
+    // Eigen-decompositions of the coefficient kernel: bisection and twisted factorisations, QL
+    // for eigenvalues closer than 1e-3 ||T|| (TE_ADJ_EIG=ql: QL always; read at every run).
+    const char* eig_env = std::getenv("TE_ADJ_EIG");
+    const int eig_fast = (eig_env && std::strcmp(eig_env, "ql") == 0) ? 0 : 1;
     double rank_tol = kAdjRankTol;
     if (const char* v = std::getenv("TE_ADJ_RANK_TOL")) rank_tol = std::max(0.0, std::atof(v));
     // One step record per step. In low-rank form a step has at most min(nodes, m) <= 64 node
@@ -1098,6 +1102,7 @@ static int backward(te_adjoint* a) {
         ca.coef_end[1] = a->cend1;
         ca.w = wk;
         ca.out = step;
+        ca.eig_fast = eig_fast;
         launch_adjoint_coef(ca, rank_tol, s);
         int n_nodes = kMatNodes;      // in low-rank form: one pass covers every node state
         if (rank_tol <= 0.0) {

@@ -198,6 +198,10 @@ StepConsts step_consts(const te_evolve_params& P, double tolRate, double numErrE
     C.numErrEst = numErrEst;
     C.n_samples = n_samples;
     C.fastIntegration = P.fast_integration;
+    // eigen-decompositions by bisection and twisted factorisations, QL when eigenvalues lie
+    // close (TE_EIG=ql: QL always; read when an evolver or batch is created)
+    const char* eig = std::getenv("TE_EIG");
+    C.eig_fast = (eig && std::strcmp(eig, "ql") == 0) ? 0 : 1;
     C.ts_rows = q.ts_rows;
     C.ts_nodes = q.ts_nodes;
     C.ts_off = q.ts_off;

@@ -61,6 +61,8 @@ struct StepConsts {
     double t, samplingStep, tolRate, tol, numErrEst;
     long long n_samples;
     int fastIntegration;
+    int eig_fast;                       // 1: bisection and twisted factorisations (QL when
+                                        //    eigenvalues lie close), 0: QL
     int ts_rows;
     const double2* ts_nodes;            // tanh-sinh (complement, weight), rows concatenated
     const int* ts_off;                  // row offsets, ts_rows + 1 entries
@@ -148,6 +150,8 @@ struct AdjCoefArgs {
     double2* coef_end[2];           // kMaxKrylov: the basis vector combination at t_k
     double* w;                      // kAdjMaxNodes quadrature weights (on [t_k, t_{k+1}])
     AdjStep* out;
+    int eig_fast;                   // 1: bisection and twisted factorisations (QL when eigenvalues
+                                    //    lie close), 0: QL
 };
 // The step's rule and node coefficients; with rank_tol > 0 the node sum is then replaced by a
 // low-rank form of at most as many terms (adjoint_rank_kernel), exact to rank_tol relative.

@@ -29,6 +29,7 @@ constexpr int kCtl = 1024;           // threads of the step controller (16 wavef
 constexpr int kPts = 1024;           // integrand points per evaluation batch
 constexpr int kGrp = 8;              // lanes per integrand point
 constexpr int kQB = 32;              // integrals per batch
+constexpr int kTmpStride = 65;       // eigen-solver scratch tmp[i * 65 + k] (no LDS bank conflicts)
 
 struct CtlShared {
     double Q[kMaxKrylov * kMaxKrylov];  // eigenvectors: column k at Q[k * 64 + row]
@@ -36,7 +37,12 @@ struct CtlShared {
     double d[kMaxKrylov], e[kMaxKrylov];
     double p[kMaxKrylov], g[kMaxKrylov], s[kMaxKrylov];
     double2 ph[kMaxKrylov];
-    double xs[kPts], fx[kPts], wt[kPts];
+    union {                            // integration points, or the eigen-solver's scratch
+        struct {
+            double xs[kPts], fx[kPts], wt[kPts];
+        };
+        double T[kMaxKrylov * kTmpStride];
+    };
     // a batch of independent error integrals
     double qa[kQB], qb[kQB], qc[kQB], qd[kQB];
     double qI1[kQB], qL1[kQB], qerr[kQB];
@@ -148,6 +154,151 @@ __device__ int wave_tridiag_ql(int n, double* d, double* e, double* Q) {
     if (lane < n) d[lane] = dr;
     if (lane < n) e[lane] = er;
     return info;
+}
+
+// ---------------------------------------------------------------------------------------
+// The same decomposition with one lane per eigenvalue, for matrices whose eigenvalues are
+// apart: lane k finds the k-th smallest eigenvalue by bisection on the Sturm count (the number
+// of negative pivots of the LDL^T factorisation of T - x I; Golub & Van Loan section 8.4, with
+// the pivot guard of LAPACK's dlaebz), then its eigenvector from the twisted factorisation of
+// T - lambda I: the top-down pivots D+ and bottom-up pivots D- meet at the index r where the twist
+// element gamma_r = D+_r + D-_r - (d_r - lambda) is smallest, and the vector is built outward from
+// x_r = 1 with the ratios -e_i / D+_i below r and -e_{i-1} / D-_i above (Dhillon and Parlett's
+// construction). Every lane runs the same chains, so the wavefront's time is one eigenvalue's
+// instead of the QL's sequence of rotations. An eigenvector built this way is accurate to about
+// n eps ||T|| / gap; when two eigenvalues lie closer than kTwistGap ||T|| the routine returns 1
+// without writing anything and the caller takes wave_tridiag_ql. On success: eigenvalues in d
+// (ascending), eigenvector k in Q[k * 64 + row], e unchanged. tmp: 64 x 65 doubles of LDS.
+// ---------------------------------------------------------------------------------------
+constexpr double kTwistGap = 1e-3;
+
+// 1 / q to within an ulp or two: the hardware reciprocal refined by two Newton steps.
+__device__ __forceinline__ double recip(double q) {
+    double r = __builtin_amdgcn_rcp(q);
+    double t = fma(-q, r, 1.0);
+    r = fma(r, t, r);
+    t = fma(-q, r, 1.0);
+    return fma(r, t, r);
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__device__ int wave_tridiag_bisect(int n, double* d, const double* e, double* Q, double* tmp) {
+    const int lane = threadIdx.x & 63;
+    const bool own = lane < n;
+    // Gershgorin interval, ||T|| and the pivot guard (dlaebz: safe minimum x max(1, max e^2))
+    double glo = 1e300, ghi = -1e300, e2max = 0.0;
+    if (own) {
+        double rad = 0.0;
+        if (lane > 0) rad += fabs(e[lane - 1]);
+        if (lane + 1 < n) {
+            rad += fabs(e[lane]);
+            e2max = e[lane] * e[lane];
+        }
+        glo = d[lane] - rad;
+        ghi = d[lane] + rad;
+    }
+    glo = wave_min(glo);
+    ghi = wave_max(ghi);
+    e2max = wave_max(e2max);
+    const double tnorm = fmax(fabs(glo), fabs(ghi));
+    const double pivmin = DBL_MIN * fmax(1.0, e2max);
+    if (n == 1) {
+        if (lane == 0) Q[0] = 1.0;
+        return 0;
+    }
+    // bisection: count(lo) <= k < count(hi) for k = lane
+    double lo = glo - 2.0 * DBL_EPSILON * tnorm - pivmin, hi = ghi + 2.0 * DBL_EPSILON * tnorm + pivmin;
+    // stop at 2 eps ||T|| (the accuracy the eigenvalues have anyway): at most about 55 halvings
+    const double tol = 2.0 * DBL_EPSILON * tnorm + pivmin;
+    bool active = own;
+    while (__ballot(active)) {
+        const double mid = 0.5 * (lo + hi);
+        if (active && (hi - lo <= tol || mid <= lo || mid >= hi)) active = false;
+        if (__ballot(active) == 0) break;
+        double q = d[0] - mid;
+        if (fabs(q) <= pivmin) q = -pivmin;
+        int c = q < 0.0;
+        for (int i = 1; i < n; i++) {
+            const double ei = e[i - 1];
+            q = (d[i] - mid) - ei * ei * recip(q);
+            if (fabs(q) <= pivmin) q = -pivmin;
+            c += q < 0.0;
+        }
+        if (active) {
+            if (c > lane) hi = mid;
+            else lo = mid;
+        }
+    }
+    const double lam = 0.5 * (lo + hi);
+    // gaps to the neighbouring eigenvalues
+    const double up = __shfl_down(lam, 1, 64), dn = __shfl_up(lam, 1, 64);
+    double gap = 1e300;
+    if (own && lane + 1 < n) gap = fmin(gap, up - lam);
+    if (own && lane > 0) gap = fmin(gap, lam - dn);
+    if (wave_min(gap) < kTwistGap * tnorm) return 1;
+    // bottom-up pivots D-_i into tmp, then top-down pivots D+_i into Q (as scratch, row i of
+    // lane k at Q[i * 64 + k]) with the twist index r = argmin |gamma_i|
+    int r = 0;
+    if (own) {
+        double dm = d[n - 1] - lam;
+        if (fabs(dm) <= pivmin) dm = -pivmin;
+        tmp[(n - 1) * kTmpStride + lane] = dm;
+        for (int i = n - 2; i >= 0; i--) {
+            const double ei = e[i];
+            dm = (d[i] - lam) - ei * ei * recip(dm);
+            if (fabs(dm) <= pivmin) dm = -pivmin;
+            tmp[i * kTmpStride + lane] = dm;
+        }
+        double dp = d[0] - lam;
+        if (fabs(dp) <= pivmin) dp = -pivmin;
+        Q[lane] = dp;
+        double gmin = fabs(tmp[lane]);                       // gamma_0 = D-_0
+        for (int i = 1; i < n; i++) {
+            const double ei = e[i - 1];
+            dp = (d[i] - lam) - ei * ei * recip(dp);
+            if (fabs(dp) <= pivmin) dp = -pivmin;
+            Q[i * 64 + lane] = dp;
+            const double g = fabs(dp + tmp[i * kTmpStride + lane] - (d[i] - lam));
+            if (g < gmin) {
+                gmin = g;
+                r = i;
+            }
+        }
+        // the vector, in tmp: x_r = 1, below r from D+ (Q scratch), above r from D- (tmp)
+        double x = 1.0, s2 = 1.0;
+        tmp[r * kTmpStride + lane] = 1.0;
+        for (int i = r - 1; i >= 0; i--) {
+            x = -e[i] * recip(Q[i * 64 + lane]) * x;
+            tmp[i * kTmpStride + lane] = x;
+            s2 += x * x;
+        }
+        x = 1.0;
+        for (int i = r + 1; i < n; i++) {
+            x = -e[i - 1] * recip(tmp[i * kTmpStride + lane]) * x;
+            tmp[i * kTmpStride + lane] = x;
+            s2 += x * x;
+        }
+        const double inv = 1.0 / sqrt(s2);
+        for (int i = 0; i < n; i++) tmp[i * kTmpStride + lane] *= inv;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // Q[k * 64 + row] = component row of vector k (lane = row); rows past n stay zero
+    for (int k = 0; k < n; k++) Q[k * 64 + lane] = own ? tmp[lane * kTmpStride + k] : 0.0;
+    if (own) d[lane] = lam;
+    return 0;
 }
 
 // f(x) = h |sum_k p_k e^{-i x lambda_k}| at S.xs[0..npts) -> S.fx; kGrp lanes per point.
@@ -630,8 +781,9 @@ __device__ int step_control_body(CtlShared& S, const StepConsts& C, StepState* s
         S.h = ks->h;
     }
     __syncthreads();
-    if (tid < 64) {
-        const int info = wave_tridiag_ql(m, S.d, S.e, S.Q);
+    if (tid < 64) {                                  // bisection, QL when eigenvalues lie close
+        int info = C.eig_fast ? wave_tridiag_bisect(m, S.d, S.e, S.Q, S.T) : 1;
+        if (info) info = wave_tridiag_ql(m, S.d, S.e, S.Q);
         if (tid == 0) S.info = info;
     }
     __syncthreads();
@@ -750,6 +902,7 @@ __global__ __launch_bounds__(kCtl) void batch_step_control_kernel(const StepCons
 // ---------------------------------------------------------------------------------------
 struct AdjShared {
     double Q[2][kMaxKrylov * kMaxKrylov];
+    double T[2][kMaxKrylov * kTmpStride];      // scratch of wave_tridiag_bisect
     double lam[2][kMaxKrylov], d[2][kMaxKrylov], e[2][kMaxKrylov], g[2][kMaxKrylov], s[2][kMaxKrylov];
     double p[2][kMaxKrylov];
     double lo[2], hi[2];
@@ -816,7 +969,8 @@ __global__ __launch_bounds__(256) void adjoint_coef_kernel(AdjCoefArgs a) {
     }
     __syncthreads();
     if (wv < 2) {                     // each wavefront decomposes one basis (no barrier inside)
-        const int info = wave_tridiag_ql(S.mm[wv], S.d[wv], S.e[wv], S.Q[wv]);
+        int info = a.eig_fast ? wave_tridiag_bisect(S.mm[wv], S.d[wv], S.e[wv], S.Q[wv], S.T[wv]) : 1;
+        if (info) info = wave_tridiag_ql(S.mm[wv], S.d[wv], S.e[wv], S.Q[wv]);
         if (lane == 0) S.info[wv] = info;
     }
     __syncthreads();
