@@ -57,7 +57,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2, help="Krylov steps the CPU baseline runs")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-plain measurement")
     ap.add_argument("--batch-count", type=int, default=512)
-    ap.add_argument("--batched-only", action="store_true", help="only the batched measurement")
+    ap.add_argument("--batched-only", action="store_true", help="only the batched measurements")
+    ap.add_argument("--shared-count", type=int, default=64, help="initial states of the shared-matrix batch")
+    ap.add_argument("--shared-size", type=int, default=10, help="L = N of the shared-matrix batch")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-gradient measurement")
     ap.add_argument("--adjoint-only", action="store_true", help="only the adjoint-gradient measurement")
     ap.add_argument("--adjoint-size", type=int, default=12)
@@ -76,7 +78,9 @@ def main():
 
     from timeevolver_amd import capi
     if args.batched_only:
-        print(json.dumps({"batched": bench_batched(np, capi, capi.Context(0), args)}), flush=True)
+        ctx = capi.Context(0)
+        print(json.dumps({"batched": bench_batched(np, capi, ctx, args),
+                          "batched_shared": bench_batched_shared(np, capi, ctx, args)}), flush=True)
         return
     if args.adjoint_only:
         print(json.dumps({"adjoint": bench_adjoint(np, capi, capi.Context(0), args)}), flush=True)
@@ -201,6 +205,7 @@ def main():
     del ev, H, obs, basis
     if not args.no_batched:
         line["batched"] = bench_batched(np, capi, ctx, args)
+        line["batched_shared"] = bench_batched_shared(np, capi, ctx, args)
     if not args.no_adjoint:
         line["adjoint"] = bench_adjoint(np, capi, ctx, args)
     print(json.dumps(line), flush=True)
@@ -298,6 +303,76 @@ def bench_batched(np, capi, ctx, args):
             "kernel_note": "one more run with HIP events around every launch kind (apply and projection with "
                            "their finish launches, controller, combine with observable sums and live count), "
                            "launched kernel by kernel; outside the timed runs"}
+
+
+def bench_batched_shared(np, capi, ctx, args):
+    """Many initial states of one Hamiltonian: `shared_count` seeded random complex states of the
+    ring Bose-Hubbard L = N = `shared_size` (couplings from the bench seed, U = 0.5), T = 5,
+    m = 30, tol = 1e-8, sampling step 0.25, observable n_0, as one batch whose instances share
+    the matrix (served four at a time by one pass over its words), as the same batch with every
+    instance on its own (TE_BATCH_SHARED=0 at creation) and one after the other through the plain
+    evolver. Metric: aggregate Krylov steps per second."""
+    from timeevolver_amd.hamiltonian import as_tuples, createNumberOperator
+    from timeevolver_amd.workloads import random_state, ring_bose_hubbard_terms, ring_couplings
+
+    T, m, tol, step = 5.0, 30, 1e-8, 0.25
+    L = N = args.shared_size
+    eps, J = ring_couplings(L, args.seed)
+    b = capi.Basis.basis(ctx, N, L)
+    H = capi.operator_build(ctx, b, as_tuples(ring_bose_hubbard_terms(L, eps, J, 0.5)))
+    obs = [capi.operator_build(ctx, b, as_tuples([createNumberOperator(0, 1)]))]
+    psis = [random_state(b.D, 7000 + q) for q in range(args.shared_count)]
+    ctx.synchronize()
+    k = len(psis)
+
+    def timed_batch(shared):
+        old = os.environ.get("TE_BATCH_SHARED")
+        os.environ["TE_BATCH_SHARED"] = "1" if shared else "0"
+        try:
+            be = capi.BatchEvolver(ctx, [H] * k, psis, T, step, [obs] * k, tol=tol, m=m)
+        finally:
+            if old is None:
+                del os.environ["TE_BATCH_SHARED"]
+            else:
+                os.environ["TE_BATCH_SHARED"] = old
+        rb = be.run(want_states=False)                   # warmup (tuner, graph capture)
+        times = []
+        for _ in range(max(args.steps, 1)):
+            ctx.synchronize()
+            t = time.perf_counter()
+            rb = be.run(want_states=False)
+            ctx.synchronize()
+            times.append(time.perf_counter() - t)
+        be.set_profiling(True)
+        be.run(want_states=False)
+        kt = be.kernel_times()
+        be.close()
+        return float(np.median(times)), rb, kt
+
+    t_sh, r_sh, kt_sh = timed_batch(True)
+    t_own, r_own, kt_own = timed_batch(False)
+    evs = [capi.Evolver(ctx, H, p, T, step, obs, tol=tol, m=m) for p in psis]
+    evs[0].run(want_state=False)
+    ctx.synchronize()
+    t = time.perf_counter()
+    rp = [ev.run(want_state=False) for ev in evs]
+    ctx.synchronize()
+    t_plain = time.perf_counter() - t
+    for ev in evs:
+        ev.close()
+    steps = sum(r["n_steps"] for r in r_sh)
+    same = all(a["n_steps"] == c["n_steps"] and np.array_equal(a["observables"], c["observables"])
+               for a, c in zip(r_sh, r_own))
+    log("batched, shared matrix: %.4f s (every instance on its own: %.4f s, plain: %.3f s), apply %.1f ms "
+        "against %.1f ms" % (t_sh, t_own, t_plain, kt_sh["apply"][0], kt_own["apply"][0]))
+    return {"workload": "%d random initial states of one ring Bose-Hubbard L=N=%d (D=%d), T=%g, m=%d, tol=%g, "
+                        "sampling step %g, observable n_0" % (k, L, b.D, T, m, tol, step),
+            "metric": "aggregate Krylov steps per second", "unit": "steps/s",
+            "value": round(steps / t_sh, 1), "krylov_steps": steps,
+            "shared_s": round(t_sh, 4), "own_s": round(t_own, 4), "plain_s": round(t_plain, 4),
+            "speedup_vs_own": round(t_own / t_sh, 3), "speedup_vs_plain": round(t_plain / t_sh, 2),
+            "apply_ms_per_run": {"shared": round(kt_sh["apply"][0], 3), "own": round(kt_own["apply"][0], 3)},
+            "bitwise_same_as_own": same}
 
 
 def bench_adjoint(np, capi, ctx, args):

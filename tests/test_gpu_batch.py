@@ -216,3 +216,35 @@ def test_batch_tuned_shape_is_bitwise_the_default(gpu, monkeypatch):
     for a, c in zip(ref, csr[0]):
         assert a["n_steps"] == c["n_steps"] and a["statusCode"] == c["statusCode"]
         assert np.linalg.norm(a["state"] - c["state"]) <= a["err"] + c["err"] + 1e-10
+
+
+def test_batch_shared_matrix_groups(gpu, oracle_lib, monkeypatch):
+    """Instances that share one matrix (many initial states of one Hamiltonian) go through the
+    apply four or two at a time, one pass over the matrix words (groups 4 + 2 + single here,
+    beside two instances of their own): every instance's result is bit for bit the one of the
+    batch with every instance on its own (TE_BATCH_SHARED=0, read at creation), and one of each
+    against the oracle."""
+    L = N = 8
+    eps, J = [0.1 * k for k in range(L)], [0.3 + 0.05 * k for k in range(L)]
+    terms = as_tuples(ring_bose_hubbard_terms(L, eps, J, 0.5))
+    b = capi.Basis.basis(gpu, N, L)
+    H = capi.operator_build(gpu, b, terms)
+    obs = [capi.operator_build(gpu, b, as_tuples([createNumberOperator(0, 1)]))]
+    others = _instances(gpu, _seeded_specs(55, 2, (6, 7)))
+    mats = [H] * 7 + [x[1] for x in others]
+    psis = [random_state(b.D, 300 + q) for q in range(7)] + [x[4] for x in others]
+    obss = [obs] * 7 + [x[2] for x in others]
+    T, step, tol, m = 1.5, 0.25, 1e-8, 30
+    rg = capi.BatchEvolver(gpu, mats, psis, T, step, obss, tol=tol, m=m).run()
+    monkeypatch.setenv("TE_BATCH_SHARED", "0")
+    rs = capi.BatchEvolver(gpu, mats, psis, T, step, obss, tol=tol, m=m).run()
+    for a, c in zip(rg, rs):
+        assert a["n_steps"] == c["n_steps"] and a["statusCode"] == c["statusCode"]
+        assert np.array_equal(a["state"].view(np.uint64), c["state"].view(np.uint64))
+        assert np.array_equal(a["observables"].view(np.uint64), c["observables"].view(np.uint64))
+        assert np.float64(a["err"]).view(np.uint64) == np.float64(c["err"]).view(np.uint64)
+    ro = _oracle_run(oracle_lib, N, L, terms, psis[5], T, step, (0,), tol, m)
+    _check(rg[5], ro, tol)
+    Lo, No, *_ = _seeded_specs(55, 2, (6, 7))[1]
+    ro = _oracle_run(oracle_lib, No, Lo, others[1][3], psis[8], T, step, (0,), tol, m)
+    _check(rg[8], ro, tol)

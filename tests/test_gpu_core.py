@@ -522,6 +522,28 @@ def test_evolver_runs_are_bitwise_repeatable(gpu):
         assert np.array_equal(r["observables"].view(np.uint64), runs[0]["observables"].view(np.uint64))
 
 
+def test_eigen_solvers_agree(gpu, monkeypatch):
+    """The step controller's eigen-decomposition by bisection and twisted factorisations (the
+    default) against the QL iteration (TE_EIG=ql, read when the evolver is created), ring
+    L = N = 10 from a random state over 12 Krylov steps: the same step count and status, states
+    and observables far inside the error bound."""
+    L = N = 10
+    gb = capi.Basis.basis(gpu, N, L)
+    H = capi.operator_build(gpu, gb, as_tuples(_ring_terms(L)))
+    obs = [capi.operator_build(gpu, gb, as_tuples([createNumberOperator(k, 1)])) for k in range(L)]
+    psi = random_state(gb.D, 12)
+    rf = capi.Evolver(gpu, H, psi, 6.0, 0.5, obs, tol=1e-8).run()
+    monkeypatch.setenv("TE_EIG", "ql")
+    rq = capi.Evolver(gpu, H, psi, 6.0, 0.5, obs, tol=1e-8).run()
+    dpsi = np.linalg.norm(rf["state"] - rq["state"])
+    dobs = np.abs(rf["observables"] - rq["observables"]).max()
+    print("steps %d/%d err %.6e/%.6e |dpsi| %.2e max|dobs| %.2e" %
+          (rf["n_steps"], rq["n_steps"], rf["err"], rq["err"], dpsi, dobs))
+    assert rf["n_steps"] == rq["n_steps"] and rf["statusCode"] == rq["statusCode"]
+    assert abs(rf["err"] - rq["err"]) <= 1e-3 * rq["err"]
+    assert dpsi <= 1e-10 and dobs <= 1e-10
+
+
 def test_ring_medium_config_vs_oracle(gpu, oracle_lib):
     """Ring Bose-Hubbard L = N = 12 (D = 1,352,078), seeded random complex initial state, T = 10,
     m = 40, tol = 1e-8, step 0.1 (101 samples of the 12 n_k); the basis and CSR it runs on are

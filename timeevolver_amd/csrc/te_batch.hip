@@ -58,6 +58,7 @@ constexpr int kBatchChunk = 8 * 64;
 // Slices per reduction group of the sliced-ELL apply: a wavefront takes the group's slices one
 // after the other and writes the group's <u_j, w> as one partial. SpMV units are whole groups.
 constexpr int kBatchGroup = 4;
+constexpr int kBatchNV = 4;          // instances of one shared matrix served by one pass over its words
 constexpr int kBatchSC = 8;          // samples per pass of the combine kernel
 constexpr int kBatchMaxObs = 4;      // diagonal observables per instance
 constexpr int kBatchTraceCap = 4096; // recorded step sizes per instance
@@ -80,6 +81,9 @@ struct BatchDev {
     const BatchInst* inst;
     const int4* sunit;      // (instance, first row block, end row block, 0), or slices for sliced ELL
     int n_sunits;
+    const int4* gunit;      // shared-matrix units (first instance, first slice, end slice, g + 1)
+    int n_gunits;
+    const int4* vgroup;     // shared-matrix groups: up to kBatchNV instances (-1 past the group)
     const int4* vunit;      // (instance, first row, end row, 0), rows local to the instance; rows
     int n_vunits;           // start on multiples of kBatchChunk
     double2* U;             // packed Krylov bases, column-major, ld rows per column
@@ -165,10 +169,98 @@ __global__ __launch_bounds__(kBlock) void batch_apply_kernel(BatchDev a, int j) 
 // differs from the block's previous one. A wavefront takes a group of kBatchGroup consecutive
 // slices; the group's <u_j, w> goes to the group's partial (a wavefront sum), whatever unit or
 // block computed it.
+// The apply epilogue of one instance in a shared-matrix pass; an instance that takes no part
+// in the iteration (finished, broken down, or past its Krylov dimension) is carried along with
+// its loads and stores switched off.
+struct ApplyEpiOpt {
+    using Pre = double2;
+    ApplyEpiVd e;
+    bool on;
+    __device__ __forceinline__ Pre load(int r) const { return on ? e.load(r) : make_double2(0.0, 0.0); }
+    __device__ __forceinline__ void row(int r, double2 sum, double2 u, Pre p) {
+        if (on) e.row(r, sum, u, p);
+    }
+};
+
+__device__ __forceinline__ ApplyEpiVd batch_apply_epi(const BatchDev& a, int i, int j) {
+    const BatchInst& I = a.inst[i];
+    const KrylovScalars* ks = a.ks + i;
+    const double sj = ks->s[j];
+    ApplyEpiVd epi;
+    epi.c1 = make_double2(a.alpha.x * sj, a.alpha.y * sj);
+    epi.c2 = j > 0 ? ks->beta[j - 1] * ks->s[j - 1] : 0.0;
+    epi.up = j > 0 ? a.U + (size_t)(j - 1) * a.ld + I.off : nullptr;
+    epi.w = a.U + (size_t)(j + 1) * a.ld + I.off;
+    epi.dot = make_double2(0.0, 0.0);
+    return epi;
+}
+
+// A unit of shared-matrix group g: NV instances (slots of the group past its size, or
+// instances not in this iteration, switched off) in one pass over the matrix words; each
+// instance's rows, sums and group partials are those of a pass of its own.
+template <int NV>
+__device__ void batch_apply_group(const BatchDev& a, int j, int4 su, const double2* vt, int lane, int wv) {
+    const int4 gi = a.vgroup[su.w - 1];
+    const int ids[4] = {gi.x, gi.y, gi.z, gi.w};
+    const SellView m = a.inst[gi.x].sell;
+    ApplyEpiOpt epi[NV];
+    const double2* xs[NV];
+    int sl0[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+        const int i = ids[v];
+        const bool on = i >= 0 && lanczos_active(a, i, j);
+        const int ii = i >= 0 ? i : gi.x;
+        epi[v].e = batch_apply_epi(a, ii, j);
+        epi[v].on = on;
+        xs[v] = a.U + (size_t)j * a.ld + a.inst[ii].off;
+        sl0[v] = a.inst[ii].sl0;
+    }
+    const int g1 = (su.z + kBatchGroup - 1) / kBatchGroup;
+    for (int g = su.y / kBatchGroup + wv; g < g1; g += kBlock / 64) {
+        const int gw = __builtin_amdgcn_readfirstlane(g);
+        const int se = min((gw + 1) * kBatchGroup, su.z);
+#pragma unroll
+        for (int v = 0; v < NV; v++) epi[v].e.dot = make_double2(0.0, 0.0);
+        for (int sl = gw * kBatchGroup; sl < se; sl++) sell_slice_n<NV>(m, xs, vt, sl, lane, epi);
+#pragma unroll
+        for (int v = 0; v < NV; v++) {
+            const double dx = wave_sum(epi[v].e.dot.x), dy = wave_sum(epi[v].e.dot.y);
+            if (lane == 0 && epi[v].on) a.spart[sl0[v] + gw] = make_double2(dx, dy);
+        }
+    }
+}
+
+// Units of shared-matrix groups (a launch of their own: the four-vector pass needs about twice
+// the registers of the single one).
+__global__ __launch_bounds__(kBlock) void batch_apply_group_kernel(BatchDev a, int j) {
+    __shared__ double2 vt[kVdMaxValues];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const double2* staged = nullptr;
+    int u0, u1, us;
+    xcd_sweep(a.n_gunits, u0, u1, us);
+    for (int u = u0; u < u1; u += us) {
+        const int4 su = a.gunit[u];
+        const int4 gi = a.vgroup[su.w - 1];
+        const bool any = lanczos_active(a, gi.x, j) || (gi.y >= 0 && lanczos_active(a, gi.y, j)) ||
+                         (gi.z >= 0 && lanczos_active(a, gi.z, j)) || (gi.w >= 0 && lanczos_active(a, gi.w, j));
+        if (!any) continue;
+        const SellView& mg = a.inst[gi.x].sell;
+        if (mg.vtab != staged) {
+            __syncthreads();                             // every wavefront is done with the old table
+            for (int q = threadIdx.x; q < mg.n_values; q += kBlock) vt[q] = mg.vtab[q];
+            __syncthreads();
+            staged = mg.vtab;
+        }
+        if (gi.z >= 0) batch_apply_group<4>(a, j, su, vt, lane, wv);
+        else batch_apply_group<2>(a, j, su, vt, lane, wv);
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void batch_apply_sell_kernel(BatchDev a, int j) {
     __shared__ double2 vt[kVdMaxValues];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    int staged = -1;
+    const double2* staged = nullptr;
     int u0, u1, us;
     xcd_sweep(a.n_sunits, u0, u1, us);
     for (int u = u0; u < u1; u += us) {
@@ -177,11 +269,11 @@ __global__ __launch_bounds__(kBlock) void batch_apply_sell_kernel(BatchDev a, in
         if (!lanczos_active(a, i, j)) continue;
         const BatchInst& I = a.inst[i];
         const SellView m = I.sell;
-        if (i != staged) {
+        if (m.vtab != staged) {
             __syncthreads();                             // every wavefront is done with the old table
             for (int q = threadIdx.x; q < m.n_values; q += kBlock) vt[q] = m.vtab[q];
             __syncthreads();
-            staged = i;
+            staged = m.vtab;
         }
         KrylovScalars* ks = a.ks + i;
         const double sj = ks->s[j];
@@ -473,10 +565,16 @@ struct te_batch {
     double tune_ms = 0.0;
     uint64_t shape_key = 0;
     std::vector<const int*> perm, iperm;
+    // shared-matrix groups (sliced ELL): instances with the same matrix, up to kBatchNV a group
+    // (sizes 4 and 2; a single instance keeps its own units); group_of[i] = g, or -1
+    std::vector<int4> groups;
+    std::vector<int> group_of;
+    int4* vgroup = nullptr;
     double2* tmp = nullptr;            // packed staging vector (reordering)
     // device buffers
     BatchInst* inst = nullptr;
-    int4 *sunit = nullptr, *vunit = nullptr;
+    int4 *sunit = nullptr, *gunit = nullptr, *vunit = nullptr;
+    int grid_g = 8;
     double2 *U = nullptr, *psi0 = nullptr, *sampled = nullptr, *wcoef = nullptr, *Y = nullptr;
     double2 *spart = nullptr, *vpart = nullptr;
     KrylovScalars* ks = nullptr;
@@ -511,7 +609,7 @@ static void batch_free(te_batch* b) {
         for (hipEvent_t e : v) hipEventDestroy(e);
     if (b->exec) hipGraphExecDestroy(b->exec);
     if (b->graph) hipGraphDestroy(b->graph);
-    void* ptrs[] = {b->inst, b->sunit,   b->vunit,  b->U,  b->psi0, b->sampled, b->wcoef, b->Y, b->tmp,
+    void* ptrs[] = {b->inst, b->sunit,   b->gunit, b->vgroup, b->vunit,  b->U,  b->psi0, b->sampled, b->wcoef, b->Y, b->tmp,
                     b->spart, b->vpart,  b->ks, b->st, b->bc, b->consts,
                     b->diag, b->opart,   b->obs,    b->nrm, b->trace, b->n_live};
     for (void* p : ptrs)
@@ -554,14 +652,24 @@ static void prof_collect(te_batch* b) {
     }
 }
 
+// The apply of iteration j for every instance: single-instance units, then shared-matrix units.
+static void launch_batch_apply(const te_batch* b, hipStream_t s, int j) {
+    const BatchDev& d = b->dev;
+    if (!b->sell) {
+        hipLaunchKernelGGL(batch_apply_kernel, dim3(b->grid_s), dim3(kBlock), 0, s, d, j);
+        return;
+    }
+    if (d.n_sunits > 0) hipLaunchKernelGGL(batch_apply_sell_kernel, dim3(b->grid_s), dim3(kBlock), 0, s, d, j);
+    if (d.n_gunits > 0) hipLaunchKernelGGL(batch_apply_group_kernel, dim3(b->grid_g), dim3(kBlock), 0, s, d, j);
+}
+
 // Enqueues one round on the context's stream (events only when timing is on, which also
 // keeps the round out of a graph).
 static void batch_round(te_batch* b, hipStream_t s) {
     const BatchDev& d = b->dev;
     for (int j = 0; j < b->m_max; j++) {
         prof_begin(b, 0, s);
-        if (b->sell) hipLaunchKernelGGL(batch_apply_sell_kernel, dim3(b->grid_s), dim3(kBlock), 0, s, d, j);
-        else hipLaunchKernelGGL(batch_apply_kernel, dim3(b->grid_s), dim3(kBlock), 0, s, d, j);
+        launch_batch_apply(b, s, j);
         hipLaunchKernelGGL(batch_finish_kernel<false>, dim3(b->n_inst), dim3(kBlock), 0, s, d, j);
         prof_end(b, 0, s);
         prof_begin(b, 1, s);
@@ -631,13 +739,18 @@ static void capture_round(te_batch* b, hipStream_t s) {
 // the CSR path keeps its fixed kBatchRB-row-block units), vector units of vec_rows rows (a
 // multiple of kBatchChunk) of one instance.
 static void shape_units(const te_batch* b, int unit_slices, int vec_rows, std::vector<int4>& su,
-                        std::vector<int4>& vu) {
+                        std::vector<int4>& gu, std::vector<int4>& vu) {
     su.clear();
+    gu.clear();
     vu.clear();
     for (int i = 0; i < b->n_inst; i++) {
-        if (b->sell)
+        const int g = b->group_of.empty() ? -1 : b->group_of[i];
+        if (b->sell && g < 0)
             for (int sl = 0; sl < b->n_slices[i]; sl += unit_slices)
                 su.push_back(make_int4(i, sl, std::min(sl + unit_slices, b->n_slices[i]), 0));
+        if (b->sell && g >= 0 && b->groups[g].x == i)    // a group's units go with its first instance
+            for (int sl = 0; sl < b->n_slices[i]; sl += unit_slices)
+                gu.push_back(make_int4(i, sl, std::min(sl + unit_slices, b->n_slices[i]), g + 1));
         for (int r = 0; r < b->dim[i]; r += vec_rows) vu.push_back(make_int4(i, r, std::min(r + vec_rows, b->dim[i]), 0));
     }
 }
@@ -652,19 +765,23 @@ static int shape_grid(const te_batch* b, int items, int blocks_per_cu) {
 static int apply_shape(te_batch* b, int unit_slices, int vec_rows, int spmv_bpc, int vec_bpc) {
     te_ctx* ctx = b->ctx;
     hipStream_t s = ctx->stream;
-    std::vector<int4> su, vu;
-    shape_units(b, unit_slices, vec_rows, su, vu);
-    if (b->sell) TE_HIP(ctx, hipMemcpyAsync(b->sunit, su.data(), sizeof(int4) * su.size(), hipMemcpyHostToDevice, s));
+    std::vector<int4> su, gu, vu;
+    shape_units(b, unit_slices, vec_rows, su, gu, vu);
+    if (b->sell && !su.empty())
+        TE_HIP(ctx, hipMemcpyAsync(b->sunit, su.data(), sizeof(int4) * su.size(), hipMemcpyHostToDevice, s));
+    if (!gu.empty()) TE_HIP(ctx, hipMemcpyAsync(b->gunit, gu.data(), sizeof(int4) * gu.size(), hipMemcpyHostToDevice, s));
     TE_HIP(ctx, hipMemcpyAsync(b->vunit, vu.data(), sizeof(int4) * vu.size(), hipMemcpyHostToDevice, s));
     // the host vectors must outlive the copies
     TE_HIP(ctx, hipStreamSynchronize(s));
     if (b->sell) b->dev.n_sunits = (int)su.size();
+    b->dev.n_gunits = (int)gu.size();
     b->dev.n_vunits = (int)vu.size();
     b->unit_slices = b->sell ? unit_slices : 0;
     b->vec_rows = vec_rows;
     b->spmv_bpc = spmv_bpc;
     b->vec_bpc = vec_bpc;
     b->grid_s = shape_grid(b, b->dev.n_sunits, spmv_bpc);
+    b->grid_g = shape_grid(b, b->dev.n_gunits, spmv_bpc);
     b->grid_v = shape_grid(b, b->dev.n_vunits, vec_bpc);
     if (b->exec || b->graph) {
         if (b->exec) hipGraphExecDestroy(b->exec);
@@ -700,10 +817,7 @@ static int tune_shape(te_batch* b) {
         for (int r = 0; r <= reps; r++) {
             TE_HIP(ctx, hipEventRecord(ev[2 * r], s));
             if (spmv) {
-                if (b->sell)
-                    hipLaunchKernelGGL(batch_apply_sell_kernel, dim3(b->grid_s), dim3(kBlock), 0, s, b->dev, 0);
-                else
-                    hipLaunchKernelGGL(batch_apply_kernel, dim3(b->grid_s), dim3(kBlock), 0, s, b->dev, 0);
+                launch_batch_apply(b, s, 0);
                 hipLaunchKernelGGL(batch_finish_kernel<false>, dim3(b->n_inst), dim3(kBlock), 0, s, b->dev, 0);
             } else {
                 hipLaunchKernelGGL(batch_project_kernel, dim3(b->grid_v), dim3(kBlock), 0, s, b->dev, 0);
@@ -917,6 +1031,35 @@ extern "C" int te_batch_create(te_ctx* ctx, int n_inst, te_matrix* const* H, con
         b->total_chunks += I.nch;
     }
     b->ld = total;
+    // Instances that share a matrix (the same te_matrix) are served kBatchNV at a time by one pass
+    // over its words (TE_BATCH_SHARED=0: every instance on its own); the results are the same bits.
+    b->group_of.assign(n_inst, -1);
+    if (b->sell) {
+        const char* sh = std::getenv("TE_BATCH_SHARED");
+        if (!(sh && sh[0] == '0')) {
+            std::map<const te_matrix*, std::vector<int>> by;
+            for (int i = 0; i < n_inst; i++) by[H[i]].push_back(i);
+            std::vector<std::vector<int>> sets;
+            for (int i = 0; i < n_inst; i++) {
+                auto it = by.find(H[i]);
+                if (it == by.end()) continue;
+                sets.push_back(it->second);
+                by.erase(it);
+            }
+            const char* nv_env = std::getenv("TE_BATCH_NV");            // group size sweeps: 2 or 4
+            const size_t nv_max = (nv_env && std::atoi(nv_env) == 2) ? 2 : kBatchNV;
+            for (const std::vector<int>& v : sets) {
+                size_t k = 0;
+                while (v.size() - k >= 2) {
+                    const size_t take = v.size() - k >= nv_max ? nv_max : 2;
+                    int4 g = make_int4(v[k], v[k + 1], take > 2 ? v[k + 2] : -1, take > 3 ? v[k + 3] : -1);
+                    for (size_t q = 0; q < take; q++) b->group_of[v[k + q]] = (int)b->groups.size();
+                    b->groups.push_back(g);
+                    k += take;
+                }
+            }
+        }
+    }
     if (b->sell) {
         int64_t per = b->total_slices / std::max(8, ctx->num_cus * b->spmv_bpc);
         b->unit_slices = (int)std::max<int64_t>(kBatchSlicesMin, std::min<int64_t>(kBatchSlicesMax, per)) /
@@ -946,6 +1089,7 @@ extern "C" int te_batch_create(te_ctx* ctx, int n_inst, te_matrix* const* H, con
             mix((uint64_t)p.second);
         }
         mix(b->sell ? 1u : 0u);
+        mix((uint64_t)b->groups.size());
         mix((uint64_t)ctx->device);
         mix((uint64_t)ctx->num_cus);
         b->shape_key = h;
@@ -956,6 +1100,8 @@ extern "C" int te_batch_create(te_ctx* ctx, int n_inst, te_matrix* const* H, con
     // one chunk
     TE_TRY(balloc(ctx, &b->sunit, b->sell ? (size_t)b->total_groups : hsu.size()));
     TE_TRY(balloc(ctx, &b->vunit, (size_t)b->total_chunks));
+    TE_TRY(balloc(ctx, &b->vgroup, std::max<size_t>(b->groups.size(), 1)));
+    TE_TRY(balloc(ctx, &b->gunit, b->sell ? (size_t)b->total_groups : 1));
     TE_TRY(balloc(ctx, &b->U, (size_t)total * (b->m_max + 1)));
     TE_TRY(balloc(ctx, &b->psi0, (size_t)total));
     TE_TRY(balloc(ctx, &b->sampled, (size_t)total));
@@ -981,6 +1127,9 @@ extern "C" int te_batch_create(te_ctx* ctx, int n_inst, te_matrix* const* H, con
     TE_TRYH(hipMemcpyAsync(b->inst, hinst.data(), sizeof(BatchInst) * n_inst, hipMemcpyHostToDevice, s));
     if (!b->sell)
         TE_TRYH(hipMemcpyAsync(b->sunit, hsu.data(), sizeof(int4) * hsu.size(), hipMemcpyHostToDevice, s));
+    if (!b->groups.empty())
+        TE_TRYH(hipMemcpyAsync(b->vgroup, b->groups.data(), sizeof(int4) * b->groups.size(), hipMemcpyHostToDevice,
+                               s));
     TE_TRYH(hipMemsetAsync(b->psi0, 0, sizeof(double2) * total, s));
     TE_TRYH(hipMemsetAsync(b->diag, 0, sizeof(double) * std::max(n_obs, 1) * total, s));
     std::vector<StepConsts> hc(n_inst);
@@ -1025,6 +1174,8 @@ extern "C" int te_batch_create(te_ctx* ctx, int n_inst, te_matrix* const* H, con
     BatchDev& d = b->dev;
     d.inst = b->inst;
     d.sunit = b->sunit;
+    d.vgroup = b->vgroup;
+    d.gunit = b->gunit;
     d.n_sunits = (int)hsu.size();
     d.vunit = b->vunit;
     d.U = b->U;
