@@ -209,7 +209,8 @@ typedef struct te_batch te_batch;
  * observables obs[i * n_obs + o] (diagonal TE_OBS_SPARSE matrices only; n_obs <= 4). One
  * parameter set for all. Validation per instance as the constructor: TE_ERR_FATAL names the
  * instance whose state is not normalised (:45-48). Matrices are borrowed (must outlive the
- * batch). */
+ * batch). Instances given the same matrix share its passes: four (or two) of them are served by
+ * one pass over its words, with the same result bits (TE_BATCH_SHARED=0: each on its own). */
 int te_batch_create(te_ctx* ctx, int n_inst, te_matrix* const* H, const double* const* psi0,
                     const te_evolve_params* params, const te_observable* obs, int n_obs, te_batch** out);
 void te_batch_destroy(te_batch* b);
