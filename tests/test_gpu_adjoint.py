@@ -195,3 +195,20 @@ def test_adjoint_eigen_solvers_agree(gpu, K, monkeypatch):
     assert l1 == l0
     assert i1["nodes"] == i0["nodes"]
     assert rel <= 1e-10
+
+
+def test_adjoint_krylov_dimension_above_40(gpu):
+    """m = 48 (the coefficient and rank kernels' wider form) against the dense Frechet gradient."""
+    L = N = 6
+    T = 3.0
+    theta = _theta(L, 19)
+    prob = ring_problem(gpu, L, N, 0.5, T, tol=1e-8, m=48, checkpoint_interval=1, theta0=theta)
+    psi0 = fock_state(prob.basis.D)
+    loss, grad, res, info = prob.loss_and_grad(theta, psi0)
+    l_ref, g_ref = _dense_grad(prob, theta, psi0, T)
+    rel = np.linalg.norm(grad - g_ref) / np.linalg.norm(g_ref)
+    print("m=48: steps %d nodes %d node states %d rel err %.2e" % (info["n_steps"], info["nodes"],
+                                                                   info["node_states"], rel))
+    assert res["krylovDim"] == 48
+    assert abs(loss - l_ref) <= 2 * N * res["err"] + 1e-12
+    assert rel <= 1e-6

@@ -78,8 +78,9 @@ struct MatArgs {
 constexpr int kMatRows = 4;        // rows formed by one lane
 constexpr int kMatTile = 64 * kMatRows;
 
+template <int MS>                  // the largest Krylov dimension of the launch (LDS table rows)
 __global__ __launch_bounds__(256) void materialise_nodes_kernel(MatArgs a) {
-    __shared__ double2 cs[kMaxKrylov * (kMatNodes + 1)];      // cs[j * (nq + 1) + q], q = nq: cend
+    __shared__ double2 cs[MS * (kMatNodes + 1)];              // cs[j * (nq + 1) + q], q = nq: cend
     const int nn = a.step->n_nodes;
     const int m = a.step->m[a.basis];
     const int nloc = max(0, min(kMatNodes, nn - a.node0));
@@ -1119,8 +1120,15 @@ static int backward(te_adjoint* a) {
                        (node0 == 0 && a->K >= 1) ? a->psi_next : nullptr};
             MatArgs ml{a->Vl, n, n, 1, a->coef1, a->cend1, step, node0, lam_nodes,
                        node0 == 0 ? a->lam_next : nullptr};
-            hipLaunchKernelGGL(materialise_nodes_kernel, dim3(mg), dim3(256), 0, s, mp);
-            hipLaunchKernelGGL(materialise_nodes_kernel, dim3(mg), dim3(256), 0, s, ml);
+            // m <= m0: a 40-row coefficient table (41.6 KB) lets the blocks share CUs with the
+            // term-wise pass's
+            if (a->m0 <= 40) {
+                hipLaunchKernelGGL(materialise_nodes_kernel<40>, dim3(mg), dim3(256), 0, s, mp);
+                hipLaunchKernelGGL(materialise_nodes_kernel<40>, dim3(mg), dim3(256), 0, s, ml);
+            } else {
+                hipLaunchKernelGGL(materialise_nodes_kernel<kMaxKrylov>, dim3(mg), dim3(256), 0, s, mp);
+                hipLaunchKernelGGL(materialise_nodes_kernel<kMaxKrylov>, dim3(mg), dim3(256), 0, s, ml);
+            }
             if (overlap) {
                 pend = term_args(step, wk, node0, psi_nodes, lam_nodes);
                 pending = true;

@@ -1088,12 +1088,15 @@ __global__ __launch_bounds__(256) void adjoint_coef_kernel(AdjCoefArgs a) {
 // 28 nodes: 1e-14 of the largest at the 15th). The residual left out is exact, not
 // estimated: each column is updated in place and its norm recomputed.
 // One block; the Gram-Schmidt runs in wavefront 0, lane j owning column j of C (in LDS).
-constexpr int kRankChunk = 32;     // nodes staged in LDS at a time while C is formed
+constexpr int kRankChunk = 8;      // nodes staged in LDS at a time while C is formed
 
+// MS: the largest Krylov dimension the launch serves (the LDS stride of C and the staged
+// nodes): 40 keeps the block at 36 KB of LDS, which fits beside the term-wise pass's blocks.
+template <int MS>
 __global__ __launch_bounds__(256) void adjoint_rank_kernel(AdjCoefArgs a, double rank_tol) {
-    __shared__ double2 C[kMaxKrylov * kMaxKrylov];     // C[i * kMaxKrylov + j]
-    __shared__ double2 As[kRankChunk * kMaxKrylov], Bs[kRankChunk * kMaxKrylov];
-    __shared__ double2 qv[kMaxKrylov];
+    __shared__ double2 C[MS * MS];                       // C[i * MS + j]
+    __shared__ double2 As[kRankChunk * MS], Bs[kRankChunk * MS];
+    __shared__ double2 qv[MS];
     __shared__ int s_go, s_piv;
     __shared__ double s_inv;
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
@@ -1102,15 +1105,15 @@ __global__ __launch_bounds__(256) void adjoint_rank_kernel(AdjCoefArgs a, double
     const double2* A = a.coef[1];
     const double2* B = a.coef[0];
     // C = sum_q w_q conj(a_q) b_q^T: thread tid accumulates the entries tid + 256 t (row i =
-    // e / 64, column j = e % 64) over chunks of nodes staged in LDS (w_q folded into a_q)
-    constexpr int kPer = kMaxKrylov * kMaxKrylov / 256;
+    // e / MS, column j = e % MS) over chunks of nodes staged in LDS (w_q folded into a_q)
+    constexpr int kPer = (MS * MS + 255) / 256;
     double2 acc[kPer];
 #pragma unroll
     for (int t = 0; t < kPer; t++) acc[t] = make_double2(0.0, 0.0);
     for (int q0 = 0; q0 < nn; q0 += kRankChunk) {
         const int nq = min(kRankChunk, nn - q0);
-        for (int e = tid; e < nq * kMaxKrylov; e += blockDim.x) {
-            const int q = e / kMaxKrylov, i = e % kMaxKrylov;
+        for (int e = tid; e < nq * MS; e += blockDim.x) {
+            const int q = e / MS, i = e % MS;
             const double w = a.w[q0 + q];
             const double2 x = i < m1 ? A[(size_t)(q0 + q) * kMaxKrylov + i] : make_double2(0.0, 0.0);
             As[e] = make_double2(w * x.x, w * x.y);
@@ -1119,10 +1122,11 @@ __global__ __launch_bounds__(256) void adjoint_rank_kernel(AdjCoefArgs a, double
         __syncthreads();
 #pragma unroll
         for (int t = 0; t < kPer; t++) {
-            const int e = tid + 256 * t, i = e / kMaxKrylov, j = e % kMaxKrylov;
+            const int e = tid + 256 * t, i = e / MS, j = e % MS;
+            if (e >= MS * MS) break;
             double re = acc[t].x, im = acc[t].y;
             for (int q = 0; q < nq; q++) {
-                const double2 x = As[q * kMaxKrylov + i], y = Bs[q * kMaxKrylov + j];
+                const double2 x = As[q * MS + i], y = Bs[q * MS + j];
                 re += x.x * y.x + x.y * y.y;          // conj(x) y
                 im += x.x * y.y - x.y * y.x;
             }
@@ -1131,7 +1135,8 @@ __global__ __launch_bounds__(256) void adjoint_rank_kernel(AdjCoefArgs a, double
         __syncthreads();
     }
 #pragma unroll
-    for (int t = 0; t < kPer; t++) C[tid + 256 * t] = acc[t];
+    for (int t = 0; t < kPer; t++)
+        if (tid + 256 * t < MS * MS) C[tid + 256 * t] = acc[t];
     __syncthreads();
     const int rmax = min(nn, min(m0, m1));
     double thr = 0.0;
@@ -1141,7 +1146,7 @@ __global__ __launch_bounds__(256) void adjoint_rank_kernel(AdjCoefArgs a, double
             double nrm = 0.0;
             if (lane < m0)
                 for (int i = 0; i < m1; i++) {
-                    const double2 c = C[i * kMaxKrylov + lane];
+                    const double2 c = C[i * MS + lane];
                     nrm += c.x * c.x + c.y * c.y;
                 }
             const double tot = wave_sum(nrm);
@@ -1166,7 +1171,7 @@ __global__ __launch_bounds__(256) void adjoint_rank_kernel(AdjCoefArgs a, double
         __syncthreads();
         if (!s_go) break;
         if (wv == 0 && lane < m1) {
-            const double2 c = C[lane * kMaxKrylov + s_piv];
+            const double2 c = C[lane * MS + s_piv];
             qv[lane] = make_double2(c.x * s_inv, c.y * s_inv);
         }
         __syncthreads();
@@ -1174,13 +1179,13 @@ __global__ __launch_bounds__(256) void adjoint_rank_kernel(AdjCoefArgs a, double
             if (lane < m0) {
                 double rr = 0.0, ri = 0.0;
                 for (int i = 0; i < m1; i++) {         // R = <q, C[:, lane]>
-                    const double2 q = qv[i], c = C[i * kMaxKrylov + lane];
+                    const double2 q = qv[i], c = C[i * MS + lane];
                     rr += q.x * c.x + q.y * c.y;
                     ri += q.x * c.y - q.y * c.x;
                 }
                 for (int i = 0; i < m1; i++) {         // C[:, lane] -= q R
                     const double2 q = qv[i];
-                    double2& c = C[i * kMaxKrylov + lane];
+                    double2& c = C[i * MS + lane];
                     c.x -= q.x * rr - q.y * ri;
                     c.y -= q.x * ri + q.y * rr;
                 }
@@ -1197,7 +1202,10 @@ __global__ __launch_bounds__(256) void adjoint_rank_kernel(AdjCoefArgs a, double
 
 void launch_adjoint_coef(const AdjCoefArgs& a, double rank_tol, hipStream_t s) {
     hipLaunchKernelGGL(adjoint_coef_kernel, dim3(1), dim3(256), 0, s, a);
-    if (rank_tol > 0.0) hipLaunchKernelGGL(adjoint_rank_kernel, dim3(1), dim3(256), 0, s, a, rank_tol);
+    if (rank_tol > 0.0) {
+        if (std::max(a.m[0], a.m[1]) <= 40) hipLaunchKernelGGL(adjoint_rank_kernel<40>, dim3(1), dim3(256), 0, s, a, rank_tol);
+        else hipLaunchKernelGGL(adjoint_rank_kernel<kMaxKrylov>, dim3(1), dim3(256), 0, s, a, rank_tol);
+    }
 }
 
 // ---------------------------------------------------------------------------------------
