@@ -437,6 +437,7 @@ static void adj_free_bases(te_adjoint* a) {
 static void adj_free(te_adjoint* a) {
     if (!a) return;
     hipSetDevice(a->ctx->device);
+    if (a->s2) hipStreamSynchronize(a->s2);        // no term-wise pass left reading the buffers
     adj_free_bases(a);
     void* ptrs[] = {a->soff, a->scol, a->slab, a->svid, a->sval, a->slen, a->vtab, a->gdiag,  a->odiag,  a->U,      a->psiT,
                     a->scratch, a->psi0, a->ks,   a->ks_hist, a->st,  a->wcoef, a->Y,      a->trace,
@@ -949,6 +950,8 @@ static int forward(te_adjoint* a, te_result* R) {
 static int backward(te_adjoint* a) {
     te_ctx* ctx = a->ctx;
     hipStream_t s = ctx->stream;
+    // a previous run that ended early on an error may have left a term-wise pass queued there
+    TE_HIP(ctx, hipStreamSynchronize(a->s2));
     const int64_t n = a->n;
     const double2 alpha_b = make_double2(0.0, 1.0 * a->prm.exp_factor);   // +i e H: evolution back in time
     const int nsteps = (int)a->t_steps.size();

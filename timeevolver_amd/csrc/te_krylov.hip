@@ -196,6 +196,9 @@ __device__ __forceinline__ double wave_max(double v) {
 __device__ int wave_tridiag_bisect(int n, double* d, const double* e, double* Q, double* tmp) {
     const int lane = threadIdx.x & 63;
     const bool own = lane < n;
+    // a non-finite entry goes to QL, which reports it through its iteration cap
+    const bool bad = own && (!isfinite(d[lane]) || (lane + 1 < n && !isfinite(e[lane])));
+    if (__ballot(bad)) return 1;
     // Gershgorin interval, ||T|| and the pivot guard (dlaebz: safe minimum x max(1, max e^2))
     double glo = 1e300, ghi = -1e300, e2max = 0.0;
     if (own) {
@@ -222,7 +225,11 @@ __device__ int wave_tridiag_bisect(int n, double* d, const double* e, double* Q,
     // stop at 2 eps ||T|| (the accuracy the eigenvalues have anyway): at most about 55 halvings
     const double tol = 2.0 * DBL_EPSILON * tnorm + pivmin;
     bool active = own;
+    int iter = 0;
     while (__ballot(active)) {
+        // at most 2 + log2(range / tol) halvings; a NaN in T never meets the tests: give up
+        // after 256 and leave the matrix to QL (whose own iteration cap reports the failure)
+        if (++iter > 256) return 1;
         const double mid = 0.5 * (lo + hi);
         if (active && (hi - lo <= tol || mid <= lo || mid >= hi)) active = false;
         if (__ballot(active) == 0) break;
