@@ -53,6 +53,7 @@ struct CtlShared {
     double h;
     int m;
     int info;
+    int eflag;
 };
 
 // sqrt(a^2 + b^2) without the scaling steps of hypot when no overflow or underflow can occur.
@@ -193,13 +194,17 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
-__device__ int wave_tridiag_bisect(int n, double* d, const double* e, double* Q, double* tmp) {
+// Gershgorin interval of T, ||T|| and the pivot guard of the Sturm recurrence (dlaebz: safe
+// minimum x max(1, max e^2)), over the calling wavefront; bad: a non-finite entry.
+struct TridiagBounds {
+    double glo, ghi, tnorm, pivmin;
+    bool bad;
+};
+
+__device__ TridiagBounds wave_tridiag_bounds(int n, const double* d, const double* e) {
     const int lane = threadIdx.x & 63;
     const bool own = lane < n;
-    // a non-finite entry goes to QL, which reports it through its iteration cap
     const bool bad = own && (!isfinite(d[lane]) || (lane + 1 < n && !isfinite(e[lane])));
-    if (__ballot(bad)) return 1;
-    // Gershgorin interval, ||T|| and the pivot guard (dlaebz: safe minimum x max(1, max e^2))
     double glo = 1e300, ghi = -1e300, e2max = 0.0;
     if (own) {
         double rad = 0.0;
@@ -211,43 +216,35 @@ __device__ int wave_tridiag_bisect(int n, double* d, const double* e, double* Q,
         glo = d[lane] - rad;
         ghi = d[lane] + rad;
     }
-    glo = wave_min(glo);
-    ghi = wave_max(ghi);
-    e2max = wave_max(e2max);
-    const double tnorm = fmax(fabs(glo), fabs(ghi));
-    const double pivmin = DBL_MIN * fmax(1.0, e2max);
-    if (n == 1) {
-        if (lane == 0) Q[0] = 1.0;
-        return 0;
-    }
-    // bisection: count(lo) <= k < count(hi) for k = lane
-    double lo = glo - 2.0 * DBL_EPSILON * tnorm - pivmin, hi = ghi + 2.0 * DBL_EPSILON * tnorm + pivmin;
-    // stop at 2 eps ||T|| (the accuracy the eigenvalues have anyway): at most about 55 halvings
-    const double tol = 2.0 * DBL_EPSILON * tnorm + pivmin;
-    bool active = own;
-    int iter = 0;
-    while (__ballot(active)) {
-        // at most 2 + log2(range / tol) halvings; a NaN in T never meets the tests: give up
-        // after 256 and leave the matrix to QL (whose own iteration cap reports the failure)
-        if (++iter > 256) return 1;
-        const double mid = 0.5 * (lo + hi);
-        if (active && (hi - lo <= tol || mid <= lo || mid >= hi)) active = false;
-        if (__ballot(active) == 0) break;
-        double q = d[0] - mid;
+    TridiagBounds b;
+    b.bad = __ballot(bad) != 0;
+    b.glo = wave_min(glo);
+    b.ghi = wave_max(ghi);
+    b.tnorm = fmax(fabs(b.glo), fabs(b.ghi));
+    b.pivmin = DBL_MIN * fmax(1.0, wave_max(e2max));
+    return b;
+}
+
+// Number of eigenvalues of T below x: negative pivots of the LDL^T factorisation of T - x I.
+__device__ __forceinline__ int sturm_count(int n, const double* d, const double* e, double x, double pivmin) {
+    double q = d[0] - x;
+    if (fabs(q) <= pivmin) q = -pivmin;
+    int c = q < 0.0;
+    for (int i = 1; i < n; i++) {
+        const double ei = e[i - 1];
+        q = (d[i] - x) - ei * ei * recip(q);
         if (fabs(q) <= pivmin) q = -pivmin;
-        int c = q < 0.0;
-        for (int i = 1; i < n; i++) {
-            const double ei = e[i - 1];
-            q = (d[i] - mid) - ei * ei * recip(q);
-            if (fabs(q) <= pivmin) q = -pivmin;
-            c += q < 0.0;
-        }
-        if (active) {
-            if (c > lane) hi = mid;
-            else lo = mid;
-        }
+        c += q < 0.0;
     }
-    const double lam = 0.5 * (lo + hi);
+    return c;
+}
+
+// Lane k's eigenvector for its eigenvalue lam (lanes k < n, ascending), from the twisted
+// factorisation; 1 (nothing written) when two eigenvalues lie closer than kTwistGap ||T||.
+__device__ int wave_twisted_vectors(int n, double* d, const double* e, double* Q, double* tmp, double lam,
+                                    double tnorm, double pivmin) {
+    const int lane = threadIdx.x & 63;
+    const bool own = lane < n;
     // gaps to the neighbouring eigenvalues
     const double up = __shfl_down(lam, 1, 64), dn = __shfl_up(lam, 1, 64);
     double gap = 1e300;
@@ -306,6 +303,121 @@ __device__ int wave_tridiag_bisect(int n, double* d, const double* e, double* Q,
     for (int k = 0; k < n; k++) Q[k * 64 + lane] = own ? tmp[lane * kTmpStride + k] : 0.0;
     if (own) d[lane] = lam;
     return 0;
+}
+
+__device__ int wave_tridiag_bisect(int n, double* d, const double* e, double* Q, double* tmp) {
+    const int lane = threadIdx.x & 63;
+    const bool own = lane < n;
+    // a non-finite entry goes to QL, which reports it through its iteration cap
+    const TridiagBounds b = wave_tridiag_bounds(n, d, e);
+    if (b.bad) return 1;
+    if (n == 1) {
+        if (lane == 0) Q[0] = 1.0;
+        return 0;
+    }
+    // bisection: count(lo) <= k < count(hi) for k = lane
+    double lo = b.glo - 2.0 * DBL_EPSILON * b.tnorm - b.pivmin, hi = b.ghi + 2.0 * DBL_EPSILON * b.tnorm + b.pivmin;
+    // stop at 2 eps ||T|| (the accuracy the eigenvalues have anyway): at most about 55 halvings
+    const double tol = 2.0 * DBL_EPSILON * b.tnorm + b.pivmin;
+    bool active = own;
+    int iter = 0;
+    while (__ballot(active)) {
+        // at most 2 + log2(range / tol) halvings; give up after 256 and leave the matrix to QL
+        if (++iter > 256) return 1;
+        const double mid = 0.5 * (lo + hi);
+        if (active && (hi - lo <= tol || mid <= lo || mid >= hi)) active = false;
+        if (__ballot(active) == 0) break;
+        const int c = sturm_count(n, d, e, mid, b.pivmin);
+        if (active) {
+            if (c > lane) hi = mid;
+            else lo = mid;
+        }
+    }
+    return wave_twisted_vectors(n, d, e, Q, tmp, 0.5 * (lo + hi), b.tnorm, b.pivmin);
+}
+
+// The same decomposition by a whole block of W = blockDim.x / 64 wavefronts: each round every
+// wavefront evaluates the Sturm count of every eigenvalue's interval at one of W inner points
+// (lane k, point lo_k + (w + 1)(hi_k - lo_k)/(W + 1)), and the interval shrinks to the two
+// neighbouring points that bracket the k-th eigenvalue: log2(W + 1) bits a round (16
+// wavefronts: 4 bits) instead of one. Wavefront 0 then builds the vectors. Called by every
+// thread of the block; returns the same value in all. scratch: LDS, 3 x 64 + W x 64 doubles
+// (it may be the tmp of the vectors, which is written only after the last round).
+__device__ int block_tridiag_multisect(int n, double* d, const double* e, double* Q, double* tmp, int* flag) {
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int W = blockDim.x >> 6;
+    double* lo = tmp;                // 64
+    double* hi = tmp + 64;           // 64
+    double* cnt = tmp + 128;         // W x 64 counts (exact small integers)
+    TridiagBounds b{};
+    if (wv == 0) {
+        b = wave_tridiag_bounds(n, d, e);
+        if (lane < n) {
+            lo[lane] = b.glo - 2.0 * DBL_EPSILON * b.tnorm - b.pivmin;
+            hi[lane] = b.ghi + 2.0 * DBL_EPSILON * b.tnorm + b.pivmin;
+        }
+        if (lane == 0) flag[0] = b.bad ? 2 : (n == 1 ? 3 : 0);
+    }
+    __syncthreads();
+    const int f0 = flag[0];
+    if (f0 == 2) return 1;
+    if (f0 == 3) {
+        if (tid == 0) Q[0] = 1.0;
+        return 0;
+    }
+    // every wavefront needs the guard and the tolerance: wavefront 0's values, through LDS
+    if (wv == 0 && lane == 0) {
+        cnt[W * 64] = b.tnorm;
+        cnt[W * 64 + 1] = b.pivmin;
+    }
+    __syncthreads();
+    const double tnorm = cnt[W * 64], pivmin = cnt[W * 64 + 1];
+    const double tol = 2.0 * DBL_EPSILON * tnorm + pivmin;
+    for (int round = 0;; round++) {
+        if (round >= 64) return 1;                       // uniform: flag-driven exit below
+        if (lane < n) {
+            const double l = lo[lane], h = hi[lane];
+            const double x = l + (double)(wv + 1) * ((h - l) / (double)(W + 1));
+            cnt[wv * 64 + lane] = (h - l <= tol) ? 0.0 : (double)sturm_count(n, d, e, x, pivmin);
+        }
+        __syncthreads();
+        if (wv == 0) {
+            bool open = false;
+            if (lane < n) {
+                const double l = lo[lane], h = hi[lane];
+                if (h - l > tol) {
+                    double nl = l, nh = h;
+                    for (int w = 0; w < W; w++) {
+                        const double x = l + (double)(w + 1) * ((h - l) / (double)(W + 1));
+                        if ((int)cnt[w * 64 + lane] <= lane) {
+                            nl = x;
+                        } else {
+                            nh = x;
+                            break;
+                        }
+                    }
+                    lo[lane] = nl;
+                    hi[lane] = nh;
+                    open = nh - nl > tol && nl < nh;
+                }
+            }
+            const bool any = __ballot(open) != 0;
+            if (lane == 0) flag[0] = any ? 1 : 0;
+        }
+        __syncthreads();
+        if (flag[0] == 0) break;
+    }
+    int info = 0;
+    if (wv == 0) {
+        const double lam = lane < n ? 0.5 * (lo[lane] + hi[lane]) : 0.0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        info = wave_twisted_vectors(n, d, e, Q, tmp, lam, tnorm, pivmin);
+        if (lane == 0) flag[0] = info;
+    }
+    __syncthreads();
+    return flag[0];
 }
 
 // f(x) = h |sum_k p_k e^{-i x lambda_k}| at S.xs[0..npts) -> S.fx; kGrp lanes per point.
@@ -788,10 +900,12 @@ __device__ int step_control_body(CtlShared& S, const StepConsts& C, StepState* s
         S.h = ks->h;
     }
     __syncthreads();
-    if (tid < 64) {                                  // bisection, QL when eigenvalues lie close
-        int info = C.eig_fast ? wave_tridiag_bisect(m, S.d, S.e, S.Q, S.T) : 1;
-        if (info) info = wave_tridiag_ql(m, S.d, S.e, S.Q);
-        if (tid == 0) S.info = info;
+    // multisection over the block's wavefronts and twisted factorisations, QL when eigenvalues
+    // lie close (or on request)
+    int einfo = C.eig_fast ? block_tridiag_multisect(m, S.d, S.e, S.Q, S.T, &S.eflag) : 1;
+    if (tid < 64) {
+        if (einfo) einfo = wave_tridiag_ql(m, S.d, S.e, S.Q);
+        if (tid == 0) S.info = einfo;
     }
     __syncthreads();
     if (S.info != 0) {                               // :366-370
