@@ -27,6 +27,7 @@
 #include <cstdlib>
 #include <climits>
 #include <cstdint>
+#include <cstring>
 #include <vector>
 
 #include "te_internal.h"
@@ -649,21 +650,88 @@ extern "C" int te_basis_find(const te_basis* B, const int* occ, size_t count, in
 
 namespace te {
 
+// sig[p]: the mode at significance position p (0 = most significant).
+struct ModeOrder {
+    int sig[kMaxModes];
+};
+
 __global__ void lex_key_kernel(const int* __restrict__ occ, int64_t D, int K, unsigned long long radix,
-                               unsigned long long* keys, int* idx) {
+                               ModeOrder mo, unsigned long long* keys, int* idx) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= D) return;
     unsigned long long k = 0;
-    for (int m = 0; m < K; m++) k = k * radix + (unsigned long long)occ[i * K + m];
+    for (int p = 0; p < K; p++) k = k * radix + (unsigned long long)occ[i * K + mo.sig[p]];
     keys[i] = k;
     idx[i] = (int)i;
 }
 
-// Lexicographic order of the basis states (mode 0 most significant, mixed radix max_occ + 1),
-// as new -> old. Neighbouring states that differ in the trailing modes get neighbouring
-// indices, so a product's gathers from x stay within a few thousand entries for the transitions
-// among those modes. Null (identity) when the key does not fit 64 bits or TE_SPMV_ORDER=0.
-static int basis_locality_order(te_ctx* ctx, const te_basis* B, const int** out) {
+// Significance order of the modes for the locality order: a Cuthill-McKee numbering of the graph
+// whose vertices are the modes and whose edges join the modes of one term, from the start mode
+// that gives the smallest bandwidth (ties: the lowest start). A transition that changes modes at
+// significance positions p < q moves a state's rank by about the number of states sharing its
+// first p modes, and every row of a 64-row slice shares its leading modes, so with a small
+// bandwidth most transitions change only leading modes that a whole slice shares: their column
+// offset and value are the same on every row of the slice. On a ring of K modes this gives
+// 0, 1, K-1, 2, K-2, ... (bandwidth 2; mode 0 first and last would put the closing bond's two
+// modes at the two ends). Without two-mode terms: the modes in index order.
+static ModeOrder mode_significance(int K, const std::vector<int>& off, const std::vector<int>& modes) {
+    ModeOrder mo{};
+    if (K > kMaxModes) return mo;                        // the key cannot hold such a basis anyway
+    for (int m = 0; m < K; m++) mo.sig[m] = m;
+    std::vector<std::vector<char>> adj(K, std::vector<char>(K, 0));
+    bool any = false;
+    for (size_t t = 0; t + 1 < off.size(); t++)
+        for (int a = off[t]; a < off[t + 1]; a++)
+            for (int b = off[t]; b < off[t + 1]; b++)
+                if (modes[a] != modes[b]) adj[modes[a]][modes[b]] = 1, any = true;
+    if (!any) return mo;
+    std::vector<int> deg(K, 0);
+    for (int u = 0; u < K; u++)
+        for (int v = 0; v < K; v++) deg[u] += adj[u][v];
+    int best_bw = K + 1;
+    for (int start = 0; start < K; start++) {
+        std::vector<int> order, pos(K, -1);
+        order.reserve(K);
+        // Cuthill-McKee from `start`; a disconnected remainder restarts at its lowest mode
+        for (int seed = start, done = 0; done < K;) {
+            if (pos[seed] < 0) {
+                pos[seed] = (int)order.size();
+                order.push_back(seed);
+                for (size_t h = order.size() - 1; h < order.size(); h++) {
+                    std::vector<int> nb;
+                    for (int v = 0; v < K; v++)
+                        if (adj[order[h]][v] && pos[v] < 0) nb.push_back(v);
+                    std::stable_sort(nb.begin(), nb.end(), [&](int a, int b) { return deg[a] < deg[b]; });
+                    for (int v : nb) {
+                        pos[v] = (int)order.size();
+                        order.push_back(v);
+                    }
+                }
+            }
+            done = (int)order.size();
+            seed = 0;
+            while (seed < K && pos[seed] >= 0) seed++;
+            if (seed == K) break;
+        }
+        int bw = 0;
+        for (int u = 0; u < K; u++)
+            for (int v = 0; v < K; v++)
+                if (adj[u][v]) bw = std::max(bw, std::abs(pos[u] - pos[v]));
+        if (bw < best_bw) {
+            best_bw = bw;
+            for (int p = 0; p < K; p++) mo.sig[p] = order[p];
+        }
+    }
+    return mo;
+}
+
+// Lexicographic order of the basis states (modes in the significance order `mo`, mixed radix
+// max_occ + 1), as new -> old. Neighbouring states that differ in the trailing modes get
+// neighbouring indices, so a product's gathers from x stay within a few thousand entries for the
+// transitions among those modes. Computed once per basis, by the first operator with two-mode
+// terms that is built on it. Null (identity) when the key does not fit 64 bits or
+// TE_SPMV_ORDER=0; TE_SPMV_ORDER=lex keeps the modes in index order.
+static int basis_locality_order(te_ctx* ctx, const te_basis* B, ModeOrder mo, const int** out) {
     *out = nullptr;
     const char* e = std::getenv("TE_SPMV_ORDER");
     if (e && e[0] == '0') return TE_OK;
@@ -693,7 +761,8 @@ static int basis_locality_order(te_ctx* ctx, const te_basis* B, const int** out)
         hipMalloc(&i_in, sizeof(int) * D) != hipSuccess || hipMalloc(&perm, sizeof(int) * D) != hipSuccess)
         rc = fail(ctx, TE_ERR_ALLOC, "locality order allocation");
     if (rc == TE_OK) {
-        hipLaunchKernelGGL(lex_key_kernel, dim3(blocks_for(D)), dim3(256), 0, s, B->occ, D, B->K, radix, k_in, i_in);
+        hipLaunchKernelGGL(lex_key_kernel, dim3(blocks_for(D)), dim3(256), 0, s, B->occ, D, B->K, radix, mo, k_in,
+                           i_in);
         if (hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k_in, k_out, i_in, perm, (int)D, 0, bits, s) !=
                 hipSuccess ||
             hipMalloc(&tmp, std::max<size_t>(bytes, 1)) != hipSuccess)
@@ -828,7 +897,13 @@ extern "C" int te_operator_build(te_ctx* ctx, const te_basis* B, const te_op_ter
     }
     if (rc == TE_OK) rc = finalize_matrix(ctx, A);
     const int* order = nullptr;
-    if (rc == TE_OK && !A->diagonal) rc = basis_locality_order(ctx, B, &order);
+    if (rc == TE_OK && !A->diagonal) {
+        ModeOrder mo = mode_significance(B->K, off, modes);
+        const char* e = std::getenv("TE_SPMV_ORDER");
+        if (e && std::strcmp(e, "lex") == 0)
+            for (int m = 0; m < B->K; m++) mo.sig[m] = m;
+        rc = basis_locality_order(ctx, B, mo, &order);
+    }
     if (rc == TE_OK) rc = prepare_spmv(ctx, A, order);
     void* frees[] = {d_off, d_ops, d_modes, d_caps, d_coef, d_sqrt, ws, counts, rp64};
     for (void* p : frees)
