@@ -59,10 +59,12 @@ int te_matrix_info(const te_matrix* A, size_t* n_rows, size_t* n_cols, size_t* n
  * for A. format: 0 CSR with lanes per row, 1 CSR row blocks, 2 value-dictionary row blocks
  * (32-bit columns and 16-bit indices into n_values distinct off-diagonal values, dense
  * diagonal), 3 the same in a locality order of the rows (lexicographic order of the basis
- * states for operators from te_operator_build), 4 its sliced-ELL form (64-row slices stored
+ * states for operators from te_operator_build, the modes ranked by a Cuthill-McKee numbering of
+ * the first such operator's mode couplings), 4 its sliced-ELL form (64-row slices stored
  * slot-major; 4-byte slots of a 16-bit column offset and a 16-bit value index, one base column
- * per slot of a slice; the dense diagonal as 8-byte reals when every imaginary part is +0.0,
- * else 16-byte complex values). With format 3, and with 4 in the locality order, te_spmv reads
+ * per slot of a slice; entries at the same column offset and value on every row of a slice as
+ * one 8-byte record per slice; the dense diagonal as 8-byte reals when every imaginary part is
+ * +0.0, else 16-byte complex values). With format 3, and with 4 in the locality order, te_spmv reads
  * the CSR row blocks in the basis order. stream_bytes: bytes of matrix data one product reads. */
 int te_matrix_storage(const te_matrix* A, int* format, size_t* n_values, size_t* stream_bytes);
 /* smatrix::norm1 (core/matrixDataTypes.cpp:67-79): maximal absolute column sum. */

@@ -134,9 +134,9 @@ def _aligned_slots(rows):
     return total
 
 
-def _sell_rows(c, r, order):
+def _sell_rows(c, r, order, v=None):
     """Off-diagonal columns of each row in the row order `order` (new -> old), columns renumbered
-    in the same order and ascending."""
+    in the same order and ascending; with v, (column, value bits) pairs."""
     inv = np.empty_like(order)
     inv[order] = np.arange(len(order))
     off = c != r
@@ -144,7 +144,76 @@ def _sell_rows(c, r, order):
     idx = np.lexsort((cc, rr))
     rr, cc = rr[idx], cc[idx]
     bounds = np.searchsorted(rr, np.arange(len(order) + 1))
-    return [cc[bounds[i]:bounds[i + 1]].tolist() for i in range(len(order))]
+    if v is None:
+        return [cc[bounds[i]:bounds[i + 1]].tolist() for i in range(len(order))]
+    bits = np.ascontiguousarray(v[off]).view(np.uint64).reshape(-1, 2)[idx]
+    key = (bits[:, 0] ^ (bits[:, 1] * np.uint64(0x9E3779B97F4A7C15))).tolist()
+    return [list(zip(cc[bounds[i]:bounds[i + 1]].tolist(), key[bounds[i]:bounds[i + 1]])) for i in range(len(order))]
+
+
+def _sell_layout(rows):
+    """The sliced-ELL layout of rows of (column, value) pairs: in every full 64-row slice the
+    leading and trailing entry positions at which all rows have the same column - row and the same
+    value are shared records; the rows' middle entries take aligned slots. Returns (64 * slots,
+    shared records, records past the 12 kept at fixed places per slice)."""
+    slots = shared = over = 0
+    for s0 in range(0, len(rows), 64):
+        sl = rows[s0:s0 + 64]
+        lo = hi = 0
+        if len(sl) == 64:
+            mn = min(len(row) for row in sl)
+            same = lambda pos: len({(row[p][0] - (s0 + i), row[p][1]) for i, (row, p) in enumerate(zip(sl, pos))}) == 1
+            while lo < mn and same([lo] * 64):
+                lo += 1
+            while lo + hi < mn and same([len(row) - 1 - hi for row in sl]):
+                hi += 1
+        shared += lo + hi
+        over += max(0, lo + hi - 12)
+        slots += _aligned_slots([[e[0] for e in row[lo:len(row) - hi]] for row in sl])
+    return slots, shared, over
+
+
+def _sell_bytes(rows, D, diag_bytes):
+    """Matrix bytes one sliced-ELL product streams (te_matrix_storage)."""
+    slots, shared, over = _sell_layout(rows)
+    ns = (D + 63) // 64
+    return 4 * slots + 4 * (slots // 64) + 16 * (ns + 1) + diag_bytes * D + (8 * 12 * ns + 8 * over if shared else 0)
+
+
+def _mode_significance(n_modes, terms):
+    """Significance order of the modes for the locality order: a Cuthill-McKee numbering of the
+    graph joining the modes of one term, from the start mode with the smallest bandwidth."""
+    adj = [set() for _ in range(n_modes)]
+    for t in terms:
+        ms = list(t[2])
+        for a in ms:
+            for b in ms:
+                if a != b:
+                    adj[a].add(b)
+    if not any(adj):
+        return list(range(n_modes))
+    best, best_bw = None, n_modes + 1
+    for start in range(n_modes):
+        order, pos = [], {}
+        seed = start
+        while len(order) < n_modes:
+            if seed not in pos:
+                pos[seed] = len(order)
+                order.append(seed)
+                h = len(order) - 1
+                while h < len(order):
+                    nb = sorted((u for u in adj[order[h]] if u not in pos), key=lambda u: (len(adj[u]), u))
+                    for u in nb:
+                        pos[u] = len(order)
+                        order.append(u)
+                    h += 1
+            seed = next((m for m in range(n_modes) if m not in pos), None)
+            if seed is None:
+                break
+        bw = max((abs(pos[a] - pos[b]) for a in range(n_modes) for b in adj[a]), default=0)
+        if bw < best_bw:
+            best, best_bw = order, bw
+    return best
 
 
 def _distinct_offdiag(v, c, r):
@@ -174,21 +243,21 @@ def test_value_dictionary_storage(gpu, oracle_lib, monkeypatch, L):
         assert nbytes == 6 * n_off + 4 * (ob.D + 1) + 16 * ob.D
         _spmv_check(oracle_lib, M, ob.D, v, c, r)
     # sliced ELL (default): 64-row slices, rows in the basis order (COO matrix) or in the
-    # lexicographic order of the states (built operator); 4-byte words, a 4-byte base per slot
-    # of a slice; below 65536 rows a slice has as many slots as its longest row
-    ns = (ob.D + 63) // 64
-    rowlen = np.bincount(r[c != r].astype(np.int64), minlength=ob.D)
-    states = ob.states().astype(np.int64)
-    lex = np.lexsort(states.T[::-1])                  # mode 0 most significant
+    # lexicographic order of the states with the modes ranked 0, 1, L-1, 2, L-2, ... (built
+    # operator: Cuthill-McKee on the ring); shared records, 4-byte words, a 4-byte base per slot
+    # of a slice; the real diagonal as 8-byte values
+    sig = _mode_significance(L, terms)
+    assert sig[:5] == [0, 1, L - 1, 2, L - 2]
+    states = ob.states().astype(np.int64)[:, sig]
+    lex = np.lexsort(states.T[::-1])                  # sig[0] most significant
     for M, order in ((capi.Matrix.from_coo(gpu, v, c, r, ob.D), np.arange(ob.D)),
                      (capi.operator_build(gpu, gb, terms), lex)):
         fmt, nv, nbytes = M.storage()
         assert fmt == 4 and nv == n_distinct
-        lens = np.zeros(ns * 64, np.int64)
-        lens[:ob.D] = rowlen[order]
-        slots = 64 * int(lens.reshape(ns, 64).max(1).sum())
-        assert slots == _aligned_slots(_sell_rows(c, r, order))
-        assert nbytes == 4 * slots + 4 * (slots // 64) + 4 * (ns + 1) + 8 * ob.D   # real diagonal
+        rows = _sell_rows(c, r, order, v)
+        assert nbytes == _sell_bytes(rows, ob.D, 8)
+        if order is lex and L >= 8:                   # a share of the ring's transitions: 9 % at L = 8, 28 % at 10
+            assert _sell_layout(rows)[1] * 64 > (0.05 if L == 8 else 0.2) * n_off
         _spmv_check(oracle_lib, M, ob.D, v, c, r)
     monkeypatch.setenv("TE_SPMV_VD", "0")
     P = capi.Matrix.from_coo(gpu, v, c, r, ob.D)
@@ -233,7 +302,8 @@ def test_sliced_ell_slots_beyond_16_bit_span(gpu, oracle_lib):
     slots = _aligned_slots(_sell_rows(c, r, np.arange(D)))
     longest = 64 * int(np.bincount(r[c != r].astype(np.int64), minlength=ns * 64).reshape(ns, 64).max(1).sum())
     assert slots > longest
-    assert nbytes == 4 * slots + 4 * (slots // 64) + 4 * (ns + 1) + 16 * D
+    rows = _sell_rows(c, r, np.arange(D), v)
+    assert _sell_layout(rows)[0] == slots and nbytes == _sell_bytes(rows, D, 16)
     _spmv_check(oracle_lib, A, D, v, c, r)
 
 
@@ -245,26 +315,39 @@ def test_sliced_ell_diagonal_kinds(gpu, oracle_lib, monkeypatch):
     row-block kernel's bit for bit (same rows, diagonal first, then the entries in column order):
     compared at alpha = 1 and alpha = -0.3i, where y = alpha * sum rounds the same whichever of
     its two products the compiler fuses into a multiply-add (with a general complex alpha that
-    choice may differ between the two kernels by one rounding)."""
+    choice may differ between the two kernels by one rounding). Rows in the basis order, and in
+    the locality order of a built ring operator, where most entries are shared records (the
+    same column offset and value on every row of a slice: leading, per-row and trailing parts
+    of each row's sum)."""
     L = N = 8
     ob = oracle_lib.Basis.basis(N, L)
-    v, c, r = oracle_lib.build_operator(ob, as_tuples(_ring_terms(L)))
+    terms = as_tuples(_ring_terms(L))
+    v0, c0, r0 = oracle_lib.build_operator(ob, terms)
+    D = ob.D
+    states = ob.states().astype(np.int64)[:, _mode_significance(L, terms)]
+    order = np.lexsort(states.T[::-1])                # new -> old
+    inv = np.empty_like(order)
+    inv[order] = np.arange(D)
+    c1, r1 = inv[c0.astype(np.int64)], inv[r0.astype(np.int64)]
+    srt = np.lexsort((c1, r1))                        # rows ascending, columns ascending
+    x = random_state(D, 4)
+    for v, c, r in ((v0, c0, r0), (v0[srt], c1[srt].astype(np.uint64), r1[srt].astype(np.uint64))):
+        _sell_diagonal_kinds_case(gpu, oracle_lib, monkeypatch, v, c, r, D, x)
+
+
+def _sell_diagonal_kinds_case(gpu, oracle_lib, monkeypatch, v, c, r, D, x):
     on = c == r
     assert np.all(np.ascontiguousarray(v[on].imag).view(np.uint64) == 0)
-    D = ob.D
-    ns = (D + 63) // 64
-    slots = _aligned_slots(_sell_rows(c, r, np.arange(D)))
-    fixed = 4 * slots + 4 * (slots // 64) + 4 * (ns + 1)
+    rows = _sell_rows(c, r, np.arange(D), v)      # the diagonal does not enter the layout
     v_cplx = v.copy()
     v_cplx[on] = v[on] * (1.0 + 0.25j)
     v_negz = v.copy()
     k = int(np.flatnonzero(on)[D // 2])
     v_negz[k] = complex(v[k].real, -0.0)
-    x = random_state(D, 4)
     for vals, diag_bytes in ((v, 8), (v_cplx, 16), (v_negz, 16)):
         A = capi.Matrix.from_coo(gpu, vals, c, r, D)
         fmt, _, nbytes = A.storage()
-        assert fmt == 4 and nbytes == fixed + diag_bytes * D
+        assert fmt == 4 and nbytes == _sell_bytes(rows, D, diag_bytes)
         _spmv_check(oracle_lib, A, D, vals, c, r)
         monkeypatch.setenv("TE_SPMV_SELL", "0")
         B = capi.Matrix.from_coo(gpu, vals, c, r, D)

@@ -438,7 +438,8 @@ extern "C" int te_matrix_storage(const te_matrix* A, int* format, size_t* n_valu
     if (stream_bytes) {
         const size_t n = (size_t)A->n_rows, ns = (size_t)A->vd.n_slices;
         *stream_bytes = f == 4   ? 4 * (size_t)A->vd.sell_entries + 4 * ((size_t)A->vd.sell_entries / te::kSlice) +
-                                     4 * (ns + 1) + (A->vd.diag_re ? 8 : 16) * n
+                                     16 * (ns + 1) + (A->vd.diag_re ? 8 : 16) * n +
+                                     (A->vd.sell_ufix ? 8 * te::kSellFix * ns + 8 * (size_t)A->vd.sell_overflow : 0)
                         : f >= 2 ? 6 * (size_t)A->vd.nnz + 4 * (n + 1) + 16 * n
                                  : 20 * (size_t)A->nnz + 4 * (n + 1);
     }

@@ -80,22 +80,32 @@ struct VdCsr {
     int* perm = nullptr;
     int* iperm = nullptr;
     // Sliced-ELL form of the same entries (when every row has at most 255 off-diagonal entries):
-    // slices of kSlice consecutive rows stored slot-major, slot j of row s*kSlice + l at word
-    // sell_off[s] + j*kSlice + l, so a wavefront reads every slot of its 64 rows with one
-    // coalesced load. Each row's entries keep their column order; they are dealt to slots so
-    // that the columns of one slot (over the slice's rows) lie within kSellSpan of each other:
-    // slot j starts at column sell_base[sell_off[s] / kSlice + j], and a word holds the 16-bit
-    // offset from it and, above, the 16-bit value index (kSellPadIdx: no entry, not added; the
-    // offset is then 0, a column of the slot). The far neighbours of the slice's rows, which sit
-    // at the same offsets for rows that share their leading modes, share slots and their gathers
-    // share cache lines. Optionally (TE_SELL_SIGMA=1) rows are ordered by decreasing length
-    // within windows of kSellWindow (part of perm): less padding, worse gathers.
+    // slices of kSlice consecutive rows. Shared slots: the leading and trailing entry positions
+    // at which every row of a full slice has an entry with the same column offset from its row
+    // and the same value index (transitions that change only modes the slice's rows share) are
+    // stored once per slice as records (column - row, value index): the first kSellFix in
+    // sell_ufix[s * kSellFix + q], the rest in sell_uni from the slice's overflow offset. The
+    // entries between them (the middle of each row) are stored slot-major, slot j of row
+    // s*kSlice + l at word meta[s].x + j*kSlice + l, so a wavefront reads every slot of its 64
+    // rows with one coalesced load. Each row's middle entries keep their column order; they are
+    // dealt to slots so that the columns of one slot (over the slice's rows) lie within
+    // kSellSpan of each other: slot j starts at column sell_base[meta[s].x / kSlice + j], and a
+    // word holds the 16-bit offset from it and, above, the 16-bit value index (kSellPadIdx: no
+    // entry, not added; the offset is then 0, a column of the slot). A row sums diag, leading
+    // shared, middle, trailing shared: its column order. sell_meta[s] = (first word, first
+    // overflow record, leading shared records | shared records << 16, 0); sell_meta[n_slices] =
+    // (words, overflow records, 0, 0). Optionally (TE_SELL_SIGMA=1) rows are ordered by
+    // decreasing length within windows of kSellWindow (part of perm): less padding, worse gathers.
     bool sell = false;
-    int* sell_off = nullptr;            // n_slices + 1
+    int4* sell_meta = nullptr;          // n_slices + 1
     unsigned* sell_word = nullptr;      // sell_entries + kSellGroup * kSlice (zero padding)
     int* sell_base = nullptr;           // sell_entries / kSlice + kSellGroup (zero padding)
+    int2* sell_ufix = nullptr;          // n_slices * kSellFix, or null without shared slots
+    int2* sell_uni = nullptr;           // overflow records (at least one)
     int n_slices = 0;
     int64_t sell_entries = 0;
+    int64_t sell_shared = 0;            // shared records
+    int64_t sell_overflow = 0;          // of them in sell_uni
     int sell_grid = 0;
     int sell_chunk = 0;                 // SellView::chunk
     // The real parts of diag when every imaginary part is +0.0 (bit pattern 0), else nullptr:
@@ -109,11 +119,12 @@ constexpr int kSellGroup = 4;       // slots a lane loads (and gathers) before u
 constexpr int kSellWindow = 512;
 constexpr int kSellSpan = 1 << 16;  // columns one slot of a slice may span (16-bit offsets)
 constexpr unsigned kSellPadIdx = 0xffffu;   // value index of a padding slot
+constexpr int kSellFix = 12;        // shared records per slice at fixed places (sell_ufix)
 static_assert(kVdMaxValues < (int)kSellPadIdx, "value indices must stay below the padding index");
 
 // What the sliced-ELL kernels read.
 struct SellView {
-    const int* off;
+    const int4* meta;
     const unsigned* word;
     const int* base;
     const double2* vtab;
@@ -124,6 +135,8 @@ struct SellView {
     int chunk;      // 0: XCD x sweeps the x-th eighth of the slices; > 0: chunks of `chunk` slices
                     // dealt round-robin to the XCDs, so all XCDs sweep one front together
     const double* diag_re;   // real diagonal (VdCsr::diag_re) or nullptr: read diag
+    const int2* ufix;        // shared records (VdCsr::sell_ufix) or nullptr
+    const int2* uni;
 };
 
 // What the kernels read of a VdCsr.

@@ -696,8 +696,8 @@ static VdView vd_view(const te_matrix* A) {
 
 static SellView sell_view(const te_matrix* A) {
     const VdCsr& v = A->vd;
-    return SellView{v.sell_off, v.sell_word, v.sell_base, v.vtab,        v.n_values,
-                    v.diag,     v.n_slices,  A->n_rows,  v.sell_chunk, v.diag_re};
+    return SellView{v.sell_meta, v.sell_word, v.sell_base, v.vtab,      v.n_values, v.diag,
+                    v.n_slices,  A->n_rows,   v.sell_chunk, v.diag_re, v.sell_ufix, v.sell_uni};
 }
 
 SellView sell_view_of(const te_matrix* A) { return sell_view(A); }
@@ -1095,58 +1095,120 @@ __global__ __launch_bounds__(kSellWindow) void sigma_sort_kernel(const int64_t* 
     out[w0 + rank] = base ? base[r] : (int)r;
 }
 
-// Slot alignment of the slices, one wavefront per slice (lane l: row s*kSlice + l). The next
-// slot of a slice takes, from every row, the row's next entry (rows in column order) if that
-// lies within kSellSpan columns of the smallest next entry over the slice; rows whose next entry
-// lies further right get a padding word in it. Count pass (FILL false): slen[s] = the slice's
-// slots * kSlice, smax[s] = its longest row * kSlice (the slots without the span limit). Fill
-// pass: the words and slot bases at the offsets soff (exclusive scan of slen). A slot's padding
-// words point at its base column, a column of one of its entries.
+// What the two passes of sell_align_kernel read and write.
+struct SellBuild {
+    const int* vptr;
+    const int* vcol;
+    const unsigned short* vvidx;
+    int64_t n;
+    int n_slices;
+    int shared;                 // 1: look for shared slots
+    // count pass
+    int64_t* slen;              // middle slots * kSlice per slice
+    int64_t* smax;              // the slice's longest middle part * kSlice
+    int64_t* ucnt;              // shared records per slice
+    int* usplit;                // leading shared records per slice
+    int64_t* ovcnt;             // overflow records per slice
+    // fill pass
+    const int64_t* soff;        // exclusive scan of slen
+    const int64_t* ovoff;       // exclusive scan of ovcnt
+    int4* meta;
+    unsigned* word;
+    int* base;
+    int2* ufix;                 // null: no shared slots anywhere
+    int2* uni;
+};
+
+// Shared slots of a full slice (all kSlice rows exist): the leading entry positions 0..ulo-1 and
+// the trailing positions deg-uhi..deg-1 at which every row of the slice has an entry with the same
+// column offset from its row and the same value index (a transition that changes only modes the
+// slice's rows share). They are stored once per slice; the entries between them (the middle) go
+// to per-row slots. Each row's entries keep their column order: leading, middle, trailing.
+__device__ __forceinline__ void shared_slots(const SellBuild& b, int64_t r, int k, int kend, int& ulo, int& uhi) {
+    ulo = uhi = 0;
+    int mn = kend - k;
+#pragma unroll
+    for (int d = kSlice / 2; d >= 1; d >>= 1) mn = min(mn, __shfl_xor(mn, d));
+    auto same = [&](int e) {
+        const int off = b.vcol[e] - (int)r, vi = b.vvidx[e];
+        return __all(off == __shfl(off, 0) && vi == __shfl(vi, 0)) != 0;
+    };
+    while (ulo < mn && same(k + ulo)) ulo++;
+    while (ulo + uhi < mn && same(kend - 1 - uhi)) uhi++;
+}
+
+// Slot alignment of the slices, one wavefront per slice (lane l: row s*kSlice + l). After the
+// shared slots, the next slot of a slice takes, from every row, the row's next middle entry (rows
+// in column order) if that lies within kSellSpan columns of the smallest next entry over the
+// slice; rows whose next entry lies further right get a padding word in it. Count pass (FILL
+// false): middle slots, longest middle part (the slots without the span limit) and shared records
+// per slice. Fill pass: the words and slot bases at the offsets soff, the shared records at uoff.
+// A slot's padding words point at its base column, a column of one of its entries.
 template <bool FILL>
-__global__ __launch_bounds__(256) void sell_align_kernel(const int* __restrict__ vptr, const int* __restrict__ vcol,
-                                                         const unsigned short* __restrict__ vvidx, int64_t n,
-                                                         int n_slices, int64_t* slen, int64_t* smax,
-                                                         const int64_t* __restrict__ soff, int* sell_off,
-                                                         unsigned* word, int* base) {
+__global__ __launch_bounds__(256) void sell_align_kernel(SellBuild b) {
     const int s = blockIdx.x * (blockDim.x / kSlice) + (int)(threadIdx.x / kSlice);
     const int lane = threadIdx.x % kSlice;
-    if (s > n_slices) return;                                  // whole wavefronts
-    if (s == n_slices) {
-        if (!FILL && lane == 0) slen[s] = smax[s] = 0;
+    if (s > b.n_slices) return;                                // whole wavefronts
+    if (s == b.n_slices) {
+        if (!FILL && lane == 0) b.slen[s] = b.smax[s] = b.ucnt[s] = b.ovcnt[s] = 0;
         return;
     }
     const int64_t r = (int64_t)s * kSlice + lane;
-    int k = r < n ? vptr[r] : 0;
-    const int kend = r < n ? vptr[r + 1] : 0;
+    int k = r < b.n ? b.vptr[r] : 0;
+    int kend = r < b.n ? b.vptr[r + 1] : 0;
+    int ulo = 0, uhi = 0;
+    if (FILL) {
+        ulo = b.usplit[s];
+        uhi = (int)b.ucnt[s] - ulo;
+    } else if (b.shared && (int64_t)(s + 1) * kSlice <= b.n) {
+        shared_slots(b, r, k, kend, ulo, uhi);
+    }
+    if (FILL && lane == 0) {
+        const int nsh = ulo + uhi;
+        const int64_t ov = b.ovoff[s];
+        b.meta[s] = make_int4((int)b.soff[s], (int)ov, ulo | (nsh << 16), 0);
+        if (s == b.n_slices - 1) b.meta[b.n_slices] = make_int4((int)b.soff[b.n_slices], (int)b.ovoff[b.n_slices], 0, 0);
+        if (b.ufix) {
+            for (int q = 0; q < kSellFix || q < nsh; q++) {
+                int2 rec = make_int2(0, 0);
+                if (q < nsh) {
+                    const int e = q < ulo ? k + q : kend - uhi + (q - ulo);
+                    rec = make_int2(b.vcol[e] - (int)r, (int)b.vvidx[e]);
+                }
+                if (q < kSellFix) b.ufix[(int64_t)s * kSellFix + q] = rec;
+                else b.uni[ov + q - kSellFix] = rec;
+            }
+        }
+    }
+    k += ulo;
+    kend -= uhi;
     if (!FILL) {
         int len = kend - k;
 #pragma unroll
         for (int d = kSlice / 2; d >= 1; d >>= 1) len = max(len, __shfl_xor(len, d));
-        if (lane == 0) smax[s] = (int64_t)len * kSlice;
-    }
-    int64_t o = 0;
-    if (FILL) {
-        o = soff[s];
         if (lane == 0) {
-            sell_off[s] = (int)o;
-            if (s == n_slices - 1) sell_off[n_slices] = (int)soff[n_slices];
+            b.smax[s] = (int64_t)len * kSlice;
+            b.ucnt[s] = ulo + uhi;
+            b.usplit[s] = ulo;
+            b.ovcnt[s] = ulo + uhi > kSellFix ? ulo + uhi - kSellFix : 0;
         }
     }
+    const int64_t o = FILL ? b.soff[s] : 0;
     for (int j = 0;; j++) {
         const bool has = k < kend;
-        const int c = has ? vcol[k] : INT_MAX;
+        const int c = has ? b.vcol[k] : INT_MAX;
         int lo = c;
 #pragma unroll
         for (int d = kSlice / 2; d >= 1; d >>= 1) lo = min(lo, __shfl_xor(lo, d));
         if (lo == INT_MAX) {                                   // the same for all lanes
-            if (!FILL && lane == 0) slen[s] = (int64_t)j * kSlice;
+            if (!FILL && lane == 0) b.slen[s] = (int64_t)j * kSlice;
             return;
         }
         const bool take = has && (unsigned)(c - lo) < (unsigned)kSellSpan;
         if (FILL) {
-            word[o + (int64_t)j * kSlice + lane] =
-                take ? ((unsigned)vvidx[k] << 16) | (unsigned)(c - lo) : (kSellPadIdx << 16);
-            if (lane == 0) base[o / kSlice + j] = lo;
+            b.word[o + (int64_t)j * kSlice + lane] =
+                take ? ((unsigned)b.vvidx[k] << 16) | (unsigned)(c - lo) : (kSellPadIdx << 16);
+            if (lane == 0) b.base[o / kSlice + j] = lo;
         }
         k += take ? 1 : 0;
     }
@@ -1163,9 +1225,11 @@ void free_vd(te_matrix* A) {
     if (v.blocks) hipFree(v.blocks);
     if (v.perm) hipFree(v.perm);
     if (v.iperm) hipFree(v.iperm);
-    if (v.sell_off) hipFree(v.sell_off);
+    if (v.sell_meta) hipFree(v.sell_meta);
     if (v.sell_word) hipFree(v.sell_word);
     if (v.sell_base) hipFree(v.sell_base);
+    if (v.sell_ufix) hipFree(v.sell_ufix);
+    if (v.sell_uni) hipFree(v.sell_uni);
     v = VdCsr{};
     A->use_vd = false;
 }
@@ -1218,29 +1282,58 @@ static int build_sell(te_ctx* ctx, te_matrix* A) {
     const int64_t ns64 = (n + kSlice - 1) / kSlice;
     if (ns64 >= INT32_MAX / kSlice) return TE_OK;
     const int ns = (int)ns64;
-    int64_t *slen = nullptr, *soff = nullptr, *smax = nullptr, *smax_off = nullptr;
+    int64_t *slen = nullptr, *soff = nullptr, *smax = nullptr, *smax_off = nullptr, *ucnt = nullptr, *uoff = nullptr;
+    int64_t *ovcnt = nullptr, *ovoff = nullptr;
+    int* usplit = nullptr;
+    TE_HIP(ctx, hipMalloc(&ovcnt, sizeof(int64_t) * (ns + 1)));
+    TE_HIP(ctx, hipMalloc(&ovoff, sizeof(int64_t) * (ns + 1)));
     TE_HIP(ctx, hipMalloc(&slen, sizeof(int64_t) * (ns + 1)));
     TE_HIP(ctx, hipMalloc(&soff, sizeof(int64_t) * (ns + 1)));
     TE_HIP(ctx, hipMalloc(&smax, sizeof(int64_t) * (ns + 1)));
     TE_HIP(ctx, hipMalloc(&smax_off, sizeof(int64_t) * (ns + 1)));
+    TE_HIP(ctx, hipMalloc(&ucnt, sizeof(int64_t) * (ns + 1)));
+    TE_HIP(ctx, hipMalloc(&uoff, sizeof(int64_t) * (ns + 1)));
+    TE_HIP(ctx, hipMalloc(&usplit, sizeof(int) * (ns + 1)));
     const unsigned ga = (unsigned)((ns + 1 + 256 / kSlice - 1) / (256 / kSlice));
-    hipLaunchKernelGGL(sell_align_kernel<false>, dim3(ga), dim3(256), 0, st, v.ptr, v.col, v.vidx, n, ns, slen, smax,
-                       nullptr, nullptr, nullptr, nullptr);
+    SellBuild sb{};
+    sb.vptr = v.ptr;
+    sb.vcol = v.col;
+    sb.vvidx = v.vidx;
+    sb.n = n;
+    sb.n_slices = ns;
+    const char* shared_env = std::getenv("TE_SELL_SHARED");          // 0: every slot per row
+    sb.shared = !(shared_env && shared_env[0] == '0');
+    sb.slen = slen;
+    sb.smax = smax;
+    sb.ucnt = ucnt;
+    sb.usplit = usplit;
+    sb.ovcnt = ovcnt;
+    hipLaunchKernelGGL(sell_align_kernel<false>, dim3(ga), dim3(256), 0, st, sb);
     int rc = exclusive_scan64(ctx, slen, soff, ns + 1);
     if (rc == TE_OK) rc = exclusive_scan64(ctx, smax, smax_off, ns + 1);
-    int64_t total = 0, total_max = 0;
+    if (rc == TE_OK) rc = exclusive_scan64(ctx, ucnt, uoff, ns + 1);
+    if (rc == TE_OK) rc = exclusive_scan64(ctx, ovcnt, ovoff, ns + 1);
+    int64_t total = 0, total_max = 0, n_uni = 0, n_ov = 0;
     if (rc == TE_OK) {
         TE_HIP(ctx, hipMemcpyAsync(&total, soff + ns, sizeof(int64_t), hipMemcpyDeviceToHost, st));
         TE_HIP(ctx, hipMemcpyAsync(&total_max, smax_off + ns, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        TE_HIP(ctx, hipMemcpyAsync(&n_uni, uoff + ns, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        TE_HIP(ctx, hipMemcpyAsync(&n_ov, ovoff + ns, sizeof(int64_t), hipMemcpyDeviceToHost, st));
         TE_HIP(ctx, hipStreamSynchronize(st));
     }
     hipFree(smax);
     hipFree(smax_off);
+    hipFree(uoff);
+    hipFree(ovcnt);
     // Rows whose columns scatter beyond the span (no locality) would make the aligned slots pad
     // without bound: past 1.5x the slots of a layout padded only to each slice's longest row, the
     // matrix keeps the value-dictionary row-block kernels.
-    if (rc == TE_OK && total < INT32_MAX && total <= total_max + total_max / 2) {
-        TE_HIP(ctx, hipMalloc(&v.sell_off, sizeof(int) * (ns + 1)));
+    if (rc == TE_OK && total < INT32_MAX && n_ov < INT32_MAX && total <= total_max + total_max / 2) {
+        TE_HIP(ctx, hipMalloc(&v.sell_meta, sizeof(int4) * (ns + 1)));
+        if (n_uni > 0) {
+            TE_HIP(ctx, hipMalloc(&v.sell_ufix, sizeof(int2) * (size_t)ns * kSellFix));
+            TE_HIP(ctx, hipMalloc(&v.sell_uni, sizeof(int2) * std::max<int64_t>(n_ov, 1)));
+        }
         // a slice's last group of slots may run up to kSellGroup - 1 slots past its end: zero
         // padding (words: column base + 0, value index 0, never added)
         const int64_t pad = (int64_t)kSellGroup * kSlice;
@@ -1249,11 +1342,19 @@ static int build_sell(te_ctx* ctx, te_matrix* A) {
         TE_HIP(ctx, hipMalloc(&v.sell_base, sizeof(int) * (nslots + kSellGroup)));
         TE_HIP(ctx, hipMemsetAsync(v.sell_word + total, 0, sizeof(unsigned) * pad, st));
         TE_HIP(ctx, hipMemsetAsync(v.sell_base + nslots, 0, sizeof(int) * kSellGroup, st));
-        hipLaunchKernelGGL(sell_align_kernel<true>, dim3(ga), dim3(256), 0, st, v.ptr, v.col, v.vidx, n, ns, slen,
-                           nullptr, soff, v.sell_off, v.sell_word, v.sell_base);
+        sb.soff = soff;
+        sb.ovoff = ovoff;
+        sb.meta = v.sell_meta;
+        sb.word = v.sell_word;
+        sb.base = v.sell_base;
+        sb.ufix = v.sell_ufix;                             // null without shared slots
+        sb.uni = v.sell_uni;
+        hipLaunchKernelGGL(sell_align_kernel<true>, dim3(ga), dim3(256), 0, st, sb);
         TE_HIP(ctx, hipStreamSynchronize(st));
         v.n_slices = ns;
         v.sell_entries = total;
+        v.sell_shared = n_uni;
+        v.sell_overflow = n_ov;
         // Three resident blocks per CU (12 waves): the waves of an XCD sweep its slices together,
         // and a narrow window of slices in flight keeps more of x's lines in the XCD's L2. On the
         // L = N = 14 ring (apply, µs): 1296, 826, 806, 867, 885 for 1..5 blocks per CU (5 is the
@@ -1279,6 +1380,9 @@ static int build_sell(te_ctx* ctx, te_matrix* A) {
     }
     hipFree(slen);
     hipFree(soff);
+    hipFree(ucnt);
+    hipFree(ovoff);
+    hipFree(usplit);
     return rc;
 }
 
