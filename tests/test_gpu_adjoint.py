@@ -197,6 +197,27 @@ def test_adjoint_eigen_solvers_agree(gpu, K, monkeypatch):
     assert rel <= 1e-10
 
 
+@pytest.mark.parametrize("m", [40, 48])
+def test_adjoint_node_states_on_matrix_cores(gpu, monkeypatch, m):
+    """The node states formed on the matrix cores (materialise_mfma_kernel, the default: 4-deep
+    FP64 MFMA steps over the basis columns) against the vector form (TE_ADJ_MAT=valu, one FMA per
+    column): the sums differ only in their order, so the gradients agree to rounding, and the
+    loss (the forward pass) is the same bit for bit. m = 48 takes the wider coefficient table."""
+    L = N = 8
+    T = 2.0
+    theta = _theta(L, 37)
+    prob = ring_problem(gpu, L, N, 0.5, T, tol=1e-8, m=m, checkpoint_interval=1, theta0=theta)
+    psi0 = fock_state(prob.basis.D)
+    l1, g1, _, i1 = prob.loss_and_grad(theta, psi0)
+    monkeypatch.setenv("TE_ADJ_MAT", "valu")
+    l0, g0, _, i0 = prob.loss_and_grad(theta, psi0)
+    rel = np.linalg.norm(np.asarray(g1) - np.asarray(g0)) / np.linalg.norm(g0)
+    print("m=%d matrix cores vs vector form: rel diff %.2e" % (m, rel))
+    assert l1 == l0
+    assert i1["nodes"] == i0["nodes"]
+    assert rel <= 1e-12
+
+
 def test_adjoint_krylov_dimension_above_40(gpu):
     """m = 48 (the coefficient and rank kernels' wider form) against the dense Frechet gradient."""
     L = N = 6

@@ -163,6 +163,84 @@ __global__ __launch_bounds__(256) void materialise_nodes_kernel(MatArgs a) {
     }
 }
 
+// The same pass on the matrix cores: per 16-row tile the complex product U[:, rows]^T C (m
+// basis columns by the pass's nodes, and the end vector as one more node) as a real product of
+// the 16 x 2m matrix A (A[r][2j] = Re U[j][r], A[r][2j+1] = Im U[j][r]) by the 2m x 2(nodes + 1)
+// matrix B (B[2j][2q] = Re C[q][j], B[2j][2q+1] = Im C[q][j], B[2j+1][2q] = -Im C[q][j],
+// B[2j+1][2q+1] = Re C[q][j]), in output tiles of 16 columns (8 nodes) by
+// v_mfma_f64_16x16x4f64: lane l holds A[l % 16][l / 16] and B[l / 16][l % 16] of each 4-deep
+// step and D[4 i + l / 16][l % 16] of the tile. A wavefront loads the tile's 2m x 16 basis
+// values once (lanes 0-31 one column, 32-63 the next: 256 contiguous bytes each) and reads B
+// from the coefficient table in LDS. The sums run over j in a different order from the vector
+// form's (4-deep steps): the node states agree with it to rounding.
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+template <int MS>
+__global__ __launch_bounds__(256) void materialise_mfma_kernel(MatArgs a) {
+    __shared__ double2 cs[MS * (kMatNodes + 1)];              // cs[j * (nq + 1) + q], q = nq: cend
+    const int nn = a.step->n_nodes;
+    const int m = a.step->m[a.basis];
+    const int nloc = max(0, min(kMatNodes, nn - a.node0));
+    const int ng = (nloc + kNodeGroup - 1) / kNodeGroup;     // blocks written by this pass
+    const int nq = ng * kNodeGroup;
+    for (int i = threadIdx.x; i < m * (nq + 1); i += blockDim.x) {
+        const int j = i / (nq + 1), q = i % (nq + 1);
+        double2 c = make_double2(0.0, 0.0);
+        if (q < nloc) c = a.coef[(size_t)(a.node0 + q) * kMaxKrylov + j];
+        else if (q == nq && a.end) c = a.cend[j];
+        cs[i] = c;
+    }
+    __syncthreads();
+    const int ncol = 2 * (nq + (a.end ? 1 : 0));                // real output columns
+    const int ntile = (ncol + 15) / 16;
+    if (ntile == 0) return;
+    const int ks = (m + 1) / 2;                                 // 4-deep steps (2 basis columns each)
+    const int lane = threadIdx.x & 63;
+    const int waves = blockDim.x >> 6;
+    const int jl = lane >> 5, part = (lane >> 4) & 1;           // A and B: column 2s + jl, re/im part
+    const int64_t tiles = (a.n + 15) / 16;
+    const double* U = reinterpret_cast<const double*>(a.U);
+    for (int64_t t = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * waves + (threadIdx.x >> 6))); t < tiles;
+         t += (int64_t)gridDim.x * waves) {
+        const int64_t row = t * 16 + (lane & 15);
+        double av[MS / 2 + 1];
+#pragma unroll
+        for (int st = 0; st < MS / 2 + 1; st++) {
+            const int j = 2 * st + jl;
+            av[st] = (st < ks && j < m && row < a.n) ? U[((size_t)j * a.ld + row) * 2 + part] : 0.0;
+        }
+        for (int T = 0; T < ntile; T++) {
+            const int col = 16 * T + (lane & 15);
+            const int q = col >> 1, p = col & 1;
+            const bool qin = q <= nq;                           // q == nq: the end vector
+            d4v acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int st = 0; st < MS / 2 + 1; st++) {
+                if (st < ks) {                                  // the same for the whole wavefront
+                    const int j = 2 * st + jl;
+                    double b = 0.0;
+                    if (qin && j < m) {
+                        const double2 c = cs[j * (nq + 1) + q];
+                        b = part == 0 ? (p == 0 ? c.x : c.y) : (p == 0 ? -c.y : c.x);
+                    }
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[st], b, acc, 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int64_t r = t * 16 + 4 * i + (lane >> 4);
+                if (r >= a.n || !qin) continue;
+                if (q < nq) {
+                    double* o = reinterpret_cast<double*>(a.out + ((size_t)(q >> 3) * a.n + r) * kNodeGroup + (q & 7));
+                    o[p] = acc[i];
+                } else if (a.end) {
+                    reinterpret_cast<double*>(a.end + r)[p] = acc[i];
+                }
+            }
+        }
+    }
+}
+
 // The node blocks of one materialisation pass (the step's nodes [node0, node0 + 64), count from
 // the step record): for every row r and node q, L_q = w_q conj(lambda_q(r)),
 //   Z(r) += sum_q Im(L_q psi_q(r))                              (diagonal generators)
@@ -971,6 +1049,10 @@ static int backward(te_adjoint* a) {
     // TE_ADJ_TERM_BLOCKS_PER_CU, 0: the streaming kernels' grid, also the largest). On the
     // L = N = 12 ring, K = 1: 2 blocks per CU for the term-wise pass took 72.4 ms a run against
     // 74.0 ms with the streaming grid (4); 2, 4 or 8 for the materialisation within 0.1 ms.
+    // node states on the matrix cores (materialise_mfma_kernel), or with TE_ADJ_MAT=valu (read at
+    // every run) by the vector form
+    const char* mat_env = std::getenv("TE_ADJ_MAT");
+    const bool mat_mfma = !(mat_env && std::strcmp(mat_env, "valu") == 0);
     static const int mat_per_cu = [] {
         const char* v = std::getenv("TE_ADJ_MAT_BLOCKS_PER_CU");
         return v ? std::max(1, std::atoi(v)) : 8;
@@ -1125,7 +1207,13 @@ static int backward(te_adjoint* a) {
                        node0 == 0 ? a->lam_next : nullptr};
             // m <= m0: a 40-row coefficient table (41.6 KB) lets the blocks share CUs with the
             // term-wise pass's
-            if (a->m0 <= 40) {
+            if (mat_mfma && a->m0 <= 40) {
+                hipLaunchKernelGGL(materialise_mfma_kernel<40>, dim3(mg), dim3(256), 0, s, mp);
+                hipLaunchKernelGGL(materialise_mfma_kernel<40>, dim3(mg), dim3(256), 0, s, ml);
+            } else if (mat_mfma) {
+                hipLaunchKernelGGL(materialise_mfma_kernel<kMaxKrylov>, dim3(mg), dim3(256), 0, s, mp);
+                hipLaunchKernelGGL(materialise_mfma_kernel<kMaxKrylov>, dim3(mg), dim3(256), 0, s, ml);
+            } else if (a->m0 <= 40) {
                 hipLaunchKernelGGL(materialise_nodes_kernel<40>, dim3(mg), dim3(256), 0, s, mp);
                 hipLaunchKernelGGL(materialise_nodes_kernel<40>, dim3(mg), dim3(256), 0, s, ml);
             } else {
