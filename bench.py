@@ -260,8 +260,25 @@ def bench_batched(np, capi, ctx, args):
     ctx.synchronize()
     t_prof = time.perf_counter() - t
     kt = be.kernel_times()
+    per_launch = be.kernel_launch_times()
+    rounds_prof = be.rounds()
     be.set_profiling(False)
     be.close()
+    # per round: the m Lanczos iterations' applies and projections, the controller, the combine
+    # that follows it (the first combine takes the t = 0 sample); live instances and rows from
+    # the instances' Krylov step counts (instance i steps in rounds 0 .. n_steps_i - 1)
+    per_it = len(per_launch["apply"]) // max(rounds_prof, 1)
+    per_round = {
+        "live": [sum(1 for r in rb if r["n_steps"] > q) for q in range(rounds_prof)],
+        "live_rows": [sum(d for r, d in zip(rb, dims) if r["n_steps"] > q) for q in range(rounds_prof)],
+        "apply_ms": [round(float(per_launch["apply"][q * per_it:(q + 1) * per_it].sum()), 3)
+                     for q in range(rounds_prof)],
+        "project_ms": [round(float(per_launch["project"][q * per_it:(q + 1) * per_it].sum()), 3)
+                       for q in range(rounds_prof)],
+        "control_ms": [round(float(x), 3) for x in per_launch["control"]],
+        "combine_ms": [round(float(x), 3) for x in per_launch["combine"][1:]],
+        "t0_combine_ms": round(float(per_launch["combine"][0]), 3) if len(per_launch["combine"]) else None,
+    }
 
     evs = [capi.Evolver(ctx, H, p, T, step, o, tol=tol, m=m) for H, p, o in zip(Hs, psis, obs)]
     evs[0].run(want_state=False)                         # warmup
@@ -298,6 +315,7 @@ def bench_batched(np, capi, ctx, args):
                                "iteration) and projections (48 D) over their event-timed totals (finish launches "
                                "included); an instance that broke down moved fewer",
             "kernel_ms_per_run": {k: round(v[0], 3) for k, v in kt.items()},
+            "per_round": per_round,
             "kernel_launches_per_run": {k: v[1] for k, v in kt.items()},
             "profiled_run_s": round(t_prof, 4),
             "kernel_note": "one more run with HIP events around every launch kind (apply and projection with "

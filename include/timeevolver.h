@@ -236,8 +236,10 @@ int te_batch_set_graph(te_batch* b, int enable);
  * iteration of every instance each: unit_slices in {16, 32, 64, 128}, vec_rows in {1024, 2048,
  * 4096, 8192}, blocks per CU in {4, 8, 16}), keeps the fastest and caches it for the process
  * under the instances' sorted (dimension, nnz) pairs; TE_BATCH_TUNE=0 in the environment keeps
- * the fixed defaults instead. The shape changes no result bit: every per-instance sum adds
- * partials of 4 slices and of 512 rows in index order. te_batch_set_shape fixes a shape (tuner off); all four
+ * the fixed defaults instead. Each round's launches cover the units of the instances still
+ * live, cut into smaller pieces as fewer remain (TE_BATCH_PLAN in the environment: units per CU
+ * to aim for, default 8, 0 never cuts). Neither the shape nor the cuts change a result bit: every
+ * per-instance sum adds partials of 4 slices and of 512 rows in index order. te_batch_set_shape fixes a shape (tuner off); all four
  * arguments 0 turn the tuner back on for the next run. te_batch_shape reports the shape in use,
  * source (0 defaults, 1 measured, 2 the process cache, 3 set by the caller) and the time the
  * measurement took (tune_ms, 0 unless measured); any pointer may be NULL. */
@@ -250,9 +252,12 @@ int te_batch_shape(const te_batch* b, int* unit_slices, int* vec_rows, int* spmv
  * kernel by kernel instead of replayed from the graph (same results). kind: 0 Lanczos apply of
  * every instance with the launch that sums its partials, 1 projection with its sums, 2 step
  * controller (one block per instance), 3 combine (state update, samples, observable sums, live
- * count). total_ms and launches cover the last run. */
+ * count). total_ms and launches cover the last run; te_batch_kernel_times gives the last run's
+ * launches of one kind one by one, in launch order (ms[0 .. min(cap, *launches)); ms may be NULL
+ * with cap 0 to ask for the count). Apply and projection launches come m_max to a round. */
 int te_batch_set_profiling(te_batch* b, int kinds);
 int te_batch_kernel_time(const te_batch* b, int kind, double* total_ms, size_t* launches);
+int te_batch_kernel_times(const te_batch* b, int kind, double* ms, size_t cap, size_t* launches);
 
 /* ---- adjoint gradient (no reference counterpart: the reference only evolves forward) ---- */
 typedef struct te_adjoint te_adjoint;

@@ -218,6 +218,50 @@ def test_batch_tuned_shape_is_bitwise_the_default(gpu, monkeypatch):
         assert np.linalg.norm(a["state"] - c["state"]) <= a["err"] + c["err"] + 1e-10
 
 
+def test_batch_round_plan_is_bitwise(gpu, monkeypatch):
+    """Each round's launches go over a plan of the live instances' units (batch_plan_kernel),
+    cut into smaller pieces as the live set shrinks. Instances of different sizes finish in
+    different rounds here; the batch planned with no cutting (TE_BATCH_PLAN=0), with the default
+    target, and with every unit cut to its smallest pieces (a huge target), launched from the
+    captured round graph or kernel by kernel, gives the same bits, on the sliced-ELL path and on
+    the shared-matrix groups."""
+    specs = _seeded_specs(505, 20, (5, 6, 7, 8, 9))
+    L0, N0, eps0, J0, U0, _ = batch_instance(506, 0, (8,))
+    specs += [(L0, N0, eps0, J0, U0, (lambda D, s=k: random_state(D, 9000 + s))) for k in range(4)]
+    inst = _instances(gpu, specs, (0,))
+    # the last four instances share one matrix: same operator handle for all of them
+    Hs = [x[1] for x in inst[:-4]] + [inst[-4][1]] * 4
+    obs = [x[2] for x in inst[:-4]] + [inst[-4][2]] * 4
+    args = (Hs, [x[4] for x in inst], 2.0, 0.25, obs)
+    runs = []
+    for plan in (None, "0", "1000000"):
+        if plan is None:
+            monkeypatch.delenv("TE_BATCH_PLAN", raising=False)
+        else:
+            monkeypatch.setenv("TE_BATCH_PLAN", plan)
+        be = capi.BatchEvolver(gpu, *args, tol=1e-8, m=30)
+        runs.append(be.run())
+        rounds = be.rounds()
+        be.set_profiling(True)                          # kernel by kernel, no graph replay
+        runs.append(be.run())
+        kt, each = be.kernel_times(), be.kernel_launch_times()
+        for name, (total, n) in kt.items():             # launch by launch, in launch order
+            assert len(each[name]) == n and abs(each[name].sum() - total) <= 1e-9 * max(total, 1.0)
+        assert len(each["apply"]) == len(each["project"]) == be.rounds() * 30
+        assert len(each["control"]) == be.rounds() and len(each["combine"]) == be.rounds() + 1
+        be.close()
+    monkeypatch.delenv("TE_BATCH_PLAN", raising=False)
+    steps = [r["n_steps"] for r in runs[0]]
+    print("rounds", rounds, "steps per instance", steps)
+    assert rounds > 1 and len(set(steps)) > 1           # the live set shrinks over the rounds
+    for rr in runs[1:]:
+        for a, c in zip(runs[0], rr):
+            assert a["n_steps"] == c["n_steps"] and a["statusCode"] == c["statusCode"]
+            assert np.float64(a["err"]).view(np.uint64) == np.float64(c["err"]).view(np.uint64)
+            assert np.array_equal(a["state"].view(np.uint64), c["state"].view(np.uint64))
+            assert np.array_equal(a["observables"].view(np.uint64), c["observables"].view(np.uint64))
+
+
 def test_batch_shared_matrix_groups(gpu, oracle_lib, monkeypatch):
     """Instances that share one matrix (many initial states of one Hamiltonian) go through the
     apply four or two at a time, one pass over the matrix words (groups 4 + 2 + single here,

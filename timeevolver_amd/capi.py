@@ -101,6 +101,7 @@ _SIGS = {
     "te_batch_set_shape": (_c.c_int, [_p, _c.c_int, _c.c_int, _c.c_int, _c.c_int]),
     "te_batch_shape": (_c.c_int, [_p, _ip, _ip, _ip, _ip, _ip, _dp]),
     "te_batch_kernel_time": (_c.c_int, [_p, _c.c_int, _dp, _szp]),
+    "te_batch_kernel_times": (_c.c_int, [_p, _c.c_int, _dp, _c.c_size_t, _szp]),
     "te_adjoint_create": (_c.c_int, [_p, _p, _c.c_int, _pp, _p, _c.POINTER(te_evolve_params), _c.c_int, _pp]),
     "te_adjoint_destroy": (None, [_p]),
     "te_adjoint_run": (_c.c_int, [_p, _dp, _dp, _dp, _c.POINTER(te_result), _c.POINTER(te_adjoint_info)]),
@@ -508,6 +509,18 @@ class BatchEvolver:
             ms, n = _c.c_double(), _c.c_size_t()
             self.ctx.check(self.ctx.L.te_batch_kernel_time(self.h, k, _c.byref(ms), _c.byref(n)), "kernel_time")
             out[name] = (ms.value, n.value)
+        return out
+
+    def kernel_launch_times(self):
+        """{kind: array of the last run's launch times in ms, in launch order}."""
+        out = {}
+        for k, name in enumerate(self.KINDS):
+            n = _c.c_size_t()
+            self.ctx.check(self.ctx.L.te_batch_kernel_times(self.h, k, None, 0, _c.byref(n)), "kernel_times")
+            ms = np.zeros(n.value)
+            self.ctx.check(self.ctx.L.te_batch_kernel_times(self.h, k, _ptr(ms, _c.c_double), n.value, _c.byref(n)),
+                           "kernel_times")
+            out[name] = ms
         return out
 
     def rounds(self):

@@ -86,6 +86,12 @@ struct BatchDev {
     const int4* vgroup;     // shared-matrix groups: up to kBatchNV instances (-1 past the group)
     const int4* vunit;      // (instance, first row, end row, 0), rows local to the instance; rows
     int n_vunits;           // start on multiples of kBatchChunk
+    // A round's plan (batch_plan_kernel): the units of its live instances, in table order and
+    // split into pieces when few are live; counts {sliced ELL, shared matrix, vector}.
+    int4* lsunit;
+    int4* lgunit;
+    int4* lvunit;
+    int* n_plan;
     double2* U;             // packed Krylov bases, column-major, ld rows per column
     int64_t ld;
     double2 alpha;          // -i expFactor
@@ -238,9 +244,9 @@ __global__ __launch_bounds__(kBlock) void batch_apply_group_kernel(BatchDev a, i
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const double2* staged = nullptr;
     int u0, u1, us;
-    xcd_sweep(a.n_gunits, u0, u1, us);
+    xcd_sweep(a.n_plan[1], u0, u1, us);
     for (int u = u0; u < u1; u += us) {
-        const int4 su = a.gunit[u];
+        const int4 su = a.lgunit[u];
         const int4 gi = a.vgroup[su.w - 1];
         const bool any = lanczos_active(a, gi.x, j) || (gi.y >= 0 && lanczos_active(a, gi.y, j)) ||
                          (gi.z >= 0 && lanczos_active(a, gi.z, j)) || (gi.w >= 0 && lanczos_active(a, gi.w, j));
@@ -257,14 +263,18 @@ __global__ __launch_bounds__(kBlock) void batch_apply_group_kernel(BatchDev a, i
     }
 }
 
-__global__ __launch_bounds__(kBlock) void batch_apply_sell_kernel(BatchDev a, int j) {
+// Four wavefronts per SIMD (at most 128 registers): the slices' chains of loads are latency
+// bound, and on 512 ring instances with L in {6..10} the apply took 92.7-93.8 ms a run against
+// 97.8 ms with the 143 registers the compiler otherwise takes (3 per SIMD); the few values it
+// then keeps in scratch are the unit's, outside the slice loop.
+__global__ __launch_bounds__(kBlock, 4) void batch_apply_sell_kernel(BatchDev a, int j) {
     __shared__ double2 vt[kVdMaxValues];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const double2* staged = nullptr;
     int u0, u1, us;
-    xcd_sweep(a.n_sunits, u0, u1, us);
+    xcd_sweep(a.n_plan[0], u0, u1, us);
     for (int u = u0; u < u1; u += us) {
-        const int4 su = a.sunit[u];
+        const int4 su = a.lsunit[u];
         const int i = su.x;
         if (!lanczos_active(a, i, j)) continue;
         const BatchInst& I = a.inst[i];
@@ -303,9 +313,9 @@ __global__ __launch_bounds__(kBlock) void batch_apply_sell_kernel(BatchDev a, in
 __global__ __launch_bounds__(kBlock) void batch_project_kernel(BatchDev a, int j) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     int u0, u1, us;
-    xcd_sweep(a.n_vunits, u0, u1, us);
+    xcd_sweep(a.n_plan[2], u0, u1, us);
     for (int u = u0; u < u1; u += us) {
-        const int4 vu = a.vunit[u];
+        const int4 vu = a.lvunit[u];
         const int i = vu.x;
         if (!lanczos_active(a, i, j)) continue;
         const BatchInst& I = a.inst[i];
@@ -397,9 +407,9 @@ __global__ __launch_bounds__(kBlock) void batch_combine_kernel(BatchDev a) {
     __shared__ double2 cf[(1 + kBatchSC) * kMaxKrylov];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     int u0, u1, us;
-    xcd_sweep(a.n_vunits, u0, u1, us);
+    xcd_sweep(a.n_plan[2], u0, u1, us);
     for (int u = u0; u < u1; u += us) {
-        const int4 vu = a.vunit[u];
+        const int4 vu = a.lvunit[u];
         const int i = vu.x;
         if (!a.bc[i].stepped) continue;
         const int m = a.bc[i].cm;
@@ -515,6 +525,87 @@ __global__ __launch_bounds__(kBlock) void batch_obs_finish_kernel(BatchDev a) {
     }
 }
 
+// A round's plan: the unit tables filtered to the instances live at the round's start (vector
+// units also to those whose step the combine may still take), in table order. When fewer units
+// than `target` are left, each is cut f = ceil(target / left) ways into pieces of whole
+// reduction groups (kBatchGroup slices) or chunks (kBatchChunk rows), at least one of them a
+// wavefront, so that the launches of a shrinking live set still spread over every XCD (each
+// takes an eighth of the planned list) and CU. Partials are per group and per chunk: the plan
+// changes no result bit. One block; the sliced-ELL list only on the sliced-ELL path.
+constexpr int kPlanThreads = 1024;
+constexpr int kPlanUnitsPerCU = 8;      // the target: planned units per CU
+
+// Exclusive prefix sum of v over the block's threads in thread order; *total gets the sum.
+__device__ int plan_scan(int v, int* sh, int* total) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int d = 1; d < kPlanThreads; d <<= 1) {
+        const int add = t >= d ? sh[t - d] : 0;
+        __syncthreads();
+        sh[t] += add;
+        __syncthreads();
+    }
+    const int incl = sh[t];
+    *total = sh[kPlanThreads - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+// Piece length for a unit of len items cut f ways: whole blocks of `align` items, at least
+// min_len items (one piece when the unit is shorter).
+__device__ __forceinline__ int plan_piece(int len, int f, int min_len, int align) {
+    const int p = ((len + f - 1) / f + align - 1) / align * align;
+    return max(p, min_len);
+}
+
+// The pieces of the live units of in[0, n) to out (capacity: one piece per align items of the
+// table), their number to *count. Thread t takes a contiguous run of units.
+template <class Live>
+__device__ void plan_list(const int4* in, int n, int4* out, int* count, int target, int min_len, int align,
+                          Live live, int* sh) {
+    const int per = (n + kPlanThreads - 1) / kPlanThreads;
+    const int u0 = min(n, (int)threadIdx.x * per), u1 = min(n, u0 + per);
+    int c = 0;
+    for (int u = u0; u < u1; u++) c += live(in[u]) ? 1 : 0;
+    int left = 0;
+    plan_scan(c, sh, &left);
+    const int f = (left == 0 || left >= target) ? 1 : (target + left - 1) / left;
+    int np = 0;
+    for (int u = u0; u < u1; u++) {
+        const int4 e = in[u];
+        if (!live(e)) continue;
+        const int len = e.z - e.y, pl = plan_piece(len, f, min_len, align);
+        np += (len + pl - 1) / pl;
+    }
+    int total = 0;
+    int o = plan_scan(np, sh, &total);
+    for (int u = u0; u < u1; u++) {
+        const int4 e = in[u];
+        if (!live(e)) continue;
+        const int pl = plan_piece(e.z - e.y, f, min_len, align);
+        for (int k = e.y; k < e.z; k += pl) out[o++] = make_int4(e.x, k, min(k + pl, e.z), e.w);
+    }
+    if (threadIdx.x == 0) *count = total;
+}
+
+__global__ __launch_bounds__(kPlanThreads) void batch_plan_kernel(BatchDev a, int target, int sell) {
+    __shared__ int sh[kPlanThreads];
+    const BatchCtl* bc = a.bc;
+    const int4* vg = a.vgroup;
+    constexpr int kWaves = kBlock / 64;
+    plan_list(a.sunit, sell ? a.n_sunits : 0, a.lsunit, a.n_plan + 0, target, kWaves * kBatchGroup, kBatchGroup,
+              [&](int4 e) { return bc[e.x].live != 0; }, sh);
+    plan_list(a.gunit, a.n_gunits, a.lgunit, a.n_plan + 1, target, kWaves * kBatchGroup, kBatchGroup,
+              [&](int4 e) {
+                  const int4 g = vg[e.w - 1];
+                  return bc[g.x].live != 0 || (g.y >= 0 && bc[g.y].live != 0) || (g.z >= 0 && bc[g.z].live != 0) ||
+                         (g.w >= 0 && bc[g.w].live != 0);
+              }, sh);
+    plan_list(a.vunit, a.n_vunits, a.lvunit, a.n_plan + 2, target, kWaves * kBatchChunk, kBatchChunk,
+              [&](int4 e) { return bc[e.x].live != 0 || bc[e.x].stepped != 0; }, sh);
+}
+
 // Number of live instances, read by the host after each round.
 __global__ __launch_bounds__(kBlock) void batch_count_kernel(BatchDev a) {
     __shared__ double2 sh[kBlock / 64];
@@ -574,6 +665,8 @@ struct te_batch {
     // device buffers
     BatchInst* inst = nullptr;
     int4 *sunit = nullptr, *gunit = nullptr, *vunit = nullptr;
+    int4 *lsunit = nullptr, *lgunit = nullptr, *lvunit = nullptr;   // a round's plan
+    int* n_plan = nullptr;
     int grid_g = 8;
     double2 *U = nullptr, *psi0 = nullptr, *sampled = nullptr, *wcoef = nullptr, *Y = nullptr;
     double2 *spart = nullptr, *vpart = nullptr;
@@ -598,6 +691,7 @@ struct te_batch {
     size_t prof_used[4] = {0, 0, 0, 0};
     double prof_ms[4] = {0, 0, 0, 0};
     size_t prof_n[4] = {0, 0, 0, 0};
+    std::vector<double> prof_each[4];  // the last run's launches, in launch order
 };
 
 namespace te {
@@ -611,7 +705,8 @@ static void batch_free(te_batch* b) {
     if (b->graph) hipGraphDestroy(b->graph);
     void* ptrs[] = {b->inst, b->sunit,   b->gunit, b->vgroup, b->vunit,  b->U,  b->psi0, b->sampled, b->wcoef, b->Y, b->tmp,
                     b->spart, b->vpart,  b->ks, b->st, b->bc, b->consts,
-                    b->diag, b->opart,   b->obs,    b->nrm, b->trace, b->n_live};
+                    b->diag, b->opart,   b->obs,    b->nrm, b->trace, b->n_live, b->lsunit, b->lgunit,
+                    b->lvunit, b->n_plan};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (b->n_live_host) hipHostFree(b->n_live_host);
@@ -644,12 +739,24 @@ static void prof_collect(te_batch* b) {
     for (int k = 0; k < 4; k++) {
         b->prof_ms[k] = 0.0;
         b->prof_n[k] = b->prof_used[k] / 2;
+        b->prof_each[k].clear();
         for (size_t i = 0; i + 1 < b->prof_used[k]; i += 2) {
             float ms = 0.f;
             hipEventElapsedTime(&ms, b->prof_ev[k][i], b->prof_ev[k][i + 1]);
             b->prof_ms[k] += ms;
+            b->prof_each[k].push_back(ms);
         }
     }
+}
+
+// The plan of the next round (or of the initial combine, or of a tuner's candidate shape).
+// TE_BATCH_PLAN sets the planned units per CU to aim for (0: live units are never split; read
+// at every plan, so a captured round keeps the value of its capture).
+static void launch_plan(const te_batch* b, hipStream_t s) {
+    int per_cu = kPlanUnitsPerCU;
+    if (const char* e = std::getenv("TE_BATCH_PLAN")) per_cu = std::min(std::max(0, std::atoi(e)), 1 << 20);
+    hipLaunchKernelGGL(batch_plan_kernel, dim3(1), dim3(kPlanThreads), 0, s, b->dev, b->ctx->num_cus * per_cu,
+                       b->sell ? 1 : 0);
 }
 
 // The apply of iteration j for every instance: single-instance units, then shared-matrix units.
@@ -667,6 +774,7 @@ static void launch_batch_apply(const te_batch* b, hipStream_t s, int j) {
 // keeps the round out of a graph).
 static void batch_round(te_batch* b, hipStream_t s) {
     const BatchDev& d = b->dev;
+    launch_plan(b, s);
     for (int j = 0; j < b->m_max; j++) {
         prof_begin(b, 0, s);
         launch_batch_apply(b, s, j);
@@ -814,6 +922,7 @@ static int tune_shape(te_batch* b) {
     hipEvent_t ev[2 * (reps + 1)];
     for (auto& e : ev) TE_HIP(ctx, hipEventCreate(&e));
     auto time_kind = [&](bool spmv, float* med) -> int {
+        launch_plan(b, s);                       // the candidate's units
         for (int r = 0; r <= reps; r++) {
             TE_HIP(ctx, hipEventRecord(ev[2 * r], s));
             if (spmv) {
@@ -1120,6 +1229,12 @@ extern "C" int te_batch_create(te_ctx* ctx, int n_inst, te_matrix* const* H, con
     TE_TRY(balloc(ctx, &b->nrm, n_inst));
     TE_TRY(balloc(ctx, &b->trace, (size_t)n_inst * kBatchTraceCap));
     TE_TRY(balloc(ctx, &b->n_live, 1));
+    // a round's plan: at most one piece per group of slices and per chunk of rows
+    TE_TRY(balloc(ctx, &b->lsunit, b->sell ? (size_t)b->total_groups : 1));
+    TE_TRY(balloc(ctx, &b->lgunit, b->sell ? (size_t)b->total_groups : 1));
+    TE_TRY(balloc(ctx, &b->lvunit, (size_t)b->total_chunks));
+    TE_TRY(balloc(ctx, &b->n_plan, 3));
+    TE_TRYH(hipMemsetAsync(b->n_plan, 0, sizeof(int) * 3, ctx->stream));
     TE_TRYH(hipHostMalloc((void**)&b->n_live_host, sizeof(int)));
     TE_TRY(upload_quadrature(ctx, b->quad));
 
@@ -1178,6 +1293,10 @@ extern "C" int te_batch_create(te_ctx* ctx, int n_inst, te_matrix* const* H, con
     d.gunit = b->gunit;
     d.n_sunits = (int)hsu.size();
     d.vunit = b->vunit;
+    d.lsunit = b->lsunit;
+    d.lgunit = b->lgunit;
+    d.lvunit = b->lvunit;
+    d.n_plan = b->n_plan;
     d.U = b->U;
     d.ld = total;
     d.alpha = make_double2(0.0, -1.0 * params->exp_factor);                               // :31
@@ -1274,6 +1393,14 @@ extern "C" int te_batch_kernel_time(const te_batch* b, int kind, double* total_m
     return TE_OK;
 }
 
+extern "C" int te_batch_kernel_times(const te_batch* b, int kind, double* ms, size_t cap, size_t* launches) {
+    if (!b || kind < 0 || kind > 3 || !launches || (cap > 0 && !ms)) return TE_ERR_ARG;
+    const std::vector<double>& v = b->prof_each[kind];
+    *launches = v.size();
+    for (size_t i = 0; i < std::min(cap, v.size()); i++) ms[i] = v[i];
+    return TE_OK;
+}
+
 #undef TE_TRY
 #undef TE_TRYH
 
@@ -1341,6 +1468,7 @@ extern "C" int te_batch_run(te_batch* b, double* obs_values, double* const* stat
         TE_HIP(ctx, hipStreamSynchronize(s));
     }
     TE_HIP(ctx, hipMemsetAsync(b->obs, 0, sizeof(double) * n_inst * std::max(b->nobs, 1) * ns, s));
+    launch_plan(b, s);                           // every instance takes the t = 0 sample
     batch_combine(b, s);
     TE_HIP(ctx, hipGetLastError());
     if (b->P.fast_integration)
