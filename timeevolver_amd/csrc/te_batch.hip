@@ -263,10 +263,15 @@ __global__ __launch_bounds__(kBlock) void batch_apply_group_kernel(BatchDev a, i
     }
 }
 
-// Four wavefronts per SIMD (at most 128 registers): the slices' chains of loads are latency
-// bound, and on 512 ring instances with L in {6..10} the apply took 92.7-93.8 ms a run against
-// 97.8 ms with the 143 registers the compiler otherwise takes (3 per SIMD); the few values it
-// then keeps in scratch are the unit's, outside the slice loop.
+// Four wavefronts per SIMD (at most 128 registers): a slice is a chain of dependent loads, so
+// more wavefronts hide more of it. With the 12 shared-record gathers the single-instance apply
+// issues at a slice's start the compiler takes 143 registers (3 per SIMD); the batch's instances
+// are small rings (L <= 10: at most 8 shared records a slice, 2.9 on average at L = 10), so a
+// slice issues kBatchIssue of them up front and the rest in groups of 4 in their order (the same
+// sums): 111 registers. On 512 ring instances with L in {6..10}: apply 97.8 ms a run at 143
+// registers, 83.9 at 128 (12 issued, 8 values spilled), 78.7 at 111 (4 issued); 2 issued at 5
+// waves per SIMD 78.5, 6 issued 79.2.
+constexpr int kBatchIssue = 4;
 __global__ __launch_bounds__(kBlock, 4) void batch_apply_sell_kernel(BatchDev a, int j) {
     __shared__ double2 vt[kVdMaxValues];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -299,7 +304,7 @@ __global__ __launch_bounds__(kBlock, 4) void batch_apply_sell_kernel(BatchDev a,
             const int gw = __builtin_amdgcn_readfirstlane(g);
             const int se = min((gw + 1) * kBatchGroup, su.z);
             epi.dot = make_double2(0.0, 0.0);
-            for (int sl = gw * kBatchGroup; sl < se; sl++) sell_slice(m, x, vt, sl, lane, epi);
+            for (int sl = gw * kBatchGroup; sl < se; sl++) sell_slice<ApplyEpiVd, kBatchIssue>(m, x, vt, sl, lane, epi);
             const double dx = wave_sum(epi.dot.x), dy = wave_sum(epi.dot.y);
             if (lane == 0) a.spart[I.sl0 + gw] = make_double2(dx, dy);   // the group's partial
         }

@@ -86,13 +86,16 @@ struct SellShared {
 // + sum_j vt[vidx_j] x[col_j] (diagonal first, then column order: leading shared slots, the
 // per-row slots, trailing shared slots; the same sums as the row-block kernel, and for each
 // vector the same as a pass of its own), handed to epi[v].row. Padding slots gather the slot's
-// base column and are not added. The first kIssue shared records of the slice are scalar loads
-// and their gathers, leading and trailing, are all issued before the per-row slots' first group. A group of kSellGroup per-row slots is loaded before any is used, and the next group's
-// slots load while the current group's gathers are in flight. vt: the value table in LDS.
-template <int NV, class Epi>
+// base column and are not added. The first KI shared records of the slice are scalar loads and
+// their gathers, leading and trailing, are all issued before the per-row slots' first group (the
+// rest follow in groups of 4 at their place in the sum; KI changes registers, not results). A
+// group of kSellGroup per-row slots is loaded before any is used, and the next group's slots load
+// while the current group's gathers are in flight. vt: the value table in LDS.
+template <int NV, class Epi, int KI = SellShared<NV>::kIssue>
 __device__ __forceinline__ void sell_slice_n(const SellView& m, const double2* const* x, const double2* vt, int s,
                                              int lane, Epi* epi) {
-    constexpr int kIssue = SellShared<NV>::kIssue;
+    constexpr int kIssue = KI;
+    static_assert(KI >= 1 && KI <= kSellFix, "the issued records sit at fixed places");
     const int4 h = scalar_load(m.meta + s);                        // scalar loads: wave-uniform
     const int o = h.x;
     const int ls = (scalar_load(reinterpret_cast<const int*>(m.meta + s + 1)) - o) / kSlice;   // per-row slots
@@ -213,12 +216,12 @@ __device__ __forceinline__ void sell_slice_n(const SellView& m, const double2* c
     }
 }
 
-// One vector: sell_slice_n with NV = 1.
-template <class Epi>
+// One vector: sell_slice_n with NV = 1 (KI shared records issued at the slice's start).
+template <class Epi, int KI = SellShared<1>::kIssue>
 __device__ __forceinline__ void sell_slice(const SellView& m, const double2* __restrict__ x, const double2* vt, int s,
                                            int lane, Epi& epi) {
     const double2* xs[1] = {x};
-    sell_slice_n<1>(m, xs, vt, s, lane, &epi);
+    sell_slice_n<1, Epi, KI>(m, xs, vt, s, lane, &epi);
 }
 
 // Lanczos apply epilogue on one row: w = c1 (A u_j)_r + c2 u_{j-1}[r] (non-temporal store),
