@@ -315,6 +315,9 @@ __global__ __launch_bounds__(kBlock, 4) void batch_apply_sell_kernel(BatchDev a,
 // unless j + 1 is the instance's Krylov dimension; breakdown when beta_j < tolRate
 // (core/krylovTimeEvolver.cpp:551-567). A wavefront takes a chunk of kBatchChunk rows (lane l:
 // rows l, l + 64, ..., in order) and writes the chunk's |w'|^2 as its partial.
+// u_j and w are read and w' written non-temporally (on 512 ring instances with L in {6..10} the
+// projection took 22.1-22.2 ms a run against 24.1 with ordinary loads and stores, and the next
+// apply, which gathers w', was no slower).
 __global__ __launch_bounds__(kBlock) void batch_project_kernel(BatchDev a, int j) {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     int u0, u1, us;
@@ -342,9 +345,9 @@ __global__ __launch_bounds__(kBlock) void batch_project_kernel(BatchDev a, int j
                 const int r = r0 + 64 * q;
                 uu[q] = make_double2(0.0, 0.0);
                 wr[q] = make_double2(0.0, 0.0);
-                if (r < vu.z) {
-                    uu[q] = uj[r];
-                    wr[q] = w[r];
+                if (r < vu.z) {                   // streamed: read and written once a pass
+                    uu[q] = load_stream(&uj[r]);
+                    wr[q] = load_stream(&w[r]);
                 }
             }
             double nrm = 0.0;
@@ -354,7 +357,7 @@ __global__ __launch_bounds__(kBlock) void batch_project_kernel(BatchDev a, int j
                 wr[q].x -= c3.x * uu[q].x - c3.y * uu[q].y;
                 wr[q].y -= c3.x * uu[q].y + c3.y * uu[q].x;
                 if (r < vu.z) {
-                    if (store) w[r] = wr[q];
+                    if (store) store_stream(&w[r], wr[q]);
                     nrm += wr[q].x * wr[q].x + wr[q].y * wr[q].y;
                 }
             }
@@ -471,16 +474,18 @@ __global__ __launch_bounds__(kBlock) void batch_combine_kernel(BatchDev a) {
                             }
                         }
                     };
-                    // kBatchBurst basis columns in flight per lane (sums still in order of jj)
+                    // kBatchBurst basis columns in flight per lane (sums still in order of jj), read
+                    // non-temporally: a column is read once a pass (combine 11.4-11.5 -> 10.7-10.9
+                    // ms a run of the 512-instance batch)
                     int jj = 0;
                     for (; jj + kBatchBurst <= m; jj += kBatchBurst) {
                         double2 uv[kBatchBurst];
 #pragma unroll
-                        for (int q = 0; q < kBatchBurst; q++) uv[q] = a.U[(size_t)(jj + q) * a.ld + gr];
+                        for (int q = 0; q < kBatchBurst; q++) uv[q] = load_stream(&a.U[(size_t)(jj + q) * a.ld + gr]);
 #pragma unroll
                         for (int q = 0; q < kBatchBurst; q++) add(jj + q, uv[q]);
                     }
-                    for (; jj < m; jj++) add(jj, a.U[(size_t)jj * a.ld + gr]);
+                    for (; jj < m; jj++) add(jj, load_stream(&a.U[(size_t)jj * a.ld + gr]));
                     if (ww) a.U[gr] = aw;
                     if (lastpass && sc > 0) {
                         double2 v = as[0];
