@@ -1036,17 +1036,22 @@ static void combine_step(te_evolver* ev, int m, long long base, int S, bool writ
     struct Launch {
         long long y0;
         int S, o0, no;
+        bool tiled;
     };
     std::vector<Launch> L;
     for (int o0 = 0; o0 < nd; o0 += combine_obs_per_launch()) {
         const int no = std::min(nd - o0, combine_obs_per_launch());
-        const int sc = combine_samples_per_launch(no);
-        for (int s0 = 0; s0 < S; s0 += sc) L.push_back({base + s0, std::min(sc, S - s0), o0, no});
+        int sc = combine_samples_per_launch(no);
+        // more samples than one pass of combine_kernel takes: the tiled pass, when it applies
+        const int tsc = combine_tiled_samples(no);
+        const bool tiled = S > sc && tsc > sc;
+        if (tiled) sc = tsc;
+        for (int s0 = 0; s0 < S; s0 += sc) L.push_back({base + s0, std::min(sc, S - s0), o0, no, tiled});
     }
     if (L.empty()) {
         if (!write_w && !(write_last && S > 0)) return;
-        if (S > 0) L.push_back({base + S - 1, 1, 0, 0});
-        else L.push_back({base, 0, 0, 0});
+        if (S > 0) L.push_back({base + S - 1, 1, 0, 0, false});
+        else L.push_back({base, 0, 0, 0, false});
     }
     for (size_t i = 0; i < L.size(); i++) {
         const bool last = i + 1 == L.size();
@@ -1062,7 +1067,8 @@ static void combine_step(te_evolver* ev, int m, long long base, int S, bool writ
         a.write_w = (last && write_w) ? 1 : 0;
         a.write_last = (last && write_last && S > 0) ? 1 : 0;
         a.last_out = ev->sampled;
-        launch_combine(a, ev->grid_stream, ev->ctx->stream);
+        if (L[i].tiled) launch_combine_tiled(a, ev->grid_stream, ev->ctx->stream);
+        else launch_combine(a, ev->grid_stream, ev->ctx->stream);
     }
 }
 

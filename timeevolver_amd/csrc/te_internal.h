@@ -211,7 +211,7 @@ void free_vd(te_matrix* A);
 void launch_spmv(const te_matrix* A, double2 alpha, const double2* x, double2* y, hipStream_t s);
 // Deterministic reductions with a last-block-done finish. partials: >= grid entries.
 struct ReduceScratch {
-    double2* partials = nullptr;   // grid * 32 doubles2 (observable chunks use up to 32 slots)
+    double2* partials = nullptr;   // grid * 128 doubles2 (a combine launch uses up to 128 slots a block)
     unsigned int* ticket = nullptr;
     double2* result = nullptr;     // 64 slots
     int grid = 0;

@@ -181,6 +181,10 @@ void launch_batch_step_control(const StepConsts* c, StepState* st, KrylovScalars
 int combine_samples_per_launch(int nobs);
 int combine_obs_per_launch();
 void launch_combine(const CombineArgs& a, int grid, hipStream_t s);
+// Samples a pass of the tiled combine (1 to 16 diagonal observables a launch; grid x 128
+// partials), or 0 when it does not apply.
+int combine_tiled_samples(int nobs);
+void launch_combine_tiled(const CombineArgs& a, int grid, hipStream_t s);
 void launch_store_obs(const double2* res, int slot, int mode, double* out, hipStream_t s);
 
 }  // namespace te

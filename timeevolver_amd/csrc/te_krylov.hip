@@ -1454,6 +1454,139 @@ __global__ __launch_bounds__(kBlock) void combine_finish_kernel(CombineArgs a, i
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// The same pass for a step with more samples than combine_kernel<SC, NOB> takes at once, for
+// up to kTiledObs diagonal observables: kTiledSC samples a pass, the observable sums through LDS
+// instead of per-thread accumulators (which limit combine_kernel to NOB * SC <= 32). A block's
+// rows go in tiles of kBlock, one row a thread as in combine_kernel (the same U w, U y_s and last
+// sampled state, column by column in order); per tile each thread puts its row's |psi_s(r)|^2 and
+// its observables' diagonal entries (IDX: their one-byte indices into the value tables; else the
+// 8-byte values) in LDS, then thread t adds, for observable t % kTiledObs and every sample, the
+// tile's rows of part t / kTiledObs in row order. After the last tile the parts of each
+// (observable, sample) pair are added in part order into the block's partial;
+// combine_tiled_finish_kernel adds the blocks' partials in block order. The two storage forms
+// add the same doubles in the same order.
+// ---------------------------------------------------------------------------------------
+constexpr int kTiledSC = 8;
+constexpr int kTiledObs = 16;
+constexpr int kTiledParts = kBlock / kTiledObs;          // row parts of a tile (16 rows each)
+constexpr int kTiledPairs = kTiledObs * kTiledSC;
+constexpr int kTabPad = kDiagTab + 1;                   // a table row per observable, padded for LDS banks
+
+template <bool IDX>
+__global__ __launch_bounds__(kBlock) void combine_tiled_kernel(CombineArgs a) {
+    __shared__ double2 cf[(1 + kTiledSC) * kMaxKrylov];
+    __shared__ double pt[kBlock * (kTiledSC + 1)];       // |psi_s(r)|^2, row-major, padded
+    __shared__ __align__(16) unsigned char ix[IDX ? kBlock * kTiledObs : 16];   // IDX: the rows' index bytes
+    __shared__ double tabs[IDX ? kTiledObs * kTabPad : kBlock * (kTiledObs + 1)];   // tables, or the rows' values
+    const int tid = threadIdx.x;
+    const int m = a.m, S = a.S;
+    if (IDX)
+        for (int i = tid; i < kTiledObs * kTabPad; i += kBlock) {
+            const int o = i / kTabPad, v = i % kTabPad;
+            tabs[i] = (o < a.nobs && v < kDiagTab) ? a.diag_tab[o * kDiagTab + v] : 0.0;
+        }
+    for (int i = tid; i < (1 + kTiledSC) * kMaxKrylov; i += kBlock) {
+        const int c = i / kMaxKrylov, j = i % kMaxKrylov;
+        double2 v = make_double2(0.0, 0.0);
+        if (j < m) {
+            if (c == 0) {
+                if (a.write_w) v = a.wcoef[j];
+            } else if (c - 1 < S) {
+                v = a.Y[(size_t)(a.y_base + c - 1) * kMaxKrylov + j];
+            }
+        }
+        cf[i] = v;
+    }
+    const int ro = tid % kTiledObs, rp = tid / kTiledObs;   // this thread's observable and row part
+    double acc[kTiledSC];
+#pragma unroll
+    for (int s = 0; s < kTiledSC; s++) acc[s] = 0.0;
+    __syncthreads();
+    for (int64_t t0 = (int64_t)blockIdx.x * kBlock; t0 < a.n; t0 += (int64_t)gridDim.x * kBlock) {
+        const int64_t row = t0 + tid;
+        double2 aw = make_double2(0.0, 0.0);
+        double2 as[kTiledSC];
+#pragma unroll
+        for (int s = 0; s < kTiledSC; s++) as[s] = make_double2(0.0, 0.0);
+        typedef unsigned u4v __attribute__((ext_vector_type(4)));
+        u4v idx = {0u, 0u, 0u, 0u};
+        if (row < a.n) {
+            auto add = [&](int j, double2 u) {
+                if (a.write_w) {
+                    const double2 t = cmul(u, cf[j]);
+                    aw.x += t.x;
+                    aw.y += t.y;
+                }
+#pragma unroll
+                for (int s = 0; s < kTiledSC; s++) {
+                    const double2 t = cmul(u, cf[(1 + s) * kMaxKrylov + j]);
+                    as[s].x += t.x;
+                    as[s].y += t.y;
+                }
+            };
+            constexpr int B = combine_burst<kTiledSC>();
+            int j = 0;
+            for (; j + B <= m; j += B) {
+                double2 u[B];
+#pragma unroll
+                for (int q = 0; q < B; q++) u[q] = load_stream(&a.U[(size_t)(j + q) * a.ld + row]);
+#pragma unroll
+                for (int q = 0; q < B; q++) add(j + q, u[q]);
+            }
+            for (; j < m; j++) add(j, load_stream(&a.U[(size_t)j * a.ld + row]));
+            if (a.write_w) a.U0[row] = aw;
+            if (a.write_last && S > 0) {
+                double2 v = as[0];
+#pragma unroll
+                for (int s = 1; s < kTiledSC; s++)
+                    if (s == S - 1) v = as[s];
+                a.last_out[row] = v;
+            }
+            if (IDX)
+                idx = __builtin_nontemporal_load(
+                    reinterpret_cast<const u4v*>(a.diag_idx + (size_t)row * a.idx_stride));
+        }
+#pragma unroll
+        for (int s = 0; s < kTiledSC; s++) pt[tid * (kTiledSC + 1) + s] = as[s].x * as[s].x + as[s].y * as[s].y;
+        if (IDX) {
+            *reinterpret_cast<u4v*>(ix + tid * kTiledObs) = idx;
+        } else {
+#pragma unroll
+            for (int o = 0; o < kTiledObs; o++)
+                tabs[tid * (kTiledObs + 1) + o] = (row < a.n && o < a.nobs) ? a.diag[(size_t)o * a.n + row] : 0.0;
+        }
+        __syncthreads();
+        for (int k = 0; k < kBlock / kTiledParts; k++) {
+            const int i = rp * (kBlock / kTiledParts) + k;
+            const double d = IDX ? tabs[ro * kTabPad + ix[i * kTiledObs + ro]] : tabs[i * (kTiledObs + 1) + ro];
+#pragma unroll
+            for (int s = 0; s < kTiledSC; s++) acc[s] += d * pt[i * (kTiledSC + 1) + s];
+        }
+        __syncthreads();                                  // pt and ix are refilled by the next tile
+    }
+    // parts of each pair in part order (pt is free again: kTiledParts x kTiledPairs doubles fit)
+    static_assert(kTiledParts * kTiledPairs <= kBlock * (kTiledSC + 1), "the parts fit in pt");
+#pragma unroll
+    for (int s = 0; s < kTiledSC; s++) pt[rp * kTiledPairs + ro * kTiledSC + s] = acc[s];
+    __syncthreads();
+    if (tid < kTiledPairs) {
+        double t = 0.0;
+        for (int q = 0; q < kTiledParts; q++) t += pt[q * kTiledPairs + tid];
+        a.partials[(size_t)blockIdx.x * kTiledPairs + tid] = make_double2(t, 0.0);
+    }
+}
+
+// One block after combine_tiled_kernel: thread p adds pair p's block partials in block order.
+__global__ __launch_bounds__(kBlock) void combine_tiled_finish_kernel(CombineArgs a, int nparts) {
+    const int p = threadIdx.x;
+    if (p >= kTiledPairs) return;
+    double t = 0.0;
+    for (int b = 0; b < nparts; b++) t += a.partials[(size_t)b * kTiledPairs + p].x;
+    const int o = p / kTiledSC, s = p % kTiledSC;
+    if (o < a.nobs && s < a.S) a.obs_values[(size_t)(a.obs_base + o) * a.n_samples + a.y_base + s] = t;
+}
+
 __global__ void store_obs_kernel(const double2* res, int slot, int mode, double* out) {
     const double2 r = res[slot];
     *out = mode == 0 ? r.x : r.x * r.x + r.y * r.y;
@@ -1493,6 +1626,14 @@ void launch_combine(const CombineArgs& a, int grid, hipStream_t s) {
     else if (a.nobs <= 8) launch_combine_t<4, 8>(a, grid, s);
     else if (a.nobs <= 16) launch_combine_t<2, 16>(a, grid, s);
     else launch_combine_t<1, 32>(a, grid, s);
+}
+
+int combine_tiled_samples(int nobs) { return nobs >= 1 && nobs <= kTiledObs ? kTiledSC : 0; }
+
+void launch_combine_tiled(const CombineArgs& a, int grid, hipStream_t s) {
+    if (a.diag) hipLaunchKernelGGL(combine_tiled_kernel<false>, dim3(grid), dim3(kBlock), 0, s, a);
+    else hipLaunchKernelGGL(combine_tiled_kernel<true>, dim3(grid), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL(combine_tiled_finish_kernel, dim3(1), dim3(kBlock), 0, s, a, grid);
 }
 
 void launch_store_obs(const double2* res, int slot, int mode, double* out, hipStream_t s) {

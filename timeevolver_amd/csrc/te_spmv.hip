@@ -1578,7 +1578,7 @@ void launch_nrm2sq(const double2* x, int64_t n, ReduceScratch& rs, int slot, hip
 
 int reduce_scratch_alloc(te_ctx* ctx, ReduceScratch& rs, int grid) {
     rs.grid = grid;
-    TE_HIP(ctx, hipMalloc(&rs.partials, sizeof(double2) * (size_t)grid * 64));
+    TE_HIP(ctx, hipMalloc(&rs.partials, sizeof(double2) * (size_t)grid * 128));
     TE_HIP(ctx, hipMalloc(&rs.ticket, sizeof(unsigned int) * 4));
     TE_HIP(ctx, hipMalloc(&rs.result, sizeof(double2) * 64));
     TE_HIP(ctx, hipMemset(rs.ticket, 0, sizeof(unsigned int) * 4));
