@@ -762,6 +762,47 @@ def test_stop_request_records_earlier_observables(gpu, oracle_lib, at):
     assert np.linalg.norm(r["state"] - ro["state"]) <= r["err"] + ro["err"] + 1e-10
 
 
+@pytest.mark.parametrize("bytes_form", [True, False])
+def test_tiled_combine_many_samples(gpu, oracle_lib, monkeypatch, bytes_form):
+    """Steps that cover more samples than one pass of combine_kernel takes (9 diagonal
+    observables: 2 samples a pass) go through combine_tiled_kernel, 8 samples a pass. Against
+    passes of combine_kernel only (TE_COMBINE_TILED=0): the same states, errors and step counts
+    bit for bit (each sample's state is the same column-by-column sum), the observables to 1e-13
+    (the same terms added in another order), and both within tol of the oracle; with the
+    observables held as byte tables and as 8-byte diagonals (TE_DIAG_BYTES=0)."""
+    L = N = 8
+    ob = oracle_lib.Basis.basis(N, L)
+    Hc = oracle_lib.build_operator(ob, as_tuples(_ring_terms(L)))
+    rng = np.random.default_rng(77)
+    d = rng.normal(size=ob.D)                                    # 6435 distinct values: 8 bytes
+    d = d if not bytes_form else np.round(d * 4) / 4             # a few dozen values: one byte
+    idx = np.arange(ob.D, dtype=np.int64)
+    dcoo = (d.astype(np.complex128), idx, idx)
+    nks = [oracle_lib.build_operator(ob, as_tuples([createNumberOperator(k, 1)])) for k in range(L)]
+    psi = random_state(ob.D, 43)
+    T, step = 2.0, 0.02                                          # many samples a Krylov step
+    res = {}
+    for tiled in ("1", "0"):
+        monkeypatch.setenv("TE_COMBINE_TILED", tiled)
+        gb = capi.Basis.basis(gpu, N, L)
+        H = capi.operator_build(gpu, gb, as_tuples(_ring_terms(L)))
+        obs = [capi.operator_build(gpu, gb, [createNumberOperator(k, 1)]) for k in range(L)]
+        obs.append(capi.Matrix.from_coo(gpu, *dcoo, ob.D))
+        res[tiled] = capi.Evolver(gpu, H, psi, T, step, obs, tol=1e-8).run()
+    monkeypatch.delenv("TE_COMBINE_TILED", raising=False)
+    a, b = res["1"], res["0"]
+    print("steps %d samples %d max|dobs| tiled vs passes %.3e" %
+          (a["n_steps"], a["numSamples"], np.abs(a["observables"] - b["observables"]).max()))
+    assert a["numSamples"] > 4 * a["n_steps"]                    # steps with more than 2 samples
+    assert a["n_steps"] == b["n_steps"] and a["statusCode"] == b["statusCode"]
+    assert np.float64(a["err"]).view(np.uint64) == np.float64(b["err"]).view(np.uint64)
+    assert np.array_equal(a["state"].view(np.uint64), b["state"].view(np.uint64))
+    assert np.abs(a["observables"] - b["observables"]).max() <= 1e-13
+    ro = oracle_lib.evolve(ob.D, Hc, psi, T, step, [("sparse", o) for o in nks] + [("sparse", dcoo)], tol=1e-8)
+    assert np.abs(a["observables"] - ro["observables"]).max() <= 1e-8
+    assert np.linalg.norm(a["state"] - ro["state"]) <= a["err"] + ro["err"] + 1e-10
+
+
 @pytest.mark.parametrize("n_distinct", [2, 256, 257, 5000])
 def test_diagonal_observables_byte_tables(gpu, oracle_lib, monkeypatch, n_distinct):
     """Diagonal observables with at most 256 distinct values are read by the combine as one-byte

@@ -1043,7 +1043,9 @@ static void combine_step(te_evolver* ev, int m, long long base, int S, bool writ
         const int no = std::min(nd - o0, combine_obs_per_launch());
         int sc = combine_samples_per_launch(no);
         // more samples than one pass of combine_kernel takes: the tiled pass, when it applies
-        const int tsc = combine_tiled_samples(no);
+        // (TE_COMBINE_TILED=0: passes of combine_kernel only; read at every step)
+        const char* te = std::getenv("TE_COMBINE_TILED");
+        const int tsc = (te && te[0] == '0') ? 0 : combine_tiled_samples(no);
         const bool tiled = S > sc && tsc > sc;
         if (tiled) sc = tsc;
         for (int s0 = 0; s0 < S; s0 += sc) L.push_back({base + s0, std::min(sc, S - s0), o0, no, tiled});
