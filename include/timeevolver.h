@@ -208,7 +208,8 @@ typedef struct te_batch te_batch;
  * plain method of te_evolver_create/te_evolver_run (same Lanczos recurrence, step control,
  * sampling and status rules) with its own Hamiltonian H[i] (square, rows of at most 257
  * entries), initial state psi0[i] (H[i] dimension complex128, host memory, copied) and n_obs
- * observables obs[i * n_obs + o] (diagonal TE_OBS_SPARSE matrices only; n_obs <= 4). One
+ * observables obs[i * n_obs + o] (diagonal TE_OBS_SPARSE matrices only; n_obs <= 64, read 4 a pass
+ * over each instance's Krylov basis). One
  * parameter set for all. Validation per instance as the constructor: TE_ERR_FATAL names the
  * instance whose state is not normalised (:45-48). Matrices are borrowed (must outlive the
  * batch). Instances given the same matrix share its passes: four (or two) of them are served by

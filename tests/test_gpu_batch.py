@@ -139,6 +139,37 @@ def test_batch_errors(gpu):
     assert e.value.code == capi.TE_ERR_UNSUPPORTED
 
 
+def test_batch_many_observables(gpu, oracle_lib):
+    """More diagonal observables than one combine pass takes (4): 9 per instance go in passes of
+    4, 4 and 1 over each instance's basis, the last writing the new start vector and the sampled
+    state. Per instance the observables match the plain GPU path within tol with the same steps
+    and status, one instance matches the oracle, and the first four observables and the states are
+    bit for bit those of the same batch with only those four."""
+    specs = _seeded_specs(606, 6, (6, 7, 8))
+    modes = tuple(range(6))
+    inst = _instances(gpu, specs, modes)
+    extra = []                                          # three more diagonals: n_0 + n_1, 2 n_2, 1.5 n_3
+    for b, H, obs, terms, psi in inst:
+        extra.append([capi.operator_build(gpu, b, as_tuples([createNumberOperator(0, 1), createNumberOperator(1, 1)])),
+                      capi.operator_build(gpu, b, as_tuples([createNumberOperator(2, 2)])),
+                      capi.operator_build(gpu, b, as_tuples([createNumberOperator(3, 1.5)]))])
+    T, step, tol, m = 1.5, 0.25, 1e-8, 30
+    mats, psis = [x[1] for x in inst], [x[4] for x in inst]
+    obs9 = [x[2] + e for x, e in zip(inst, extra)]
+    r9 = capi.BatchEvolver(gpu, mats, psis, T, step, obs9, tol=tol, m=m).run()
+    r4 = capi.BatchEvolver(gpu, mats, psis, T, step, [x[2][:4] for x in inst], tol=tol, m=m).run()
+    for i, (a, c) in enumerate(zip(r9, r4)):
+        assert a["observables"].shape[0] == 9
+        assert np.array_equal(a["observables"][:4].view(np.uint64), c["observables"].view(np.uint64))
+        assert np.array_equal(a["state"].view(np.uint64), c["state"].view(np.uint64))
+        p = capi.Evolver(gpu, mats[i], psis[i], T, step, obs9[i], tol=tol, m=m).run()
+        assert a["n_steps"] == p["n_steps"] and a["statusCode"] == p["statusCode"]
+        assert np.abs(a["observables"] - p["observables"]).max() <= tol
+    L, N, *_ = specs[2]
+    ro = _oracle_run(oracle_lib, N, L, inst[2][3], psis[2], T, step, modes, tol, m)
+    _check(dict(r9[2], observables=r9[2]["observables"][:6]), ro, tol)
+
+
 def test_batch_sell_and_csr_paths(gpu, monkeypatch):
     """The sliced-ELL batch (instances in their locality order) and the CSR row-block batch
     (TE_BATCH_SELL=0) agree with each other and with the plain evolver."""

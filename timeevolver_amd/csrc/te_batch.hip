@@ -60,7 +60,8 @@ constexpr int kBatchChunk = 8 * 64;
 constexpr int kBatchGroup = 4;
 constexpr int kBatchNV = 4;          // instances of one shared matrix served by one pass over its words
 constexpr int kBatchSC = 8;          // samples per pass of the combine kernel
-constexpr int kBatchMaxObs = 4;      // diagonal observables per instance
+constexpr int kBatchObsPass = 4;     // diagonal observables per combine pass over U (registers)
+constexpr int kBatchMaxObs = 64;     // diagonal observables per instance
 constexpr int kBatchTraceCap = 4096; // recorded step sizes per instance
 
 struct BatchInst {
@@ -408,9 +409,11 @@ __global__ __launch_bounds__(kBlock) void batch_finish_kernel(BatchDev a, int j)
 // kept as the sampled state, and per-chunk partial sums of the diagonal observables
 // sum_r Re(O_rr) |psi_s(r)|^2 (a wavefront per kBatchChunk-row chunk, lane l: rows l, l + 64,
 // ...). Samples go kBatchSC per pass over U; the pass that covers the last sample is the one
-// that overwrites column 0, after every pass has read it.
+// that overwrites column 0, after every pass has read it. Observables go NOB (at most
+// kBatchObsPass) a launch, from obs0 on; only the launch of the last group (last_group) writes
+// column 0 and the sampled state, so every launch reads the step's own basis.
 template <int NOB>
-__global__ __launch_bounds__(kBlock) void batch_combine_kernel(BatchDev a) {
+__global__ __launch_bounds__(kBlock) void batch_combine_kernel(BatchDev a, int obs0, int last_group) {
     constexpr int kBatchBurst = 8;
     __shared__ double2 cf[(1 + kBatchSC) * kMaxKrylov];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -433,7 +436,7 @@ __global__ __launch_bounds__(kBlock) void batch_combine_kernel(BatchDev a) {
         for (int c0 = 0; c0 < S || (c0 == 0 && write_w); c0 += kBatchSC) {
             const int sc = min(kBatchSC, S - c0);      // samples of this pass (may be <= 0)
             const bool lastpass = c0 + kBatchSC >= S;
-            const bool ww = lastpass && write_w;
+            const bool ww = lastpass && write_w && last_group;
             for (int q = threadIdx.x; q < (1 + kBatchSC) * kMaxKrylov; q += kBlock) {
                 const int c = q / kMaxKrylov, jj = q % kMaxKrylov;
                 double2 v = make_double2(0.0, 0.0);
@@ -487,7 +490,7 @@ __global__ __launch_bounds__(kBlock) void batch_combine_kernel(BatchDev a) {
                     }
                     for (; jj < m; jj++) add(jj, load_stream(&a.U[(size_t)jj * a.ld + gr]));
                     if (ww) a.U[gr] = aw;
-                    if (lastpass && sc > 0) {
+                    if (last_group && lastpass && sc > 0) {
                         double2 v = as[0];
 #pragma unroll
                         for (int s = 1; s < kBatchSC; s++)
@@ -496,21 +499,23 @@ __global__ __launch_bounds__(kBlock) void batch_combine_kernel(BatchDev a) {
                     }
                     double dg[NOB > 0 ? NOB : 1];              // each observable's diagonal read once a row
 #pragma unroll
-                    for (int o = 0; o < NOB; o++) dg[o] = o < a.nobs ? a.diag[(size_t)o * a.ld + gr] : 0.0;
+                    for (int o = 0; o < NOB; o++)
+                        dg[o] = obs0 + o < a.nobs ? a.diag[(size_t)(obs0 + o) * a.ld + gr] : 0.0;
 #pragma unroll
                     for (int s = 0; s < kBatchSC; s++) {
                         const double pr = as[s].x * as[s].x + as[s].y * as[s].y;
 #pragma unroll
                         for (int o = 0; o < NOB; o++)
-                            if (o < a.nobs && s < sc) acc[o * kBatchSC + s] += dg[o] * pr;
+                            if (obs0 + o < a.nobs && s < sc) acc[o * kBatchSC + s] += dg[o] * pr;
                     }
                 }
 #pragma unroll
                 for (int q = 0; q < NOB * kBatchSC; q++) {
                     const int o = q / kBatchSC, s = q % kBatchSC;
-                    if (o < a.nobs && s < sc) {          // uniform in the wavefront
+                    if (obs0 + o < a.nobs && s < sc) {   // uniform in the wavefront
                         const double t = wave_sum(acc[q]);
-                        if (lane == 0) a.opart[((size_t)(I.ch0 + ch) * a.nobs + o) * a.n_samples + y0 + c0 + s] = t;
+                        if (lane == 0)
+                            a.opart[((size_t)(I.ch0 + ch) * a.nobs + obs0 + o) * a.n_samples + y0 + c0 + s] = t;
                     }
                 }
             }
@@ -803,10 +808,15 @@ static void batch_round(te_batch* b, hipStream_t s) {
 static void batch_combine(te_batch* b, hipStream_t s) {
     const BatchDev& d = b->dev;
     prof_begin(b, 3, s);
-    if (b->nobs == 0) hipLaunchKernelGGL(batch_combine_kernel<0>, dim3(b->grid_v), dim3(kBlock), 0, s, d);
-    else if (b->nobs == 1) hipLaunchKernelGGL(batch_combine_kernel<1>, dim3(b->grid_v), dim3(kBlock), 0, s, d);
-    else if (b->nobs == 2) hipLaunchKernelGGL(batch_combine_kernel<2>, dim3(b->grid_v), dim3(kBlock), 0, s, d);
-    else hipLaunchKernelGGL(batch_combine_kernel<4>, dim3(b->grid_v), dim3(kBlock), 0, s, d);
+    // one launch per group of kBatchObsPass observables (one pass over each stepped instance's
+    // basis each); the last writes the new column 0 and the sampled states
+    if (b->nobs == 0) hipLaunchKernelGGL(batch_combine_kernel<0>, dim3(b->grid_v), dim3(kBlock), 0, s, d, 0, 1);
+    for (int o0 = 0; o0 < b->nobs; o0 += kBatchObsPass) {
+        const int no = std::min(kBatchObsPass, b->nobs - o0), last = o0 + kBatchObsPass >= b->nobs ? 1 : 0;
+        if (no == 1) hipLaunchKernelGGL(batch_combine_kernel<1>, dim3(b->grid_v), dim3(kBlock), 0, s, d, o0, last);
+        else if (no == 2) hipLaunchKernelGGL(batch_combine_kernel<2>, dim3(b->grid_v), dim3(kBlock), 0, s, d, o0, last);
+        else hipLaunchKernelGGL(batch_combine_kernel<4>, dim3(b->grid_v), dim3(kBlock), 0, s, d, o0, last);
+    }
     hipLaunchKernelGGL(batch_obs_finish_kernel, dim3(b->n_inst), dim3(kBlock), 0, s, d);
     hipLaunchKernelGGL(batch_count_kernel, dim3(1), dim3(kBlock), 0, s, d);
     prof_end(b, 3, s);
