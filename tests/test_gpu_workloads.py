@@ -8,6 +8,7 @@
     the bound the returned error implies, 2 ||n_k|| err = 2 N err (|<psi|O|psi> - <phi|O|phi>|
     <= 2 ||O|| ||psi - phi|| for unit vectors);
   - sum_k <n_k> = N and <psi|psi> = 1 at every sample (an identity observable), within 1e-9;
+  - the same two checks at L = N = 15 (D = 77,558,760), the largest ring one GPU holds;
   - U = 0.5 (the timed workload, seed 2024) over a short horizon (T = 0.5, sampling step 0.25)
     against the oracle on the CSR the device built: ||psi_gpu - psi_oracle|| <= err_gpu +
     err_oracle + 1e-10, every observable within tol, the same step count and status.
@@ -51,13 +52,14 @@ def ring14(gpu):
         o.close()
 
 
-def test_ring14_free_bosons_closed_form(gpu, ring14):
-    L, N, b, obs = ring14
+def _free_bosons_closed_form(gpu, L, N, b, obs):
+    """U = 0 from |N,0,...,0> to T = 10 (sampling step 0.5, m = 40, tol = 1e-8): every sample of
+    every n_k against the single-particle closed form, sum_k n_k = N and <psi|psi> = 1."""
     D = b.D
     eps, J = ring_couplings(L, SEED)
     terms = ring_bose_hubbard_terms(L, eps, J, 0.0)[:3 * L]     # number terms and hops, no U term
     H = capi.operator_build(gpu, b, as_tuples(terms))
-    assert H.nnz == ring_nnz(L, N) == 311_275_100
+    assert H.nnz == ring_nnz(L, N)
     idx = np.arange(D, dtype=np.uint64)
     ident = capi.Matrix.from_coo(gpu, np.ones(D, dtype=np.complex128), idx, idx, D)
     del idx
@@ -74,14 +76,20 @@ def test_ring14_free_bosons_closed_form(gpu, ring14):
     nk = r["observables"][:L]
     dev = np.abs(nk - ref).max()
     norm2 = r["observables"][L]
-    print("L=N=14 free bosons: steps %d status %d err %.3e max|n_k - closed form| %.3e max|sum n_k - N| %.3e "
-          "max|<psi|psi> - 1| %.3e" % (r["n_steps"], r["statusCode"], r["err"], dev,
+    print("L=N=%d free bosons: D %d steps %d status %d err %.3e max|n_k - closed form| %.3e max|sum n_k - N| %.3e "
+          "max|<psi|psi> - 1| %.3e" % (L, D, r["n_steps"], r["statusCode"], r["err"], dev,
                                         np.abs(nk.sum(0) - N).max(), np.abs(norm2 - 1).max()))
     assert r["numSamples"] == r["n_samples"] == 21
     assert r["err"] <= tol
     assert dev <= 2 * N * r["err"] + 1e-12
     assert np.abs(nk.sum(0) - N).max() <= N * 1e-9
     assert np.abs(norm2 - 1).max() <= 1e-9
+
+
+def test_ring14_free_bosons_closed_form(gpu, ring14):
+    L, N, b, obs = ring14
+    assert ring_nnz(L, N) == 311_275_100
+    _free_bosons_closed_form(gpu, L, N, b, obs)
 
 
 def test_ring14_bose_hubbard_short_horizon_vs_oracle(gpu, oracle_lib, ring14):
@@ -110,6 +118,23 @@ def test_ring14_bose_hubbard_short_horizon_vs_oracle(gpu, oracle_lib, ring14):
     assert rg["numSamples"] == ro["numSamples"] == 3
     assert dpsi <= rg["err"] + ro["err"] + 1e-10
     assert dobs <= tol
+
+
+def test_ring15_free_bosons_closed_form(gpu):
+    """The largest ring that fits one GPU: L = N = 15, D = 77,558,760, nnz = 1,281,056,760 (below
+    2^31, the limit of 32-bit entry offsets; about 80 GB of HBM with the Krylov basis), the same
+    closed-form and invariant checks as at L = N = 14."""
+    L = N = 15
+    b = capi.Basis.basis(gpu, N, L)
+    assert b.D == ring_dimension(L, N) == 77_558_760
+    assert ring_nnz(L, N) == 1_281_056_760
+    obs = [capi.operator_build(gpu, b, as_tuples([createNumberOperator(k, 1)])) for k in range(L)]
+    try:
+        _free_bosons_closed_form(gpu, L, N, b, obs)
+    finally:
+        for o in obs:
+            o.close()
+        b.close()
 
 
 # ---------------------------------------------------------------------------------------- 512 instances
