@@ -323,3 +323,48 @@ def test_batch_shared_matrix_groups(gpu, oracle_lib, monkeypatch):
     Lo, No, *_ = _seeded_specs(55, 2, (6, 7))[1]
     ro = _oracle_run(oracle_lib, No, Lo, others[1][3], psis[8], T, step, (0,), tol, m)
     _check(rg[8], ro, tol)
+
+
+def test_batch_blackhole_golden_and_rings(gpu):
+    """Model families and couplings of uneven size in one batch: the reference's black-hole
+    instance (tensor basis of 588 states, FourPointA and spherical terms, example/
+    exampleHamiltonian.cpp:27-138), the same model with other couplings (C0 = 0.5, Cm = 0.7), and
+    a ring L = N = 10 (92,378 states); 10 occupation observables each, T = 10, sampling step 0.01
+    (1001 samples), tol = 1e-8, m = 40. The first instance against the reference's stored values
+    (output/ResultBlackHole_N20_Nm2_K4_..._tol1e-08_samplingStep0.01_m40_fastIntegration0.h5) within
+    the reference test's 1e-8 (test/testBlackHole.cpp:21-45); every instance against the plain
+    evolver: same steps, status and sample count, observables within tol, states within the sum
+    of the error bounds."""
+    import os
+
+    from models import blackhole, number_operator_terms
+    T, step, tol, m = 10.0, 0.01, 1e-8, 40
+    inst = []
+    for C0, Cm in ((1.0, 1.0), (0.5, 0.7)):
+        args, ham, init = blackhole(N0=20, Nm=2, K=4, C0=C0, Cm=Cm)
+        b = capi.Basis.tensor(gpu, *args)
+        H = capi.operator_build(gpu, b, as_tuples(ham))
+        obs = [capi.operator_build(gpu, b, as_tuples(t)) for t in number_operator_terms(args[1] + args[3])]
+        psi = np.zeros(b.D, complex)
+        psi[b.find(np.asarray([init], dtype=np.int32))[0]] = 1
+        inst.append((b, H, obs, psi))
+    L = 10
+    b, H, obs, _, psi = _instances(gpu, _seeded_specs(4242, 1, (L,)), tuple(range(L)))[0]
+    inst.append((b, H, obs, psi))
+    assert [x[0].D for x in inst] == [588, 588, 92378]
+    rb = capi.BatchEvolver(gpu, [x[1] for x in inst], [x[3] for x in inst], T, step, [x[2] for x in inst],
+                           tol=tol, m=m).run()
+    gold = np.loadtxt(os.path.join(os.path.dirname(__file__), "golden", "blackhole_N20_Nm2_K4.txt"))
+    dg = np.abs(rb[0]["observables"] - gold).max()
+    print("batched black hole vs reference output: max |diff| %.3e" % dg)
+    assert dg <= 1e-8
+    for (b, H, obs, psi), r in zip(inst, rb):
+        rp = capi.Evolver(gpu, H, psi, T, step, obs, tol=tol, m=m).run()
+        dpsi = np.linalg.norm(r["state"] - rp["state"])
+        dobs = np.abs(r["observables"] - rp["observables"]).max()
+        print("D=%d steps batch/plain %d/%d status %d/%d |dpsi| %.3e max|dobs| %.3e" %
+              (r["dim"], r["n_steps"], rp["n_steps"], r["statusCode"], rp["statusCode"], dpsi, dobs))
+        assert r["n_steps"] == rp["n_steps"] and r["statusCode"] == rp["statusCode"]
+        assert r["numSamples"] == rp["numSamples"] == 1001
+        assert dpsi <= r["err"] + rp["err"] + 1e-10
+        assert dobs <= tol
