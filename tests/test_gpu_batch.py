@@ -368,3 +368,25 @@ def test_batch_blackhole_golden_and_rings(gpu):
         assert r["numSamples"] == rp["numSamples"] == 1001
         assert dpsi <= r["err"] + rp["err"] + 1e-10
         assert dobs <= tol
+
+
+def test_batch_exp_factor(gpu):
+    """One parameter set with expFactor = -0.7 (exp(+0.7 i H t); core/krylovTimeEvolver.cpp:31)
+    for three uneven instances: each against the plain evolver with the same factor (same steps,
+    status and samples, states within the sum of the error bounds, observables within tol)."""
+    specs = _seeded_specs(77, 1, (5,)) + _seeded_specs(78, 1, (6,)) + _seeded_specs(79, 1, (7,))
+    inst = _instances(gpu, specs)
+    assert [x[0].D for x in inst] == [126, 462, 1716]
+    T, step, tol, m, f = 2.0, 0.25, 1e-8, 30, -0.7
+    rb = capi.BatchEvolver(gpu, [x[1] for x in inst], [x[4] for x in inst], T, step, [x[2] for x in inst],
+                           exp_factor=f, tol=tol, m=m).run()
+    for (b, H, obs, terms, psi), r in zip(inst, rb):
+        rp = capi.Evolver(gpu, H, psi, T, step, obs, exp_factor=f, tol=tol, m=m).run()
+        dpsi = np.linalg.norm(r["state"] - rp["state"])
+        dobs = np.abs(r["observables"] - rp["observables"]).max()
+        print("D=%d steps batch/plain %d/%d status %d/%d |dpsi| %.3e max|dobs| %.3e" %
+              (r["dim"], r["n_steps"], rp["n_steps"], r["statusCode"], rp["statusCode"], dpsi, dobs))
+        assert r["n_steps"] == rp["n_steps"] and r["statusCode"] == rp["statusCode"]
+        assert r["numSamples"] == rp["numSamples"]
+        assert dpsi <= r["err"] + rp["err"] + 1e-10
+        assert dobs <= tol

@@ -562,6 +562,8 @@ def _ring_pair(gpu, oracle, L, N, psi_fn, T, step, m=40, tol=1e-8, **kw):
     ro = oracle.evolve(ob.D, H, psi, T, step, oobs, tol=tol, m=m, **kw)
     b, Hg, gobs = _gpu_model(gpu, "basis", (N, L), terms, [[createNumberOperator(k, 1)] for k in range(L)])
     gkw = {"fast_integration": kw["fastIntegration"]} if "fastIntegration" in kw else {}
+    if "expFactor" in kw:
+        gkw["exp_factor"] = kw["expFactor"]
     rg = capi.Evolver(gpu, Hg, psi, T, step, gobs, tol=tol, m=m, **gkw).run()
     return ro, rg
 
@@ -639,6 +641,30 @@ def test_ring_medium_config_vs_oracle(gpu, oracle_lib):
 def test_ring_random_state_vs_oracle(gpu, oracle_lib):
     ro, rg = _ring_pair(gpu, oracle_lib, 6, 6, lambda D: random_state(D, 9), 3.0, 0.05, m=30)
     _check_pair(ro, rg)
+
+
+@pytest.mark.parametrize("f", [-1.0, 0.5, 2.5])
+def test_exp_factor_vs_oracle(gpu, oracle_lib, f):
+    """exp(-i f H t), the constructor's expFactor (core/krylovTimeEvolver.cpp:31, applied in every
+    Lanczos product at :530): L = N = 6 ring, random complex state, T = 3, step 0.1, m = 40,
+    against the oracle with the same factor (the same bars as every other evolution)."""
+    ro, rg = _ring_pair(gpu, oracle_lib, 6, 6, lambda D: random_state(D, 21), 3.0, 0.1, expFactor=f)
+    _check_pair(ro, rg)
+
+
+def test_exp_factor_round_trip(gpu):
+    """exp(+iHT) exp(-iHT) psi0 = psi0: the L = N = 8 ring from a random state forward to T = 2
+    (expFactor 1) and back (expFactor -1), within the sum of the two returned error bounds."""
+    L = N = 8
+    b, H, _ = _gpu_model(gpu, "basis", (N, L), as_tuples(_ring_terms(L)), [])
+    psi0 = random_state(b.D, 5)
+    fw = capi.Evolver(gpu, H, psi0, 2.0, 2.0, [], tol=1e-8).run()
+    bw = capi.Evolver(gpu, H, fw["state"], 2.0, 2.0, [], exp_factor=-1.0, tol=1e-8).run()
+    d = np.linalg.norm(bw["state"] - psi0)
+    print("round trip: |psi_back - psi0| %.3e, err %.3e + %.3e, |psi(T) - psi0| %.3e" %
+          (d, fw["err"], bw["err"], np.linalg.norm(fw["state"] - psi0)))
+    assert np.linalg.norm(fw["state"] - psi0) > 0.1          # the state did move
+    assert d <= fw["err"] + bw["err"] + 1e-10
 
 
 def test_complex_hermitian_ring_vs_oracle(gpu, oracle_lib):
