@@ -125,7 +125,8 @@ typedef struct {
     double sampling_step;  /* :26 samplingStep */
     double exp_factor;     /* :26 expFactor (default 1.0): evolves exp(-i expFactor H t) */
     double tol;            /* :26 tol (default 1e-6) */
-    int m;                 /* :26 mm (default 40; capped at 64 and at the dimension) */
+    int m;                 /* :26 mm (default 40); the run uses min(m, dimension) (:35), which must
+                              be at most 64 (else TE_ERR_UNSUPPORTED) */
     int fast_integration;  /* :26 fastIntegration (default 0) */
 } te_evolve_params;
 

@@ -652,6 +652,30 @@ def test_exp_factor_vs_oracle(gpu, oracle_lib, f):
     _check_pair(ro, rg)
 
 
+@pytest.mark.parametrize("m,tol", [(4, 1e-6), (10, 1e-10), (64, 1e-8)])
+def test_krylov_dimension_and_tol_vs_oracle(gpu, oracle_lib, m, tol):
+    """The constructor's mm and tol (core/krylovTimeEvolver.cpp:26, :35): a short basis at the
+    default tol (many Krylov steps), a tight tol, and the largest Krylov dimension the device
+    takes, on the L = N = 6 ring from a random state to T = 3 (step 0.1) against the oracle."""
+    ro, rg = _ring_pair(gpu, oracle_lib, 6, 6, lambda D: random_state(D, 23), 3.0, 0.1, m=m, tol=tol)
+    _check_pair(ro, rg, tol=tol)
+    assert rg["krylovDim"] == ro["krylovDim"] == m
+
+
+def test_krylov_dimension_limit(gpu):
+    """min(mm, dimension) above 64 is refused with TE_ERR_UNSUPPORTED; mm = 65 on a dimension-10
+    basis runs with m = 10 (:35)."""
+    L = N = 6
+    b, H, _ = _gpu_model(gpu, "basis", (N, L), as_tuples(_ring_terms(L)), [])
+    with pytest.raises(capi.TEError) as e:
+        capi.Evolver(gpu, H, random_state(b.D, 1), 1.0, 0.5, [], tol=1e-8, m=65)
+    assert e.value.code == capi.TE_ERR_UNSUPPORTED and "65" in str(e.value)
+    b3, H3, _ = _gpu_model(gpu, "basis", (3, 3), as_tuples(_ring_terms(3)), [])
+    assert b3.D == 10
+    r = capi.Evolver(gpu, H3, random_state(b3.D, 1), 1.0, 0.5, [], tol=1e-8, m=65).run()
+    assert r["krylovDim"] == 10 and r["err"] <= 1e-8
+
+
 def test_exp_factor_round_trip(gpu):
     """exp(+iHT) exp(-iHT) psi0 = psi0: the L = N = 8 ring from a random state forward to T = 2
     (expFactor 1) and back (expFactor -1), within the sum of the two returned error bounds."""
