@@ -676,6 +676,25 @@ def test_krylov_dimension_limit(gpu):
     assert r["krylovDim"] == 10 and r["err"] <= 1e-8
 
 
+@pytest.mark.parametrize("T,step,ns", [(1.0, 0.3, 4), (0.95, 0.1, 10), (1.0, 2.0, 1)])
+def test_sampling_grid_vs_oracle(gpu, oracle_lib, T, step, ns):
+    """Sampling steps that do not divide t, and one longer than t: n_samples = floor(t/step) + 1
+    (core/krylovTimeEvolver.cpp:57), samples taken where the reference takes them (:405-437), and
+    the loop ends once every sample is taken (:332), so with one sample (t = 0) no Krylov step
+    runs and the returned state is psi0 at evolvedTime 0; L = N = 5 ring from a random state,
+    against the oracle."""
+    psi0 = lambda D: random_state(D, 31)
+    ro, rg = _ring_pair(gpu, oracle_lib, 5, 5, psi0, T, step)
+    _check_pair(ro, rg)
+    assert rg["n_samples"] == ro["n_samples"] == ns
+    assert rg["n_steps"] == ro["n_steps"] and rg["evolvedTime"] == ro["evolvedTime"]
+    if ns == 1:
+        assert rg["n_steps"] == 0 and rg["evolvedTime"] == 0.0
+        assert np.array_equal(rg["state"], psi0(len(rg["state"])))
+    else:
+        assert rg["evolvedTime"] >= (ns - 1) * step
+
+
 def test_exp_factor_round_trip(gpu):
     """exp(+iHT) exp(-iHT) psi0 = psi0: the L = N = 8 ring from a random state forward to T = 2
     (expFactor 1) and back (expFactor -1), within the sum of the two returned error bounds."""
