@@ -695,6 +695,35 @@ def test_sampling_grid_vs_oracle(gpu, oracle_lib, T, step, ns):
         assert rg["evolvedTime"] >= (ns - 1) * step
 
 
+def test_diagonal_hamiltonian_closed_form(gpu, oracle_lib):
+    """No hopping (J = 0, the on-site and interaction terms only): H is diagonal, its sliced-ELL
+    copy has no off-diagonal slot and no shared record. psi(T) = exp(-i d T) psi0 exactly and
+    every n_k is constant in time; L = N = 6 from a random state to T = 3 (step 0.1), against
+    that closed form within the returned error bound and against the oracle."""
+    L = N = 6
+    eps, J = ring_couplings(L, 11)
+    terms = as_tuples(ring_bose_hubbard_terms(L, eps, np.zeros(L), 0.5))
+    ob = oracle_lib.Basis.basis(N, L)
+    Hc = oracle_lib.build_operator(ob, terms)
+    assert np.array_equal(Hc[1], Hc[2])                      # diagonal only
+    oobs = [("sparse", oracle_lib.build_operator(ob, as_tuples([createNumberOperator(k, 1)]))) for k in range(L)]
+    psi0 = random_state(ob.D, 41)
+    ro = oracle_lib.evolve(ob.D, Hc, psi0, 3.0, 0.1, oobs, tol=1e-8, m=40)
+    b, H, gobs = _gpu_model(gpu, "basis", (N, L), terms, [[createNumberOperator(k, 1)] for k in range(L)])
+    assert H.nnz == ob.D
+    rg = capi.Evolver(gpu, H, psi0, 3.0, 0.1, gobs, tol=1e-8, m=40).run()
+    _check_pair(ro, rg)
+    d = np.zeros(ob.D)
+    d[Hc[2].astype(np.int64)] = Hc[0].real
+    exact = np.exp(-1j * d * 3.0) * psi0
+    occ = ob.states().astype(float)
+    n0 = (np.abs(psi0) ** 2) @ occ                           # <n_k> at t = 0, for all t
+    print("diagonal H: |psi - exact| %.3e err %.3e max|n_k(t) - n_k(0)| %.3e" %
+          (np.linalg.norm(rg["state"] - exact), rg["err"], np.abs(rg["observables"] - n0[:, None]).max()))
+    assert np.linalg.norm(rg["state"] - exact) <= rg["err"] + 1e-12
+    assert np.abs(rg["observables"] - n0[:, None]).max() <= 1e-12
+
+
 def test_exp_factor_round_trip(gpu):
     """exp(+iHT) exp(-iHT) psi0 = psi0: the L = N = 8 ring from a random state forward to T = 2
     (expFactor 1) and back (expFactor -1), within the sum of the two returned error bounds."""
